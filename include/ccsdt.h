@@ -1,0 +1,220 @@
+/*
+ * ccsdt.h -- C ABI of libccsdt.so, the MI355X (gfx950) CCSD(T) triples body.
+ *
+ * Two kinds of entry point:
+ *
+ *  1. Drop-ins with the reference's exact argument lists, column-major layouts
+ *     and 1-based index conventions.  They are synchronous: the results are in
+ *     the caller's host arrays on return.  Like the reference they return void;
+ *     failures are reported through ccsdt_last_error() and, under the default
+ *     policy, abort the process with a message (see ccsdt_set_error_policy).
+ *
+ *  2. An explicit-residency job API: the caller uploads a job's arrays once,
+ *     then runs lists of (a,i,j,k,klo) triples on the device.  These return a
+ *     status (0 = success, negative = CCSDT_E_*).
+ *
+ * All array arguments are double precision (IEEE binary64), all index and size
+ * arguments 32-bit int, as in the reference (LP64 Fortran INTEGER).
+ */
+#ifndef CCSDT_H
+#define CCSDT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- drop-ins */
+
+/* One (a,i,j,k) triple: the 16 DGEMMs into f1n..f4t (overwritten; each
+ * nvir*nvir, column major), eaijk, the energy reduction, and
+ *   *emp4 += emp4i; *emp5 += emp5i; if (*i != *k) { *emp4 += emp4k; *emp5 += emp5k; }
+ * Replaces ccsd_trpdrv_omp_cbody_, omp/ccsd_trpdrv_omp_cbody.c:31-44 (caller
+ * prototype omp/test.c:39-52).  k-indexed operands (tkj, kkj, jkj, jka, kka,
+ * tka, xka) are read at slice (k-klo); tia, xia, jij, kij are read over their
+ * first nocc*nvir elements; dint and t1v vectors over nvir elements. */
+void ccsd_trpdrv_omp_cbody_(double *f1n, double *f1t, double *f2n, double *f2t,
+                            double *f3n, double *f3t, double *f4n, double *f4t,
+                            double *eorb, int *ncor, int *nocc, int *nvir,
+                            double *emp4, double *emp5,
+                            int *a, int *i, int *j, int *k, int *klo,
+                            double *tij, double *tkj, double *tia, double *tka,
+                            double *xia, double *xka, double *jia, double *jka,
+                            double *kia, double *kka, double *jij, double *jkj,
+                            double *kij, double *kkj,
+                            double *dintc1, double *dintx1, double *t1v1,
+                            double *dintc2, double *dintx2, double *t1v2);
+
+/* Same contract under the name of the OpenCL variant, ocl/ccsd_trpdrv_cbody.c:31. */
+void ccsd_trpdrv_cbody_(double *f1n, double *f1t, double *f2n, double *f2t,
+                        double *f3n, double *f3t, double *f4n, double *f4t,
+                        double *eorb, int *ncor, int *nocc, int *nvir,
+                        double *emp4, double *emp5,
+                        int *a, int *i, int *j, int *k, int *klo,
+                        double *tij, double *tkj, double *tia, double *tka,
+                        double *xia, double *xka, double *jia, double *jka,
+                        double *kia, double *kka, double *jij, double *jkj,
+                        double *kij, double *kkj,
+                        double *dintc1, double *dintx1, double *t1v1,
+                        double *dintc2, double *dintx2, double *t1v2);
+
+/* Same contract under the Fortran body's symbol: the argument list of
+ * ccsd_trpdrv_omp_fbody, omp/ccsd_trpdrv_omp_f77.f:1-9 (every argument by
+ * reference), bound by the USE_FORTRAN harness at omp/test.c:24-37. */
+void ccsd_trpdrv_omp_fbody_(double *f1n, double *f1t, double *f2n, double *f2t,
+                            double *f3n, double *f3t, double *f4n, double *f4t,
+                            double *eorb, int *ncor, int *nocc, int *nvir,
+                            double *emp4, double *emp5,
+                            int *a, int *i, int *j, int *k, int *klo,
+                            double *tij, double *tkj, double *tia, double *tka,
+                            double *xia, double *xka, double *jia, double *jka,
+                            double *kia, double *kka, double *jij, double *jkj,
+                            double *kij, double *kkj,
+                            double *dintc1, double *dintx1, double *t1v1,
+                            double *dintc2, double *dintx2, double *t1v2);
+
+/* Energy reduction over nvir*nvir (b,c) pairs of given f matrices; the four
+ * outputs are overwritten, not accumulated.  eaijk, ncor, nocc, nvir by value.
+ * Replaces ccsd_tengy_omp, omp/ccsd_tengy_c99.c:5-14. */
+void ccsd_tengy_omp(const double *f1n, const double *f1t,
+                    const double *f2n, const double *f2t,
+                    const double *f3n, const double *f3t,
+                    const double *f4n, const double *f4t,
+                    const double *dintc1, const double *dintx1, const double *t1v1,
+                    const double *dintc2, const double *dintx2, const double *t1v2,
+                    const double *eorb, const double eaijk,
+                    double *emp4i, double *emp5i, double *emp4k, double *emp5k,
+                    const int ncor, const int nocc, const int nvir);
+
+/* Same contract under the OpenCL variant's name, ocl/ccsd_tengy.c:3-12. */
+void ccsd_tengy(const double *f1n, const double *f1t,
+                const double *f2n, const double *f2t,
+                const double *f3n, const double *f3t,
+                const double *f4n, const double *f4t,
+                const double *dintc1, const double *dintx1, const double *t1v1,
+                const double *dintc2, const double *dintx2, const double *t1v2,
+                const double *eorb, const double eaijk,
+                double *emp4i, double *emp5i, double *emp4k, double *emp5k,
+                const int ncor, const int nocc, const int nvir);
+
+/* Fortran argument order and by-reference scalars: dintc1,dintc2,dintx1,
+ * dintx2,t1v1,t1v2 and eaijk,ncor,nocc,nvir by reference.  Replaces the
+ * Fortran ccsd_tengy_omp (symbol ccsd_tengy_omp_), omp/ccsd_tengy_f77.f:1-5. */
+void ccsd_tengy_omp_(const double *f1n, const double *f1t,
+                     const double *f2n, const double *f2t,
+                     const double *f3n, const double *f3t,
+                     const double *f4n, const double *f4t,
+                     const double *dintc1, const double *dintc2,
+                     const double *dintx1, const double *dintx2,
+                     const double *t1v1, const double *t1v2,
+                     const double *eorb, const double *eaijk,
+                     double *emp4i, double *emp5i, double *emp4k, double *emp5k,
+                     const int *ncor, const int *nocc, const int *nvir);
+
+/* ---------------------------------------------------------------- status */
+/* No counterpart in the reference, which has no error reporting
+ * (omp/ccsd_trpdrv_omp_cbody.c returns void and validates nothing). */
+enum {
+    CCSDT_OK = 0,
+    CCSDT_E_ARG = -1,       /* invalid argument (sizes, indices, null pointer) */
+    CCSDT_E_DEVICE = -2,    /* no usable gfx950 device or a HIP call failed */
+    CCSDT_E_NOMEM = -3,     /* device or pinned allocation failed */
+    CCSDT_E_BUDGET = -4,    /* serving admission: footprint above the budget */
+    CCSDT_E_STATE = -5,     /* call out of sequence (destroyed job, ...) */
+    CCSDT_E_INJECTED = -6   /* failure forced by ccsdt_debug_inject_failure */
+};
+int ccsdt_last_error(void);                /* last status of this thread */
+const char *ccsdt_last_error_string(void); /* message for it */
+void ccsdt_clear_error(void);
+/* policy 1 (default): drop-ins print the message and abort() on failure.
+ * policy 0: drop-ins record the error, set emp4 and emp5 (or the four tengy
+ * outputs) to NaN and return. */
+void ccsdt_set_error_policy(int abort_on_error);
+/* Host-side fault injection for failure-path tests: the n-th (1-based) device
+ * allocation or launch from now fails with CCSDT_E_INJECTED; 0 disables. */
+void ccsdt_debug_inject_failure(int nth);
+const char *ccsdt_version(void);
+/* Selects and initialises a device (default 0).  Returns CCSDT_OK or an error. */
+int ccsdt_init(int device);
+/* When 0, the drop-in body skips the copy of f1n..f4t back to the host (the
+ * reference leaves them filled; default 1).  Also set by CCSDT_F_WRITEBACK=0. */
+void ccsdt_set_f_writeback(int on);
+
+/* ---------------------------------------------------------------- inputs */
+/* Seeded synthetic inputs: PCG32 (XSH-RR, 64-bit state, 32-bit output; O'Neill
+ * 2014) on stream `stream`; doubles from 53 bits of two draws, scaled into
+ * [lo,hi).  No counterpart: replaces make_array, omp/test.c:55-62. */
+void ccsdt_pcg_raw(uint64_t seed, uint64_t stream, uint32_t *out, int64_t n);
+void ccsdt_pcg_uniform(uint64_t seed, uint64_t stream, double lo, double hi,
+                       double *out, int64_t n);
+
+/* ---------------------------------------------------------------- job API */
+/* Order of the arrays in ccsdt_job_desc.arrays: the body's argument order. */
+enum {
+    CCSDT_EORB = 0, CCSDT_TIJ, CCSDT_TKJ, CCSDT_TIA, CCSDT_TKA, CCSDT_XIA,
+    CCSDT_XKA, CCSDT_JIA, CCSDT_JKA, CCSDT_KIA, CCSDT_KKA, CCSDT_JIJ,
+    CCSDT_JKJ, CCSDT_KIJ, CCSDT_KKJ, CCSDT_DINTC1, CCSDT_DINTX1, CCSDT_T1V1,
+    CCSDT_DINTC2, CCSDT_DINTX2, CCSDT_T1V2, CCSDT_NARRAYS
+};
+
+/* A job holds the harness-shaped arrays of omp/test.c:112-146: eorb (nbf =
+ * ncor+nocc+nvir); tij, jia, kia (nvir^2); tkj, jka, kka (kchunk*nvir^2, slice
+ * k-klo); tia, xia, jij, kij (nocc*nvir); tka, xka, jkj, kkj (kchunk*nocc*nvir,
+ * slice k-klo); dintc1..t1v2 (nvir).  The arrays are copied to the device at
+ * creation; host buffers may be reused afterwards. */
+typedef struct {
+    int ncor, nocc, nvir, kchunk;
+    const double *arrays[CCSDT_NARRAYS];
+} ccsdt_job_desc;
+
+typedef struct ccsdt_job_s *ccsdt_job;
+
+/* Arithmetic of the K=nvir contractions. */
+enum {
+    CCSDT_MODE_FP64 = 0,        /* fused FP64 MFMA GEMM + energy epilogue */
+    CCSDT_MODE_FP64_SPLIT = 1,  /* FP64 GEMM writes f to HBM, separate tengy pass */
+    CCSDT_MODE_OZAKI_INT8 = 2,  /* Ozaki INT8 slices, INT32 accumulate, FP64 recombine */
+    CCSDT_MODE_BF16X3 = 3       /* three-term BF16 split, FP32 accumulate */
+};
+
+typedef struct {
+    int mode;     /* CCSDT_MODE_* */
+    int slices;   /* Ozaki slice count (3..8), ignored otherwise */
+    int flags;    /* reserved, 0 */
+    int reserved;
+} ccsdt_run_opts;
+
+typedef struct {
+    double emp4, emp5;          /* compensated sums over the list, i!=k rule applied */
+    double ms_total;            /* device time of the whole run (HIP events) */
+    double ms_pack;             /* operand packing */
+    double ms_main;             /* GEMM (+ fused energy) kernel */
+    double ms_tengy;            /* separate energy pass (MODE_FP64_SPLIT) */
+    double ms_reduce;           /* per-triple and per-list reductions */
+    int64_t ntriples;
+    int64_t launches;           /* kernel launches of the run */
+} ccsdt_run_result;
+
+int ccsdt_job_create(const ccsdt_job_desc *desc, ccsdt_job *out);
+/* Runs n triples (5 ints each: a,i,j,k,klo, 1-based as in the body).  terms
+ * (may be NULL) receives 4*n doubles: emp4i, emp5i, emp4k, emp5k per triple.
+ * fout (may be NULL, only for n == 1) receives the eight f matrices in the
+ * body's order f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t. */
+int ccsdt_job_run(ccsdt_job job, const int *triples, int64_t n,
+                  const ccsdt_run_opts *opts, ccsdt_run_result *res,
+                  double *terms, double *fout);
+int ccsdt_job_destroy(ccsdt_job job);
+/* Device bytes the job holds (arrays + packed operands + workspace for n triples). */
+int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples);
+
+/* ---------------------------------------------------------------- diagnostics */
+/* One v_mfma_f64_16x16x4_f64 on A (16x4 row major, 64 doubles) and B (4x16 row
+ * major, 64 doubles); d256[4*lane + r] receives each lane's four results.  No
+ * counterpart in the reference: checks the lane layout the kernels assume. */
+int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCSDT_H */

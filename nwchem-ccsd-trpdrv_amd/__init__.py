@@ -1,0 +1,368 @@
+"""Host-side interface to libccsdt.so, the MI355X (gfx950) CCSD(T) triples body.
+
+The library exports the reference's C entry points (``ccsd_trpdrv_omp_cbody_``,
+``ccsd_tengy_omp`` and their aliases, see include/ccsdt.h) plus a job API with
+explicit device residency.  This module binds both with ctypes and adds the
+synthetic inputs and triple lists of the benchmark harness (omp/test.c).
+
+There is no CPU fallback: every compute call goes to the HIP library, and a
+missing library or device raises ``CcsdtError``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccsdt.so")
+
+# Job array order = the body's argument order (omp/ccsd_trpdrv_omp_cbody.c:31-44).
+ARRAY_NAMES = ["eorb", "tij", "tkj", "tia", "tka", "xia", "xka", "jia", "jka", "kia",
+               "kka", "jij", "jkj", "kij", "kkj", "dintc1", "dintx1", "t1v1", "dintc2",
+               "dintx2", "t1v2"]
+# PCG stream of each synthetic array: its 1-based position in the body's
+# argument list; Dja/Djka/Djia (allocated by omp/test.c:137-139, never passed)
+# follow at 40-42.
+STREAM = {"eorb": 9, "tij": 20, "tkj": 21, "tia": 22, "tka": 23, "xia": 24, "xka": 25,
+          "jia": 26, "jka": 27, "kia": 28, "kka": 29, "jij": 30, "jkj": 31, "kij": 32,
+          "kkj": 33, "dintc1": 34, "dintx1": 35, "t1v1": 36, "dintc2": 37, "dintx2": 38,
+          "t1v2": 39, "dja": 40, "djka": 41, "djia": 42}
+ILLCOND_STREAM_BASE = 100   # exponent draws of the ill-conditioned variant
+JK_NAMES = ["jia", "jka", "kia", "kka", "jij", "jkj", "kij", "kkj"]
+
+MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3}
+E_CODES = {0: "OK", -1: "E_ARG", -2: "E_DEVICE", -3: "E_NOMEM", -4: "E_BUDGET",
+           -5: "E_STATE", -6: "E_INJECTED"}
+
+
+class CcsdtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (E_CODES.get(code, "?"), code, msg))
+        self.code = code
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("ncor", ctypes.c_int), ("nocc", ctypes.c_int), ("nvir", ctypes.c_int),
+                ("kchunk", ctypes.c_int), ("arrays", ctypes.c_void_p * len(ARRAY_NAMES))]
+
+
+class RunOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("slices", ctypes.c_int), ("flags", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+class RunResult(ctypes.Structure):
+    _fields_ = [("emp4", ctypes.c_double), ("emp5", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("ms_pack", ctypes.c_double),
+                ("ms_main", ctypes.c_double), ("ms_tengy", ctypes.c_double),
+                ("ms_reduce", ctypes.c_double), ("ntriples", ctypes.c_int64),
+                ("launches", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    """Loads libccsdt.so (built in-tree by build.py); raises if it is missing."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CcsdtError(-2, "libccsdt.so not built: run nwchem-ccsd-trpdrv_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        sig = {
+            "ccsdt_last_error": (ctypes.c_int, []),
+            "ccsdt_last_error_string": (ctypes.c_char_p, []),
+            "ccsdt_clear_error": (None, []),
+            "ccsdt_set_error_policy": (None, [ctypes.c_int]),
+            "ccsdt_debug_inject_failure": (None, [ctypes.c_int]),
+            "ccsdt_version": (ctypes.c_char_p, []),
+            "ccsdt_init": (ctypes.c_int, [ctypes.c_int]),
+            "ccsdt_set_f_writeback": (None, [ctypes.c_int]),
+            "ccsdt_pcg_raw": (None, [ctypes.c_uint64, ctypes.c_uint64, vp, i64]),
+            "ccsdt_pcg_uniform": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                         ctypes.c_double, vp, i64]),
+            "ccsdt_job_create": (ctypes.c_int, [ctypes.POINTER(JobDesc), ctypes.POINTER(vp)]),
+            "ccsdt_job_run": (ctypes.c_int, [vp, vp, i64, ctypes.POINTER(RunOpts),
+                                             ctypes.POINTER(RunResult), vp, vp]),
+            "ccsdt_job_destroy": (ctypes.c_int, [vp]),
+            "ccsdt_job_footprint": (i64, [ctypes.c_int] * 4 + [i64]),
+            "ccsdt_selftest_mfma": (ctypes.c_int, [vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        body_args = [_dp] * 9 + [_ip] * 3 + [_dp] * 2 + [_ip] * 5 + [_dp] * 20
+        for name in ("ccsd_trpdrv_omp_cbody_", "ccsd_trpdrv_cbody_", "ccsd_trpdrv_omp_fbody_"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = body_args
+        tengy_args = [_dp] * 15 + [ctypes.c_double] + [_dp] * 4 + [ctypes.c_int] * 3
+        for name in ("ccsd_tengy_omp", "ccsd_tengy"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = tengy_args
+        L.ccsd_tengy_omp_.restype = None
+        L.ccsd_tengy_omp_.argtypes = [_dp] * 20 + [_ip] * 3
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    L = lib()
+    return L.ccsdt_last_error(), L.ccsdt_last_error_string().decode()
+
+
+def _check(rc):
+    if rc != 0:
+        raise CcsdtError(rc, lib().ccsdt_last_error_string().decode())
+
+
+def init(device=0):
+    _check(lib().ccsdt_init(device))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+# ------------------------------------------------------------------ inputs
+def pcg_raw(seed, stream, n):
+    out = np.empty(n, dtype=np.uint32)
+    lib().ccsdt_pcg_raw(seed, stream, _ptr(out), n)
+    return out
+
+
+def pcg_uniform(seed, stream, lo, hi, n):
+    out = np.empty(n, dtype=np.float64)
+    lib().ccsdt_pcg_uniform(seed, stream, lo, hi, _ptr(out), n)
+    return out
+
+
+def config_seed(cfg):
+    """Seed of the synthetic inputs of a workload configuration."""
+    return 0xC0DE0000 + int(cfg)
+
+
+def make_inputs(nocc, nvir, kchunk, seed, ncor=0, illcond=False):
+    """Seeded arrays at the harness's shapes (omp/test.c:112-146).
+
+    eorb: ncor core energies U[-20,-10), nocc occupied U[-1.5,-0.3) and nvir
+    virtual U[0.1,4.0), each segment sorted ascending.  Every T/X/J/K/D/dint/t1v
+    array is U(-0.05,0.05).  ``illcond`` scales the J and K arrays elementwise by
+    10**u, u ~ U(-6,0), drawn from stream 100 + the array's stream.
+    """
+    vv, ov = nvir * nvir, nocc * nvir
+    nbf = ncor + nocc + nvir
+    u = pcg_uniform(seed, STREAM["eorb"], 0.0, 1.0, nbf)
+    eorb = np.empty(nbf)
+    eorb[:ncor] = np.sort(-20.0 + 10.0 * u[:ncor])
+    eorb[ncor:ncor + nocc] = np.sort(-1.5 + 1.2 * u[ncor:ncor + nocc])
+    eorb[ncor + nocc:] = np.sort(0.1 + 3.9 * u[ncor + nocc:])
+    sizes = {"tij": vv, "tkj": kchunk * vv, "tia": ov * nocc, "tka": kchunk * ov,
+             "xia": ov * nocc, "xka": kchunk * ov, "jia": vv, "jka": kchunk * vv, "kia": vv,
+             "kka": kchunk * vv, "jij": ov * nocc, "jkj": kchunk * ov, "kij": ov * nocc,
+             "kkj": kchunk * ov, "dja": ov, "djka": nvir * kchunk, "djia": nvir * nocc}
+    for name in ("dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2"):
+        sizes[name] = vv
+    arrays = {"eorb": eorb}
+    for name, n in sizes.items():
+        arrays[name] = pcg_uniform(seed, STREAM[name], -0.05, 0.05, n)
+    if illcond:
+        for name in JK_NAMES:
+            e = pcg_uniform(seed, ILLCOND_STREAM_BASE + STREAM[name], -6.0, 0.0, arrays[name].size)
+            arrays[name] = arrays[name] * np.power(10.0, e)
+    return arrays
+
+
+def blocks_uniform(nocc, length):
+    """k blocks klo..khi of a fixed length (kchunk, omp/test.c:97,160-161)."""
+    length = max(1, min(int(length), nocc))
+    return [(klo, min(nocc, klo + length - 1)) for klo in range(1, nocc + 1, length)]
+
+
+def blocks_uneven(nocc, seed, lo=8, hi=48):
+    """k blocks of lengths drawn uniformly from {lo..hi} until nocc is covered."""
+    draws = pcg_raw(seed, 77, 4 * nocc + 8)
+    blocks, klo, q = [], 1, 0
+    while klo <= nocc:
+        length = lo + int(draws[q] % (hi - lo + 1))
+        q += 1
+        blocks.append((klo, min(nocc, klo + length - 1)))
+        klo += length
+    return blocks
+
+
+def triple_list(nocc, blocks, order="ijk", a=None, ncor=0):
+    """(a,i,j,k,klo) quintuples, 1-based, as int32 array of shape (n,5).
+
+    order "ijk": i >= j >= k within each k block; "ref": the loop of
+    omp/test.c:160-165 (j and i over all occupied, k <= i).  ``a`` defaults to
+    the first virtual orbital ncor+nocc+1 (a virtual a keeps the denominators
+    away from zero, unlike omp/test.c:162's a=1).
+    """
+    if a is None:
+        a = ncor + nocc + 1
+    out = []
+    for klo, khi in blocks:
+        if order == "ijk":
+            for i in range(1, nocc + 1):
+                for j in range(1, i + 1):
+                    for k in range(klo, min(khi, j) + 1):
+                        out.append((a, i, j, k, klo))
+        elif order == "ref":
+            for j in range(1, nocc + 1):
+                for i in range(1, nocc + 1):
+                    for k in range(klo, min(khi, i) + 1):
+                        out.append((a, i, j, k, klo))
+        else:
+            raise ValueError("order must be 'ijk' or 'ref'")
+    return np.ascontiguousarray(np.array(out, dtype=np.int32).reshape(-1, 5))
+
+
+# Workload configurations (sizes and block scheme of each measured case).
+CONFIGS = {
+    0: dict(name="fp64-cpu-case", nocc=8, nvir=40, block=8),
+    1: dict(name="fp64-production", nocc=30, nvir=300, block=16),
+    2: dict(name="split-production", nocc=30, nvir=300, block=16),
+    3: dict(name="batched-uneven", nocc=40, nvir=400, block="uneven"),
+}
+
+
+def config_case(cfg, illcond=False, order="ijk"):
+    """(inputs, blocks, triples, kchunk) of a configuration."""
+    c = CONFIGS[cfg]
+    seed = config_seed(cfg)
+    if c["block"] == "uneven":
+        blocks = blocks_uneven(c["nocc"], seed)
+    else:
+        blocks = blocks_uniform(c["nocc"], c["block"])
+    kchunk = max(khi - klo + 1 for klo, khi in blocks)
+    arrays = make_inputs(c["nocc"], c["nvir"], kchunk, seed, illcond=illcond)
+    return arrays, blocks, triple_list(c["nocc"], blocks, order), kchunk
+
+
+def gemm_flops_per_triple(nocc, nvir):
+    """16*nvir^2*(nvir+nocc): the 16 GEMMs at 2 flops per multiply-add."""
+    return 16.0 * nvir * nvir * (nvir + nocc)
+
+
+def tengy_bytes_per_triple(nvir):
+    """64*nvir^2: the eight f matrices read once by the energy pass."""
+    return 64.0 * nvir * nvir
+
+
+# ------------------------------------------------------------------ job API
+class Job:
+    """Device-resident arrays of one harness-shaped job (see ccsdt_job_create)."""
+
+    def __init__(self, arrays, nocc, nvir, kchunk, ncor=0):
+        self.ncor, self.nocc, self.nvir, self.kchunk = ncor, nocc, nvir, kchunk
+        d = JobDesc(ncor, nocc, nvir, kchunk)
+        self._keep = []
+        for q, name in enumerate(ARRAY_NAMES):
+            a = np.ascontiguousarray(arrays[name], dtype=np.float64)
+            self._keep.append(a)
+            d.arrays[q] = a.ctypes.data
+        h = ctypes.c_void_p()
+        _check(lib().ccsdt_job_create(ctypes.byref(d), ctypes.byref(h)))
+        self._keep = None
+        self.handle = h
+
+    def run(self, triples, mode="fp64", slices=0, terms=False, fout=False):
+        trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
+        n = trip.shape[0]
+        opts = RunOpts(MODES[mode], slices, 0, 0)
+        res = RunResult()
+        t = np.empty((n, 4)) if terms else None
+        f = np.empty((8, self.nvir, self.nvir)) if fout else None
+        _check(lib().ccsdt_job_run(self.handle, _ptr(trip), n, ctypes.byref(opts),
+                                   ctypes.byref(res), _ptr(t) if terms else None,
+                                   _ptr(f) if fout else None))
+        out = res.as_dict()
+        if terms:
+            out["terms"] = t
+        if fout:
+            # f[m] is column major (nvir x nvir): f[m][c, b] holds element (b, c)
+            out["f"] = np.ascontiguousarray(np.swapaxes(f, 1, 2))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().ccsdt_job_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(nocc, nvir, kchunk, ntriples, ncor=0):
+    return lib().ccsdt_job_footprint(ncor, nocc, nvir, kchunk, ntriples)
+
+
+# ------------------------------------------------------------------ drop-ins
+def cbody(f, eorb, ncor, nocc, nvir, emp4, emp5, a, i, j, k, klo, arrays,
+          symbol="ccsd_trpdrv_omp_cbody_"):
+    """Calls the drop-in body with the reference's argument list.
+
+    ``f`` is a list of eight float64 arrays (f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t),
+    overwritten in place (column major, nvir*nvir each); returns (emp4, emp5)
+    after the accumulation.
+    """
+    ints = [ctypes.c_int(v) for v in (ncor, nocc, nvir, a, i, j, k, klo)]
+    e4, e5 = ctypes.c_double(emp4), ctypes.c_double(emp5)
+    arr = [np.ascontiguousarray(arrays[n], dtype=np.float64) for n in ARRAY_NAMES[1:]]
+    fn = getattr(lib(), symbol)
+    fn(*[_dptr(x) for x in f], _dptr(np.ascontiguousarray(eorb)),
+       ctypes.byref(ints[0]), ctypes.byref(ints[1]), ctypes.byref(ints[2]),
+       ctypes.byref(e4), ctypes.byref(e5),
+       *[ctypes.byref(v) for v in ints[3:]], *[_dptr(x) for x in arr])
+    return e4.value, e5.value
+
+
+def tengy(f, vec, eorb, eaijk, ncor, nocc, nvir, symbol="ccsd_tengy_omp"):
+    """Drop-in energy reduction; f as in cbody, vec = dict of the six vectors.
+
+    Returns (emp4i, emp5i, emp4k, emp5k)."""
+    out = [ctypes.c_double() for _ in range(4)]
+    L = lib()
+    fs = [_dptr(np.ascontiguousarray(x)) for x in f]
+    v = {n: _dptr(np.ascontiguousarray(vec[n])) for n in
+         ("dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2")}
+    e = _dptr(np.ascontiguousarray(eorb))
+    if symbol == "ccsd_tengy_omp_":
+        ea = ctypes.c_double(eaijk)
+        sz = [ctypes.c_int(x) for x in (ncor, nocc, nvir)]
+        L.ccsd_tengy_omp_(*fs, v["dintc1"], v["dintc2"], v["dintx1"], v["dintx2"], v["t1v1"],
+                          v["t1v2"], e, ctypes.byref(ea), *[ctypes.byref(o) for o in out],
+                          *[ctypes.byref(s) for s in sz])
+    else:
+        getattr(L, symbol)(*fs, v["dintc1"], v["dintx1"], v["t1v1"], v["dintc2"], v["dintx2"],
+                           v["t1v2"], e, eaijk, *[ctypes.byref(o) for o in out], ncor, nocc, nvir)
+    return tuple(o.value for o in out)
+
+
+def selftest_mfma(a, b):
+    d = np.empty(256)
+    _check(lib().ccsdt_selftest_mfma(_ptr(np.ascontiguousarray(a, dtype=np.float64)),
+                                     _ptr(np.ascontiguousarray(b, dtype=np.float64)), _ptr(d)))
+    return d

@@ -1,0 +1,867 @@
+// ccsdt_api.hip -- host side of libccsdt.so: the C ABI declared in
+// include/ccsdt.h (drop-ins with the reference's argument lists, the job API,
+// error reporting and the seeded input generator).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ccsdt.h"
+#include "ccsdt_internal.h"
+
+using namespace ccsdt;
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local int t_err = CCSDT_OK;
+thread_local char t_msg[512] = "";
+std::atomic<int> g_abort_policy{1};
+std::atomic<int> g_inject{0};
+std::atomic<int> g_f_writeback{-1};
+
+int set_error(int code, const char *fmt, ...)
+{
+    t_err = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_msg, sizeof t_msg, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// Counts device operations for ccsdt_debug_inject_failure.
+bool injected()
+{
+    int n = g_inject.load();
+    if (n <= 0) return false;
+    if (n == 1) {
+        g_inject.store(0);
+        return true;
+    }
+    g_inject.store(n - 1);
+    return false;
+}
+
+#define HIP_TRY(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_error(CCSDT_E_DEVICE, "%s failed: %s (%s:%d)", #call,                \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+#define INJECT_POINT(what)                                                                   \
+    do {                                                                                     \
+        if (injected()) return set_error(CCSDT_E_INJECTED, "injected failure at %s", what); \
+    } while (0)
+
+int dev_alloc(void **p, size_t bytes)
+{
+    INJECT_POINT("device allocation");
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return set_error(CCSDT_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return CCSDT_OK;
+}
+
+// ------------------------------------------------------------------ device
+std::mutex g_init_mu;
+int g_device = -1;
+
+int ensure_device(int want)
+{
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_device >= 0 && (want < 0 || want == g_device)) {
+        HIP_TRY(hipSetDevice(g_device));
+        return CCSDT_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return set_error(CCSDT_E_DEVICE, "no HIP device visible (%s); libccsdt needs a gfx950 GPU",
+                         e != hipSuccess ? hipGetErrorString(e) : "count 0");
+    }
+    const int dev = want < 0 ? 0 : want;
+    if (dev >= n) return set_error(CCSDT_E_DEVICE, "device %d requested, %d visible", dev, n);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(CCSDT_E_DEVICE, "device %d is %s; libccsdt is built for gfx950 only", dev,
+                         prop.gcnArchName);
+    HIP_TRY(hipSetDevice(dev));
+    g_device = dev;
+    return CCSDT_OK;
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+Geom make_geom(int ncor, int nocc, int nvir)
+{
+    Geom g;
+    g.ncor = ncor;
+    g.nocc = nocc;
+    g.nvir = nvir;
+    g.nvp = (int)round_up(nvir, 16);
+    g.nb = g.nvp / 16;
+    g.kdim = nvir + nocc;
+    g.kp = (int)round_up(g.kdim, 8);
+    g.nc = g.kp / 8;
+    g.npairs = g.nb * (g.nb + 1) / 2;
+    g.vecsz = (int64_t)g.nvp * g.kp;
+    return g;
+}
+
+// Element counts of the job arrays (omp/test.c:112-146 shapes; tia, xia, jij,
+// kij and the dint/t1v vectors at the lengths the body reads).
+void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n)
+{
+    const int64_t vv = (int64_t)nvir * nvir, ov = (int64_t)nocc * nvir;
+    n[CCSDT_EORB] = ncor + nocc + nvir;
+    n[CCSDT_TIJ] = vv;
+    n[CCSDT_TKJ] = kchunk * vv;
+    n[CCSDT_TIA] = ov;
+    n[CCSDT_TKA] = kchunk * ov;
+    n[CCSDT_XIA] = ov;
+    n[CCSDT_XKA] = kchunk * ov;
+    n[CCSDT_JIA] = vv;
+    n[CCSDT_JKA] = kchunk * vv;
+    n[CCSDT_KIA] = vv;
+    n[CCSDT_KKA] = kchunk * vv;
+    n[CCSDT_JIJ] = ov;
+    n[CCSDT_JKJ] = kchunk * ov;
+    n[CCSDT_KIJ] = ov;
+    n[CCSDT_KKJ] = kchunk * ov;
+    for (int q = CCSDT_DINTC1; q <= CCSDT_T1V2; ++q) n[q] = nvir;
+}
+
+// Bytes of f workspace used per chunk by the separate-energy-pass mode.
+constexpr int64_t kFWorkspaceBytes = int64_t(4) << 30;
+}  // namespace
+
+// ------------------------------------------------------------------ job
+struct ccsdt_job_s {
+    Geom g{};
+    int kchunk = 0;
+    hipStream_t st = nullptr;
+    int64_t nraw = 0;
+    int64_t off[CCSDT_NARRAYS] = {};
+    int64_t cnt[CCSDT_NARRAYS] = {};
+    double *d_raw = nullptr;
+    double *h_stage = nullptr;  // pinned staging for uploads
+    double *h_vec = nullptr;    // pinned staging for the 7 padded vectors
+    double *h_fstage = nullptr; // pinned staging for f matrices (drop-in energy pass)
+    double *d_ev = nullptr;     // virtual orbital energies, padded to nvp
+    double *d_vec6 = nullptr;   // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2, each nvp
+    double *d_pan = nullptr;
+    PanelDesc *d_desc = nullptr;
+    int npanels = 0;
+    short2 *d_pairs = nullptr;
+    Task *d_tasks = nullptr;
+    Task *h_tasks = nullptr;
+    int64_t cap_tasks = 0;
+    double *d_partial = nullptr;
+    int64_t cap_partial = 0;
+    double *d_terms = nullptr;
+    int64_t cap_terms = 0;
+    double *d_f = nullptr;
+    int64_t cap_f = 0;
+    double *d_out2 = nullptr;
+    double *h_out = nullptr;  // pinned: 2 totals + 4 terms
+    std::vector<double> eorb;
+    hipEvent_t ev[12] = {};
+};
+
+namespace {
+void job_free(ccsdt_job j)
+{
+    if (!j) return;
+    if (j->st) (void)hipStreamSynchronize(j->st);
+    double *dd[] = {j->d_raw, j->d_ev, j->d_vec6, j->d_pan, j->d_partial, j->d_terms, j->d_f, j->d_out2};
+    for (double *p : dd)
+        if (p) (void)hipFree(p);
+    if (j->d_desc) (void)hipFree(j->d_desc);
+    if (j->d_pairs) (void)hipFree(j->d_pairs);
+    if (j->d_tasks) (void)hipFree(j->d_tasks);
+    if (j->h_tasks) (void)hipHostFree(j->h_tasks);
+    if (j->h_stage) (void)hipHostFree(j->h_stage);
+    if (j->h_vec) (void)hipHostFree(j->h_vec);
+    if (j->h_fstage) (void)hipHostFree(j->h_fstage);
+    if (j->h_out) (void)hipHostFree(j->h_out);
+    for (hipEvent_t &e : j->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (j->st) (void)hipStreamDestroy(j->st);
+    delete j;
+}
+
+int grow(double **p, int64_t *cap, int64_t need)
+{
+    if (need <= *cap) return CCSDT_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    int rc = dev_alloc((void **)p, (size_t)need * sizeof(double));
+    if (rc) return rc;
+    *cap = need;
+    return CCSDT_OK;
+}
+
+int check_desc(const ccsdt_job_desc *d)
+{
+    if (!d) return set_error(CCSDT_E_ARG, "null job descriptor");
+    if (d->ncor < 0 || d->nocc < 1 || d->nvir < 1 || d->kchunk < 1 || d->kchunk > d->nocc)
+        return set_error(CCSDT_E_ARG, "invalid sizes ncor=%d nocc=%d nvir=%d kchunk=%d", d->ncor,
+                         d->nocc, d->nvir, d->kchunk);
+    if (d->nvir > 8192 || d->nocc > 4096)
+        return set_error(CCSDT_E_ARG, "nvir=%d / nocc=%d beyond supported range", d->nvir, d->nocc);
+    for (int q = 0; q < CCSDT_NARRAYS; ++q)
+        if (!d->arrays[q]) return set_error(CCSDT_E_ARG, "array %d of the job is null", q);
+    return CCSDT_OK;
+}
+
+// Refreshes eorb and the six nvir vectors (padded copies on the device).
+int job_upload_vectors(ccsdt_job j, const double *eorb, const double *const *vec6)
+{
+    const Geom &g = j->g;
+    j->eorb.assign(eorb, eorb + j->cnt[CCSDT_EORB]);
+    double *hv = j->h_vec;
+    for (int b = 0; b < g.nvp; ++b) hv[b] = b < g.nvir ? j->eorb[g.ncor + g.nocc + b] : 1.0;
+    for (int q = 0; q < 6; ++q)
+        for (int b = 0; b < g.nvp; ++b) hv[g.nvp * (1 + q) + b] = b < g.nvir ? vec6[q][b] : 0.0;
+    INJECT_POINT("upload");
+    HIP_TRY(hipMemcpyAsync(j->d_ev, hv, (size_t)g.nvp * sizeof(double), hipMemcpyHostToDevice, j->st));
+    HIP_TRY(hipMemcpyAsync(j->d_vec6, hv + g.nvp, (size_t)6 * g.nvp * sizeof(double),
+                           hipMemcpyHostToDevice, j->st));
+    HIP_TRY(hipStreamSynchronize(j->st));
+    return CCSDT_OK;
+}
+
+// Copies the host arrays into the job (via pinned staging) and refreshes the
+// derived vectors.  `slice_off[q]` (elements) selects where array q is read.
+int job_upload(ccsdt_job j, const double *const *arrays, const int64_t *slice_off)
+{
+    for (int q = 0; q < CCSDT_NARRAYS; ++q)
+        memcpy(j->h_stage + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
+               (size_t)j->cnt[q] * sizeof(double));
+    INJECT_POINT("upload");
+    HIP_TRY(hipMemcpyAsync(j->d_raw, j->h_stage, (size_t)j->nraw * sizeof(double),
+                           hipMemcpyHostToDevice, j->st));
+    // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2: the order the kernels read
+    const double *v6[6] = {arrays[CCSDT_T1V1], arrays[CCSDT_T1V2], arrays[CCSDT_DINTX1],
+                           arrays[CCSDT_DINTX2], arrays[CCSDT_DINTC1], arrays[CCSDT_DINTC2]};
+    return job_upload_vectors(j, arrays[CCSDT_EORB], v6);
+}
+
+// Task buffers of at least n entries (device + pinned host).
+int ensure_tasks(ccsdt_job j, int64_t n)
+{
+    if (n <= j->cap_tasks) return CCSDT_OK;
+    if (j->d_tasks) (void)hipFree(j->d_tasks);
+    if (j->h_tasks) (void)hipHostFree(j->h_tasks);
+    j->d_tasks = nullptr;
+    j->h_tasks = nullptr;
+    j->cap_tasks = 0;
+    int rc = dev_alloc((void **)&j->d_tasks, (size_t)n * sizeof(Task));
+    if (rc) return rc;
+    HIP_TRY(hipHostMalloc((void **)&j->h_tasks, (size_t)n * sizeof(Task), 0));
+    j->cap_tasks = n;
+    return CCSDT_OK;
+}
+
+int job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
+{
+    *out = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    rc = ensure_device(-1);
+    if (rc) return rc;
+    ccsdt_job j = new ccsdt_job_s();
+    j->g = make_geom(desc->ncor, desc->nocc, desc->nvir);
+    j->kchunk = desc->kchunk;
+    const Geom &g = j->g;
+    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, j->cnt);
+    int64_t o = 0;
+    for (int q = 0; q < CCSDT_NARRAYS; ++q) {
+        j->off[q] = o;
+        o += round_up(j->cnt[q], 32);  // 256-byte aligned sub-arrays
+    }
+    j->nraw = o;
+#define JOB_TRY(x)              \
+    do {                        \
+        int r_ = (x);           \
+        if (r_) {               \
+            job_free(j);        \
+            return r_;          \
+        }                       \
+    } while (0)
+#define JOB_HIP(x)                                                                             \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) {                                                                \
+            job_free(j);                                                                       \
+            return set_error(CCSDT_E_DEVICE, "%s failed: %s", #x, hipGetErrorString(e_));      \
+        }                                                                                      \
+    } while (0)
+    JOB_HIP(hipStreamCreate(&j->st));
+    for (hipEvent_t &e : j->ev) JOB_HIP(hipEventCreate(&e));
+    JOB_TRY(dev_alloc((void **)&j->d_raw, (size_t)j->nraw * sizeof(double)));
+    JOB_HIP(hipHostMalloc((void **)&j->h_stage, (size_t)j->nraw * sizeof(double), 0));
+    JOB_HIP(hipHostMalloc((void **)&j->h_vec, (size_t)7 * g.nvp * sizeof(double), 0));
+    JOB_HIP(hipHostMalloc((void **)&j->h_out, 64 * sizeof(double), 0));
+    JOB_TRY(dev_alloc((void **)&j->d_ev, (size_t)g.nvp * sizeof(double)));
+    JOB_TRY(dev_alloc((void **)&j->d_vec6, (size_t)6 * g.nvp * sizeof(double)));
+    JOB_TRY(dev_alloc((void **)&j->d_out2, 8 * sizeof(double)));
+    j->npanels = PAN_SLICE0 + 4 * j->kchunk;
+    JOB_TRY(dev_alloc((void **)&j->d_pan, (size_t)j->npanels * g.vecsz * sizeof(double)));
+
+    // panel descriptors: which raw arrays feed each packed operand
+    const int64_t vv = (int64_t)g.nvir * g.nvir, ov = (int64_t)g.nocc * g.nvir;
+    auto raw = [&](int q) { return j->d_raw + j->off[q]; };
+    std::vector<PanelDesc> ds(j->npanels);
+    auto set = [&](int p, const double *P, int ft, const double *Q, int st2, double sgn) {
+        ds[p] = PanelDesc{P, Q, j->d_pan + (int64_t)p * g.vecsz, ft, st2, sgn};
+    };
+    set(PAN_LJ, raw(CCSDT_JIA), 0, raw(CCSDT_TIA), 0, 1.0);
+    set(PAN_LK, raw(CCSDT_KIA), 0, raw(CCSDT_XIA), 0, 1.0);
+    set(PAN_R1T, raw(CCSDT_TIJ), 0, raw(CCSDT_KIJ), 1, -1.0);
+    set(PAN_R3T, raw(CCSDT_TIJ), 1, raw(CCSDT_JIJ), 1, -1.0);
+    for (int s = 0; s < j->kchunk; ++s) {
+        const int b = PAN_SLICE0 + 4 * s;
+        set(b + SL_R1, raw(CCSDT_TKJ) + s * vv, 0, raw(CCSDT_KKJ) + s * ov, 1, -1.0);
+        set(b + SL_R3, raw(CCSDT_TKJ) + s * vv, 1, raw(CCSDT_JKJ) + s * ov, 1, -1.0);
+        set(b + SL_LJT, raw(CCSDT_JKA) + s * vv, 0, raw(CCSDT_TKA) + s * ov, 0, 1.0);
+        set(b + SL_LKT, raw(CCSDT_KKA) + s * vv, 0, raw(CCSDT_XKA) + s * ov, 0, 1.0);
+    }
+    JOB_TRY(dev_alloc((void **)&j->d_desc, ds.size() * sizeof(PanelDesc)));
+    JOB_HIP(hipMemcpy(j->d_desc, ds.data(), ds.size() * sizeof(PanelDesc), hipMemcpyHostToDevice));
+
+    std::vector<short2> pairs;
+    for (int bi = 0; bi < g.nb; ++bi)
+        for (int ci = 0; ci <= bi; ++ci) pairs.push_back(make_short2((short)bi, (short)ci));
+    JOB_TRY(dev_alloc((void **)&j->d_pairs, pairs.size() * sizeof(short2)));
+    JOB_HIP(hipMemcpy(j->d_pairs, pairs.data(), pairs.size() * sizeof(short2), hipMemcpyHostToDevice));
+
+    JOB_TRY(job_upload(j, desc->arrays, nullptr));
+#undef JOB_TRY
+#undef JOB_HIP
+    *out = j;
+    return CCSDT_OK;
+}
+
+// Builds the device task list from (a,i,j,k,klo) quintuples.
+int job_tasks(ccsdt_job j, const int *trip, int64_t n)
+{
+    const Geom &g = j->g;
+    const int nbf = g.ncor + g.nocc + g.nvir;
+    int rc = ensure_tasks(j, n);
+    if (rc) return rc;
+    const double *e = j->eorb.data();
+    for (int64_t t = 0; t < n; ++t) {
+        const int a = trip[5 * t], i = trip[5 * t + 1], jj = trip[5 * t + 2];
+        const int k = trip[5 * t + 3], klo = trip[5 * t + 4];
+        if (a < 1 || a > nbf || i < 1 || i > g.nocc || jj < 1 || jj > g.nocc || k < 1 ||
+            k > g.nocc || klo < 1 || klo > k || k - klo >= j->kchunk)
+            return set_error(CCSDT_E_ARG,
+                             "triple %lld (a=%d i=%d j=%d k=%d klo=%d) outside nbf=%d nocc=%d kchunk=%d",
+                             (long long)t, a, i, jj, k, klo, nbf, g.nocc, j->kchunk);
+        Task &tk = j->h_tasks[t];
+        tk.a = a - 1;
+        tk.i = i - 1;
+        tk.j = jj - 1;
+        tk.k = k - 1;
+        tk.slice = k - klo;
+        tk.ik_differ = (i != k);
+        // omp/ccsd_trpdrv_omp_cbody.c:126, same operation order
+        tk.eaijk = e[a - 1] - (e[g.ncor + i - 1] + e[g.ncor + jj - 1] + e[g.ncor + k - 1]);
+    }
+    HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice,
+                           j->st));
+    return CCSDT_OK;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+// Runs n triples.  fouts (8 host pointers or null) receives the f matrices of a
+// single-triple run.
+int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
+            ccsdt_run_result *res, double *terms, double *const *fouts)
+{
+    if (!j) return set_error(CCSDT_E_STATE, "null job");
+    if (n < 1 || !trip) return set_error(CCSDT_E_ARG, "empty triple list");
+    if (fouts && n != 1) return set_error(CCSDT_E_ARG, "f output needs a single triple");
+    const int mode = opts ? opts->mode : CCSDT_MODE_FP64;
+    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT)
+        return set_error(CCSDT_E_ARG, "mode %d not available in this build", mode);
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    const Geom &g = j->g;
+    if ((int64_t)n * g.npairs > 0x7fffffff) return set_error(CCSDT_E_ARG, "triple list too long");
+    const int64_t vv = (int64_t)g.nvir * g.nvir;
+    rc = job_tasks(j, trip, n);
+    if (rc) return rc;
+    const int tnp = tengy_npairs(g);
+    const int64_t npart = (int64_t)n * (mode == CCSDT_MODE_FP64 ? g.npairs : tnp) * 4;
+    if ((rc = grow(&j->d_partial, &j->cap_partial, npart))) return rc;
+    if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * n))) return rc;
+    int64_t chunk = n;
+    if (mode == CCSDT_MODE_FP64_SPLIT) {
+        chunk = kFWorkspaceBytes / (8 * vv * (int64_t)sizeof(double));
+        if (chunk < 1) chunk = 1;
+        if (chunk > n) chunk = n;
+        if ((rc = grow(&j->d_f, &j->cap_f, chunk * 8 * vv))) return rc;
+    } else if (fouts) {
+        if ((rc = grow(&j->d_f, &j->cap_f, 8 * vv))) return rc;
+    }
+    hipEvent_t *E = j->ev;
+    int64_t launches = 0;
+    double ms_main = 0.0, ms_tengy = 0.0;
+    INJECT_POINT("launch");
+    HIP_TRY(hipEventRecord(E[0], j->st));
+    HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+    ++launches;
+    HIP_TRY(hipEventRecord(E[1], j->st));
+    if (mode == CCSDT_MODE_FP64) {
+        HIP_TRY(launch_fused(j->d_pan, g, j->d_tasks, (int)n, j->d_pairs, j->d_ev, j->d_vec6,
+                             j->d_partial, fouts ? j->d_f : nullptr, 8 * vv, true, j->st));
+        HIP_TRY(hipEventRecord(E[2], j->st));
+        HIP_TRY(launch_reduce_terms(j->d_partial, g.npairs, (int)n, j->d_terms, j->st));
+        launches += 2;
+    } else {
+        for (int64_t t0 = 0; t0 < n; t0 += chunk) {
+            const int m = (int)std::min<int64_t>(chunk, n - t0);
+            HIP_TRY(hipEventRecord(E[4], j->st));
+            HIP_TRY(launch_fused(j->d_pan, g, j->d_tasks + t0, m, j->d_pairs, j->d_ev, j->d_vec6,
+                                 nullptr, j->d_f, 8 * vv, false, j->st));
+            HIP_TRY(hipEventRecord(E[5], j->st));
+            HIP_TRY(launch_tengy(j->d_f, 8 * vv, g, j->d_tasks + t0, m, j->d_ev, j->d_vec6,
+                                 j->d_partial + t0 * tnp * 4, j->st));
+            HIP_TRY(hipEventRecord(E[6], j->st));
+            launches += 2;
+            HIP_TRY(hipEventSynchronize(E[6]));
+            ms_main += ev_ms(E[4], E[5]);
+            ms_tengy += ev_ms(E[5], E[6]);
+        }
+        HIP_TRY(hipEventRecord(E[2], j->st));
+        HIP_TRY(launch_reduce_terms(j->d_partial, tnp, (int)n, j->d_terms, j->st));
+        ++launches;
+    }
+    HIP_TRY(launch_reduce_list(j->d_terms, j->d_tasks, (int)n, j->d_out2, j->st));
+    ++launches;
+    HIP_TRY(hipEventRecord(E[3], j->st));
+    HIP_TRY(hipMemcpyAsync(j->h_out, j->d_out2, 2 * sizeof(double), hipMemcpyDeviceToHost, j->st));
+    if (terms)
+        HIP_TRY(hipMemcpyAsync(terms, j->d_terms, (size_t)4 * n * sizeof(double),
+                               hipMemcpyDeviceToHost, j->st));
+    if (fouts) {
+        for (int m = 0; m < 8; ++m)
+            if (fouts[m])
+                HIP_TRY(hipMemcpyAsync(fouts[m], j->d_f + m * vv, (size_t)vv * sizeof(double),
+                                       hipMemcpyDeviceToHost, j->st));
+    }
+    HIP_TRY(hipStreamSynchronize(j->st));
+    if (res) {
+        res->emp4 = j->h_out[0];
+        res->emp5 = j->h_out[1];
+        res->ms_total = ev_ms(E[0], E[3]);
+        res->ms_pack = ev_ms(E[0], E[1]);
+        res->ms_main = mode == CCSDT_MODE_FP64 ? ev_ms(E[1], E[2]) : ms_main;
+        res->ms_tengy = ms_tengy;
+        res->ms_reduce = ev_ms(E[2], E[3]);
+        res->ntriples = n;
+        res->launches = launches;
+    }
+    return CCSDT_OK;
+}
+
+// ------------------------------------------------------------------ drop-in context
+std::mutex g_drop_mu;
+ccsdt_job g_drop = nullptr;
+
+int drop_context(int ncor, int nocc, int nvir, const double *const *arrays, const int64_t *offs)
+{
+    if (g_drop && (g_drop->g.ncor != ncor || g_drop->g.nocc != nocc || g_drop->g.nvir != nvir)) {
+        job_free(g_drop);
+        g_drop = nullptr;
+    }
+    if (!g_drop) {
+        ccsdt_job_desc d;
+        d.ncor = ncor;
+        d.nocc = nocc;
+        d.nvir = nvir;
+        d.kchunk = 1;
+        // the first upload happens inside job_create from offset 0: point the
+        // descriptor at the slices directly
+        for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q] + (offs ? offs[q] : 0);
+        return job_create(&d, &g_drop);
+    }
+    return job_upload(g_drop, arrays, offs);
+}
+
+void drop_fail(const char *who)
+{
+    if (g_abort_policy.load()) {
+        fprintf(stderr, "libccsdt: %s failed: %s\n", who, t_msg);
+        fflush(stderr);
+        abort();
+    }
+}
+
+bool f_writeback()
+{
+    int v = g_f_writeback.load();
+    if (v < 0) {
+        const char *e = getenv("CCSDT_F_WRITEBACK");
+        v = (e && e[0] == '0') ? 0 : 1;
+        g_f_writeback.store(v);
+    }
+    return v != 0;
+}
+
+void body_impl(const char *who, double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
+               double *f3t, double *f4n, double *f4t, double *eorb, int *ncor_, int *nocc_,
+               int *nvir_, double *emp4_, double *emp5_, int *a_, int *i_, int *j_, int *k_,
+               int *klo_, double *tij, double *tkj, double *tia, double *tka, double *xia,
+               double *xka, double *jia, double *jka, double *kia, double *kka, double *jij,
+               double *jkj, double *kij, double *kkj, double *dintc1, double *dintx1,
+               double *t1v1, double *dintc2, double *dintx2, double *t1v2)
+{
+    int rc = CCSDT_OK;
+    double terms[4] = {0, 0, 0, 0};
+    if (!ncor_ || !nocc_ || !nvir_ || !a_ || !i_ || !j_ || !k_ || !klo_ || !emp4_ || !emp5_ ||
+        !eorb || !f1n || !f1t || !f2n || !f2t || !f3n || !f3t || !f4n || !f4t) {
+        rc = set_error(CCSDT_E_ARG, "null pointer argument");
+    } else {
+        const int ncor = *ncor_, nocc = *nocc_, nvir = *nvir_;
+        const int k = *k_, klo = *klo_;
+        if (ncor < 0 || nocc < 1 || nvir < 1 || klo < 1 || k < klo || k > nocc) {
+            rc = set_error(CCSDT_E_ARG, "invalid ncor=%d nocc=%d nvir=%d k=%d klo=%d", ncor, nocc,
+                           nvir, k, klo);
+        } else {
+            const int64_t vv = (int64_t)nvir * nvir, ov = (int64_t)nocc * nvir;
+            const int64_t s = k - klo;  // omp/ccsd_trpdrv_omp_cbody.c:72 slice offsets
+            const double *arr[CCSDT_NARRAYS] = {eorb, tij, tkj, tia, tka, xia, xka, jia, jka, kia,
+                                                kka, jij, jkj, kij, kkj, dintc1, dintx1, t1v1,
+                                                dintc2, dintx2, t1v2};
+            int64_t offs[CCSDT_NARRAYS] = {};
+            offs[CCSDT_TKJ] = offs[CCSDT_JKA] = offs[CCSDT_KKA] = s * vv;
+            offs[CCSDT_TKA] = offs[CCSDT_XKA] = offs[CCSDT_JKJ] = offs[CCSDT_KKJ] = s * ov;
+            for (int q = 0; q < CCSDT_NARRAYS && !rc; ++q)
+                if (!arr[q]) rc = set_error(CCSDT_E_ARG, "null array argument %d", q);
+            std::lock_guard<std::mutex> lk(g_drop_mu);
+            if (!rc) rc = drop_context(ncor, nocc, nvir, arr, offs);
+            if (!rc) {
+                // the uploaded slice is slice 0 of a kchunk=1 job: klo := k
+                const int trip[5] = {*a_, *i_, *j_, k, k};
+                double *fo[8] = {f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t};
+                ccsdt_run_opts o = {CCSDT_MODE_FP64, 0, 0, 0};
+                rc = job_run(g_drop, trip, 1, &o, nullptr, terms, f_writeback() ? fo : nullptr);
+            }
+        }
+    }
+    if (rc) {
+        drop_fail(who);
+        if (emp4_) *emp4_ = NAN;
+        if (emp5_) *emp5_ = NAN;
+        return;
+    }
+    // accumulation rule of omp/ccsd_trpdrv_omp_cbody.c:133-142
+    double emp4 = *emp4_, emp5 = *emp5_;
+    emp4 += terms[0];
+    emp5 += terms[1];
+    if (*i_ != *k_) {
+        emp4 += terms[2];
+        emp5 += terms[3];
+    }
+    *emp4_ = emp4;
+    *emp5_ = emp5;
+    t_err = CCSDT_OK;
+}
+
+void tengy_impl(const char *who, const double *f1n, const double *f1t, const double *f2n,
+                const double *f2t, const double *f3n, const double *f3t, const double *f4n,
+                const double *f4t, const double *dintc1, const double *dintx1, const double *t1v1,
+                const double *dintc2, const double *dintx2, const double *t1v2, const double *eorb,
+                double eaijk, double *emp4i, double *emp5i, double *emp4k, double *emp5k, int ncor,
+                int nocc, int nvir)
+{
+    int rc = CCSDT_OK;
+    double terms[4] = {0, 0, 0, 0};
+    const double *fs[8] = {f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t};
+    if (!emp4i || !emp5i || !emp4k || !emp5k || !eorb || !dintc1 || !dintx1 || !t1v1 || !dintc2 ||
+        !dintx2 || !t1v2)
+        rc = set_error(CCSDT_E_ARG, "null pointer argument");
+    for (int m = 0; m < 8 && !rc; ++m)
+        if (!fs[m]) rc = set_error(CCSDT_E_ARG, "null f matrix %d", m);
+    if (!rc && (ncor < 0 || nocc < 1 || nvir < 1))
+        rc = set_error(CCSDT_E_ARG, "invalid ncor=%d nocc=%d nvir=%d", ncor, nocc, nvir);
+    if (!rc) {
+        std::lock_guard<std::mutex> lk(g_drop_mu);
+        const int64_t vv = (int64_t)nvir * nvir, ov = (int64_t)nocc * nvir;
+        const double *v6[6] = {t1v1, t1v2, dintx1, dintx2, dintc1, dintc2};
+        if (g_drop && g_drop->g.ncor == ncor && g_drop->g.nocc == nocc && g_drop->g.nvir == nvir) {
+            rc = job_upload_vectors(g_drop, eorb, v6);
+        } else {
+            // A fresh context needs its T/J/K operands; the energy pass does not
+            // read them, so zeros stand in until a body call uploads real ones.
+            std::vector<double> zeros((size_t)std::max(vv, ov), 0.0);
+            const double *z = zeros.data();
+            const double *arr[CCSDT_NARRAYS] = {eorb, z, z, z, z, z, z, z, z, z, z, z, z, z, z,
+                                                dintc1, dintx1, t1v1, dintc2, dintx2, t1v2};
+            rc = drop_context(ncor, nocc, nvir, arr, nullptr);
+        }
+        ccsdt_job j = g_drop;
+        if (!rc) rc = grow(&j->d_f, &j->cap_f, 8 * vv);
+        if (!rc) rc = grow(&j->d_partial, &j->cap_partial, (int64_t)tengy_npairs(j->g) * 4);
+        if (!rc) rc = grow(&j->d_terms, &j->cap_terms, 4);
+        if (!rc) rc = ensure_tasks(j, 1);
+        if (!rc && !j->h_fstage &&
+            hipHostMalloc((void **)&j->h_fstage, (size_t)8 * vv * sizeof(double), 0) != hipSuccess) {
+            j->h_fstage = nullptr;
+            rc = set_error(CCSDT_E_NOMEM, "pinned staging allocation failed");
+        }
+        if (!rc && injected()) rc = set_error(CCSDT_E_INJECTED, "injected failure at launch");
+        if (!rc) {
+            auto run = [&]() -> int {
+                for (int m = 0; m < 8; ++m)
+                    memcpy(j->h_fstage + m * vv, fs[m], (size_t)vv * sizeof(double));
+                HIP_TRY(hipMemcpyAsync(j->d_f, j->h_fstage, (size_t)8 * vv * sizeof(double),
+                                       hipMemcpyHostToDevice, j->st));
+                j->h_tasks[0] = Task{0, 0, 0, 0, 0, 1, eaijk};
+                HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, sizeof(Task), hipMemcpyHostToDevice,
+                                       j->st));
+                HIP_TRY(launch_tengy(j->d_f, 8 * vv, j->g, j->d_tasks, 1, j->d_ev, j->d_vec6,
+                                     j->d_partial, j->st));
+                HIP_TRY(launch_reduce_terms(j->d_partial, tengy_npairs(j->g), 1, j->d_terms, j->st));
+                HIP_TRY(hipMemcpyAsync(j->h_out, j->d_terms, 4 * sizeof(double),
+                                       hipMemcpyDeviceToHost, j->st));
+                HIP_TRY(hipStreamSynchronize(j->st));
+                return CCSDT_OK;
+            };
+            rc = run();
+            if (!rc) memcpy(terms, j->h_out, sizeof terms);
+        }
+    }
+    if (rc) {
+        drop_fail(who);
+        if (emp4i) *emp4i = NAN;
+        if (emp5i) *emp5i = NAN;
+        if (emp4k) *emp4k = NAN;
+        if (emp5k) *emp5k = NAN;
+        return;
+    }
+    *emp4i = terms[0];
+    *emp5i = terms[1];
+    *emp4k = terms[2];
+    *emp5k = terms[3];
+    t_err = CCSDT_OK;
+}
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int ccsdt_last_error(void) { return t_err; }
+const char *ccsdt_last_error_string(void) { return t_msg; }
+void ccsdt_clear_error(void)
+{
+    t_err = CCSDT_OK;
+    t_msg[0] = 0;
+}
+void ccsdt_set_error_policy(int abort_on_error) { g_abort_policy.store(abort_on_error ? 1 : 0); }
+void ccsdt_debug_inject_failure(int nth) { g_inject.store(nth > 0 ? nth : 0); }
+const char *ccsdt_version(void) { return "ccsdt-mi355x 0.1 (gfx950, FP64 MFMA)"; }
+int ccsdt_init(int device) { return ensure_device(device); }
+void ccsdt_set_f_writeback(int on) { g_f_writeback.store(on ? 1 : 0); }
+
+static uint32_t pcg_step(uint64_t *state, uint64_t inc)
+{
+    const uint64_t old = *state;
+    *state = old * 6364136223846793005ULL + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+
+static void pcg_init(uint64_t *state, uint64_t *inc, uint64_t seed, uint64_t stream)
+{
+    *state = 0u;
+    *inc = (stream << 1u) | 1u;
+    pcg_step(state, *inc);
+    *state += seed;
+    pcg_step(state, *inc);
+}
+
+void ccsdt_pcg_raw(uint64_t seed, uint64_t stream, uint32_t *out, int64_t n)
+{
+    uint64_t s, inc;
+    pcg_init(&s, &inc, seed, stream);
+    for (int64_t q = 0; q < n; ++q) out[q] = pcg_step(&s, inc);
+}
+
+void ccsdt_pcg_uniform(uint64_t seed, uint64_t stream, double lo, double hi, double *out, int64_t n)
+{
+    uint64_t s, inc;
+    pcg_init(&s, &inc, seed, stream);
+    const double w = hi - lo;
+    for (int64_t q = 0; q < n; ++q) {
+        const uint32_t a = pcg_step(&s, inc) >> 5, b = pcg_step(&s, inc) >> 6;
+        out[q] = lo + w * (((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0));
+    }
+}
+
+int ccsdt_job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
+{
+    if (!out) return set_error(CCSDT_E_ARG, "null output handle");
+    int rc = job_create(desc, out);
+    if (!rc) t_err = CCSDT_OK;
+    return rc;
+}
+
+int ccsdt_job_run(ccsdt_job job, const int *triples, int64_t n, const ccsdt_run_opts *opts,
+                  ccsdt_run_result *res, double *terms, double *fout)
+{
+    double *fo[8];
+    if (fout && job) {
+        const int64_t vv = (int64_t)job->g.nvir * job->g.nvir;
+        for (int m = 0; m < 8; ++m) fo[m] = fout + m * vv;
+    }
+    int rc = job_run(job, triples, n, opts, res, terms, fout ? fo : nullptr);
+    if (!rc) t_err = CCSDT_OK;
+    return rc;
+}
+
+int ccsdt_job_destroy(ccsdt_job job)
+{
+    if (!job) return set_error(CCSDT_E_STATE, "null job");
+    job_free(job);
+    return CCSDT_OK;
+}
+
+int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples)
+{
+    if (nocc < 1 || nvir < 1 || kchunk < 1 || ncor < 0) return -1;
+    const Geom g = make_geom(ncor, nocc, nvir);
+    int64_t cnt[CCSDT_NARRAYS];
+    array_sizes(ncor, nocc, nvir, kchunk, cnt);
+    int64_t raw = 0;
+    for (int q = 0; q < CCSDT_NARRAYS; ++q) raw += round_up(cnt[q], 32);
+    const int64_t pan = (int64_t)(PAN_SLICE0 + 4 * kchunk) * g.vecsz;
+    const int64_t per_task = (int64_t)g.npairs * 4 + 4 + (int64_t)sizeof(Task) / 8;
+    return (raw + pan + 7 * g.nvp + ntriples * per_task) * (int64_t)sizeof(double);
+}
+
+int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256)
+{
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    double *d = nullptr;
+    if ((rc = dev_alloc((void **)&d, (64 + 64 + 256) * sizeof(double)))) return rc;
+    HIP_TRY(hipMemcpy(d, a64, 64 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d + 64, b64, 64 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(launch_mfma_probe(d, d + 64, d + 128, nullptr));
+    HIP_TRY(hipMemcpy(d256, d + 128, 256 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipFree(d));
+    return CCSDT_OK;
+}
+
+void ccsd_trpdrv_omp_cbody_(double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
+                            double *f3t, double *f4n, double *f4t, double *eorb, int *ncor,
+                            int *nocc, int *nvir, double *emp4, double *emp5, int *a, int *i,
+                            int *j, int *k, int *klo, double *tij, double *tkj, double *tia,
+                            double *tka, double *xia, double *xka, double *jia, double *jka,
+                            double *kia, double *kka, double *jij, double *jkj, double *kij,
+                            double *kkj, double *dintc1, double *dintx1, double *t1v1,
+                            double *dintc2, double *dintx2, double *t1v2)
+{
+    body_impl("ccsd_trpdrv_omp_cbody_", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, eorb, ncor, nocc,
+              nvir, emp4, emp5, a, i, j, k, klo, tij, tkj, tia, tka, xia, xka, jia, jka, kia, kka,
+              jij, jkj, kij, kkj, dintc1, dintx1, t1v1, dintc2, dintx2, t1v2);
+}
+
+void ccsd_trpdrv_cbody_(double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
+                        double *f3t, double *f4n, double *f4t, double *eorb, int *ncor, int *nocc,
+                        int *nvir, double *emp4, double *emp5, int *a, int *i, int *j, int *k,
+                        int *klo, double *tij, double *tkj, double *tia, double *tka, double *xia,
+                        double *xka, double *jia, double *jka, double *kia, double *kka,
+                        double *jij, double *jkj, double *kij, double *kkj, double *dintc1,
+                        double *dintx1, double *t1v1, double *dintc2, double *dintx2,
+                        double *t1v2)
+{
+    body_impl("ccsd_trpdrv_cbody_", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, eorb, ncor, nocc, nvir,
+              emp4, emp5, a, i, j, k, klo, tij, tkj, tia, tka, xia, xka, jia, jka, kia, kka, jij,
+              jkj, kij, kkj, dintc1, dintx1, t1v1, dintc2, dintx2, t1v2);
+}
+
+void ccsd_trpdrv_omp_fbody_(double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
+                            double *f3t, double *f4n, double *f4t, double *eorb, int *ncor,
+                            int *nocc, int *nvir, double *emp4, double *emp5, int *a, int *i,
+                            int *j, int *k, int *klo, double *tij, double *tkj, double *tia,
+                            double *tka, double *xia, double *xka, double *jia, double *jka,
+                            double *kia, double *kka, double *jij, double *jkj, double *kij,
+                            double *kkj, double *dintc1, double *dintx1, double *t1v1,
+                            double *dintc2, double *dintx2, double *t1v2)
+{
+    body_impl("ccsd_trpdrv_omp_fbody_", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, eorb, ncor, nocc,
+              nvir, emp4, emp5, a, i, j, k, klo, tij, tkj, tia, tka, xia, xka, jia, jka, kia, kka,
+              jij, jkj, kij, kkj, dintc1, dintx1, t1v1, dintc2, dintx2, t1v2);
+}
+
+void ccsd_tengy_omp(const double *f1n, const double *f1t, const double *f2n, const double *f2t,
+                    const double *f3n, const double *f3t, const double *f4n, const double *f4t,
+                    const double *dintc1, const double *dintx1, const double *t1v1,
+                    const double *dintc2, const double *dintx2, const double *t1v2,
+                    const double *eorb, const double eaijk, double *emp4i, double *emp5i,
+                    double *emp4k, double *emp5k, const int ncor, const int nocc, const int nvir)
+{
+    tengy_impl("ccsd_tengy_omp", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, dintc1, dintx1, t1v1,
+               dintc2, dintx2, t1v2, eorb, eaijk, emp4i, emp5i, emp4k, emp5k, ncor, nocc, nvir);
+}
+
+void ccsd_tengy(const double *f1n, const double *f1t, const double *f2n, const double *f2t,
+                const double *f3n, const double *f3t, const double *f4n, const double *f4t,
+                const double *dintc1, const double *dintx1, const double *t1v1,
+                const double *dintc2, const double *dintx2, const double *t1v2, const double *eorb,
+                const double eaijk, double *emp4i, double *emp5i, double *emp4k, double *emp5k,
+                const int ncor, const int nocc, const int nvir)
+{
+    tengy_impl("ccsd_tengy", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, dintc1, dintx1, t1v1, dintc2,
+               dintx2, t1v2, eorb, eaijk, emp4i, emp5i, emp4k, emp5k, ncor, nocc, nvir);
+}
+
+void ccsd_tengy_omp_(const double *f1n, const double *f1t, const double *f2n, const double *f2t,
+                     const double *f3n, const double *f3t, const double *f4n, const double *f4t,
+                     const double *dintc1, const double *dintc2, const double *dintx1,
+                     const double *dintx2, const double *t1v1, const double *t1v2,
+                     const double *eorb, const double *eaijk, double *emp4i, double *emp5i,
+                     double *emp4k, double *emp5k, const int *ncor, const int *nocc,
+                     const int *nvir)
+{
+    if (!eaijk || !ncor || !nocc || !nvir) {
+        set_error(CCSDT_E_ARG, "null scalar argument");
+        drop_fail("ccsd_tengy_omp_");
+        return;
+    }
+    tengy_impl("ccsd_tengy_omp_", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, dintc1, dintx1, t1v1,
+               dintc2, dintx2, t1v2, eorb, *eaijk, emp4i, emp5i, emp4k, emp5k, *ncor, *nocc, *nvir);
+}
+
+}  // extern "C"

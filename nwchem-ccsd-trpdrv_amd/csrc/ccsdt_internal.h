@@ -1,0 +1,63 @@
+// ccsdt_internal.h -- shared declarations between the HIP kernels and the
+// host C-ABI layer of libccsdt.so.  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ccsdt {
+
+// One (a,i,j,k) triple on the device.  Indices are 0-based; `slice` selects the
+// k-indexed operand slice (k-klo in the reference, omp/ccsd_trpdrv_omp_cbody.c:72).
+struct Task {
+    int a, i, j, k;
+    int slice;
+    int ik_differ;  // 1 when i != k: the k terms enter the totals
+    double eaijk;   // eorb[a] - (eorb[ncor+i] + eorb[ncor+j] + eorb[ncor+k])
+};
+
+// Shapes of one job after padding.
+struct Geom {
+    int ncor, nocc, nvir;
+    int nvp;     // nvir rounded up to 16 (MFMA block rows)
+    int nb;      // nvp / 16
+    int kdim;    // nvir + nocc: the concatenated contraction length
+    int kp;      // kdim rounded up to 8
+    int nc;      // kp / 8 contraction chunks
+    int npairs;  // nb*(nb+1)/2 lower-triangle 16x16 block pairs
+    int64_t vecsz;  // doubles in one packed panel: nvp * kp
+};
+
+// Packed-panel descriptor: panel(x, kk) for x < nvir, kk < nvir + nocc is
+//   kk <  nvir : first_t ? P[kk + x*nvir]  : P[x + kk*nvir]
+//   kk >= nvir : sgn * (second_t ? Q[l + x*nocc] : Q[x + l*nvir]),  l = kk - nvir
+// and 0 in the padding.
+struct PanelDesc {
+    const double *P;
+    const double *Q;
+    double *out;
+    int first_t, second_t;
+    double sgn;
+};
+
+// Panel order inside a job's packed buffer.
+enum { PAN_LJ = 0, PAN_LK = 1, PAN_R1T = 2, PAN_R3T = 3, PAN_SLICE0 = 4 };
+// Per k-slice s the four panels start at PAN_SLICE0 + 4*s.
+enum { SL_R1 = 0, SL_R3 = 1, SL_LJT = 2, SL_LKT = 3 };
+
+// Launchers (ccsdt_kernels.hip).  All enqueue on `st` and return hipError_t.
+hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st);
+hipError_t launch_fused(const double *pan, const Geom &g, const Task *tasks, int ntasks,
+                        const short2 *pairs, const double *ev, const double *vec6,
+                        double *partial, double *fout, int64_t fstride, bool tengy,
+                        hipStream_t st);
+hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
+                        int ntasks, const double *ev, const double *vec6, double *partial,
+                        hipStream_t st);
+int tengy_npairs(const Geom &g);
+hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,
+                               hipStream_t st);
+hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks, double *out2,
+                              hipStream_t st);
+hipError_t launch_mfma_probe(const double *a, const double *b, double *d, hipStream_t st);
+
+}  // namespace ccsdt

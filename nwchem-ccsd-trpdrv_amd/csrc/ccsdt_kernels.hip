@@ -1,0 +1,455 @@
+// ccsdt_kernels.hip -- gfx950 kernels of the CCSD(T) triples body.
+//
+// The sixteen DGEMMs of one (a,i,j,k) triple (omp/ccsd_trpdrv_omp_cbody.c:71-116)
+// are regrouped as dot products of length K = nvir + nocc between eight packed
+// operand vectors per virtual index x:
+//   LJ(x)  = [Jia(x,:)  Tia(x,:)]      R1(x)  = [Tkj(x,:)  -Kkj(:,x)]
+//   LK(x)  = [Kia(x,:)  Xia(x,:)]      R3(x)  = [Tkj(:,x)  -Jkj(:,x)]
+//   LJt(x) = [Jka(x,:)  Tka(x,:)]      R1t(x) = [Tij(x,:)  -Kij(:,x)]
+//   LKt(x) = [Kka(x,:)  Xka(x,:)]      R3t(x) = [Tij(:,x)  -Jij(:,x)]
+// so that f1n(b,c) = LJ(b).R1(c), f2n = LK.R1, f3n = LJ.R3, f4n = LK.R3 and
+// f1t = LJt.R1t, f2t = LKt.R1t, f3t = LJt.R3t, f4t = LKt.R3t.  The energy
+// reduction (omp/ccsd_tengy_c99.c:65-111) at (b,c) needs every f at (b,c) and
+// at (c,b); a wave that owns the 16x16 block pair (B,C) therefore computes the
+// eight products at (b,c) and the eight mirrored ones f(c,b) = L(c).R(b) with
+// the operand roles swapped, which puts all sixteen values of a (b,c) point in
+// the same lane and register, and finishes the energy terms in registers.
+//
+// Panels are stored pre-swizzled for v_mfma_f64_16x16x4_f64: per 16-row block
+// xb and 8-wide contraction chunk c, lane l holds a double2 with
+//   value(x = 16*xb + l%16, kk = 8*c + 2*(l/16) + s),  s = 0, 1
+// and the MFMA of step s consumes element s.  A and B use the same k mapping,
+// so every chunk contributes k = 8c..8c+7 exactly once.
+#include "ccsdt_internal.h"
+
+namespace ccsdt {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+struct Sum4 {
+    double e4i, e5i, e4k, e5k;
+};
+
+// Energy summand of one (b,c) point.  n*, t* are f?n, f?t at (b,c); nr*, tr*
+// at (c,b); d = -1/(e_b + e_c + eaijk); u = t1v(b); x = dintx(c); y = dintc(c).
+// Contraction is off so the arithmetic follows omp/ccsd_tengy_c99.c:91-111
+// operation for operation.
+__device__ __forceinline__ void tengy_point(double n1, double n2, double n3, double n4,
+                                            double t1, double t2, double t3, double t4,
+                                            double nr1, double nr2, double nr3, double nr4,
+                                            double tr1, double tr2, double tr3, double tr4,
+                                            double d, double u1, double u2, double x1,
+                                            double x2, double y1, double y2, Sum4 &s)
+{
+#pragma clang fp contract(off)
+    s.e4i += d * (t1 + nr1 + tr2 + n3 + nr4) * (t1 - t2 * 2 - t3 * 2 + t4)
+           - d * (n1 + tr1 + nr2 + nr3) * (t1 * 2 - t2 - t3 + t4 * 2)
+           + d * 3 * (n1 * (n1 + nr3 + tr4 * 2) + n2 * tr2 + n3 * t4);
+    s.e4k += d * (n1 + tr1 + nr2 + t3 + tr4) * (n1 - n2 * 2 - n3 * 2 + n4)
+           - d * (t1 + nr1 + tr2 + tr3) * (n1 * 2 - n2 - n3 + n4 * 2)
+           + d * 3 * (t1 * (t1 + tr3 + nr4 * 2) + t2 * nr2 + t3 * n4);
+    s.e5i += d * u1 * x1 * (t1 + n2 + nr4 - (t3 + n4 + nr2 + n1 + t2 + nr3) * 2
+                            + (n3 + t4 + nr1) * 4)
+           + d * u1 * y1 * (n1 + n4 + tr1 - (n2 + n3 + tr2) * 2);
+    s.e5k += d * u2 * x2 * (n1 + t2 + tr4 - (n3 + t4 + tr2 + t1 + n2 + tr3) * 2
+                            + (t3 + n4 + tr1) * 4)
+           + d * u2 * y2 * (t1 + t4 + nr1 - (t2 + t3 + nr2) * 2);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Spreads consecutive logical tiles over one XCD: the hardware hands block b to
+// XCD b % 8, so logical tile L = (b%8)*q + min(b%8, r) + b/8 keeps a task's
+// tiles (which share operand panels) in the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int total)
+{
+    const int q = total >> 3, r = total & 7;
+    const int x = bid & 7, idx = bid >> 3;
+    return x * q + min(x, r) + idx;
+}
+
+// ------------------------------------------------------------------ packing
+__global__ __launch_bounds__(256) void k_pack(const PanelDesc *__restrict__ descs, Geom g)
+{
+    const PanelDesc d = descs[blockIdx.y];
+    const int nv = g.nvir, no = g.nocc;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < g.vecsz;
+         idx += (int64_t)gridDim.x * 256) {
+        const int s = (int)(idx & 1);
+        const int l = (int)((idx >> 1) & 63);
+        const int64_t rest = idx >> 7;
+        const int c = (int)(rest % g.nc);
+        const int xb = (int)(rest / g.nc);
+        const int x = xb * 16 + (l & 15);
+        const int kk = c * 8 + (l >> 4) * 2 + s;
+        double v = 0.0;
+        if (x < nv) {
+            if (kk < nv) {
+                v = d.first_t ? d.P[kk + (int64_t)x * nv] : d.P[x + (int64_t)kk * nv];
+            } else if (kk < nv + no) {
+                const int l2 = kk - nv;
+                v = d.sgn * (d.second_t ? d.Q[l2 + (int64_t)x * no] : d.Q[x + (int64_t)l2 * nv]);
+            }
+        }
+        d.out[idx] = v;
+    }
+}
+
+hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st)
+{
+    int64_t blocks = (g.vecsz + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)blocks, npanels), dim3(256), 0, st, d_desc, g);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ fused GEMM + energy
+#define MF(p, ia, ib, s) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia][s], b[ib][s], acc[p], 0, 0, 0)
+
+template <bool TENGY, bool WRITEF>
+__global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Geom g,
+                                              const Task *__restrict__ tasks, int ntasks,
+                                              const short2 *__restrict__ pairs,
+                                              const double *__restrict__ ev,
+                                              const double *__restrict__ vec6,
+                                              double *__restrict__ partial,
+                                              double *__restrict__ fout, int64_t fstride)
+{
+    const int total = ntasks * g.npairs;
+    const int L = xcd_remap(blockIdx.x, total);
+    const int t = L / g.npairs;
+    const int p = L - t * g.npairs;
+    const Task tk = tasks[t];
+    const short2 pr = pairs[p];
+    const int bi = pr.x, ci = pr.y;
+    const int lane = threadIdx.x;
+
+    const int64_t vs = g.vecsz;
+    const int64_t blk = (int64_t)g.nc * 128;
+    const double *sl = pan + (int64_t)(PAN_SLICE0 + 4 * tk.slice) * vs;
+    const double *ab = pan + bi * blk + lane * 2;   // panels at the b block
+    const double *sb = sl + bi * blk + lane * 2;
+    const double *ac = pan + ci * blk + lane * 2;   // panels at the c block
+    const double *sc = sl + ci * blk + lane * 2;
+    // A side, index b: LJ, LK, LJt, LKt, R1, R3, R1t, R3t
+    const double *A[8] = {ab + PAN_LJ * vs, ab + PAN_LK * vs, sb + SL_LJT * vs, sb + SL_LKT * vs,
+                          sb + SL_R1 * vs, sb + SL_R3 * vs, ab + PAN_R1T * vs, ab + PAN_R3T * vs};
+    // B side, index c: R1, R3, R1t, R3t, LJ, LK, LJt, LKt
+    const double *B[8] = {sc + SL_R1 * vs, sc + SL_R3 * vs, ac + PAN_R1T * vs, ac + PAN_R3T * vs,
+                          ac + PAN_LJ * vs, ac + PAN_LK * vs, sc + SL_LJT * vs, sc + SL_LKT * vs};
+
+    dbl4 acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    dbl2 a[8], b[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        a[v] = *reinterpret_cast<const dbl2 *>(A[v]);
+        b[v] = *reinterpret_cast<const dbl2 *>(B[v]);
+    }
+    const int nc = g.nc;
+    for (int c = 0; c < nc; ++c) {
+        const int cn = (c + 1 < nc) ? c + 1 : c;  // clamped prefetch, always in bounds
+        const int64_t on = (int64_t)cn * 128;
+        dbl2 an[8], bn[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            an[v] = *reinterpret_cast<const dbl2 *>(A[v] + on);
+            bn[v] = *reinterpret_cast<const dbl2 *>(B[v] + on);
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            MF(0, 0, 0, s);  MF(1, 1, 0, s);  MF(2, 0, 1, s);  MF(3, 1, 1, s);    // f1n f2n f3n f4n (b,c)
+            MF(4, 2, 2, s);  MF(5, 3, 2, s);  MF(6, 2, 3, s);  MF(7, 3, 3, s);    // f1t f2t f3t f4t (b,c)
+            MF(8, 4, 4, s);  MF(9, 4, 5, s);  MF(10, 5, 4, s); MF(11, 5, 5, s);   // f1n..f4n (c,b)
+            MF(12, 6, 6, s); MF(13, 6, 7, s); MF(14, 7, 6, s); MF(15, 7, 7, s);   // f1t..f4t (c,b)
+        }
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            a[v] = an[v];
+            b[v] = bn[v];
+        }
+    }
+
+    // v_mfma_f64_16x16x4_f64 result: lane l, element r holds row 4*(l/16)+r, column l%16.
+    const int nv = g.nvir, nvp = g.nvp;
+    const int col = lane & 15, row0 = 4 * (lane >> 4);
+    const int cx = ci * 16 + col;
+    if (WRITEF) {
+        double *fo = fout + (int64_t)t * fstride;
+        const int64_t vv = (int64_t)nv * nv;
+        const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};  // product -> body order f1n,f1t,...
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int bx = bi * 16 + row0 + r;
+            if (bx < nv && cx < nv) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) fo[mat[q] * vv + bx + (int64_t)cx * nv] = acc[q][r];
+                if (bi != ci) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) fo[mat[q] * vv + cx + (int64_t)bx * nv] = acc[8 + q][r];
+                }
+            }
+        }
+    }
+    if (TENGY) {
+        Sum4 s = {0.0, 0.0, 0.0, 0.0};
+        const double ec = ev[cx];
+        const double u1c = vec6[cx], u2c = vec6[nvp + cx];
+        const double x1c = vec6[2 * nvp + cx], x2c = vec6[3 * nvp + cx];
+        const double y1c = vec6[4 * nvp + cx], y2c = vec6[5 * nvp + cx];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int bx = bi * 16 + row0 + r;
+            if (bx < nv && cx < nv) {
+                const double d = -1.0 / (ev[bx] + ec + tk.eaijk);
+                const double u1b = vec6[bx], u2b = vec6[nvp + bx];
+                tengy_point(acc[0][r], acc[1][r], acc[2][r], acc[3][r],
+                            acc[4][r], acc[5][r], acc[6][r], acc[7][r],
+                            acc[8][r], acc[9][r], acc[10][r], acc[11][r],
+                            acc[12][r], acc[13][r], acc[14][r], acc[15][r],
+                            d, u1b, u2b, x1c, x2c, y1c, y2c, s);
+                if (bi != ci) {  // the (c,b) point of the same pair of blocks
+                    const double x1b = vec6[2 * nvp + bx], x2b = vec6[3 * nvp + bx];
+                    const double y1b = vec6[4 * nvp + bx], y2b = vec6[5 * nvp + bx];
+                    tengy_point(acc[8][r], acc[9][r], acc[10][r], acc[11][r],
+                                acc[12][r], acc[13][r], acc[14][r], acc[15][r],
+                                acc[0][r], acc[1][r], acc[2][r], acc[3][r],
+                                acc[4][r], acc[5][r], acc[6][r], acc[7][r],
+                                d, u1c, u2c, x1b, x2b, y1b, y2b, s);
+                }
+            }
+        }
+        s.e4i = wave_sum(s.e4i);
+        s.e5i = wave_sum(s.e5i);
+        s.e4k = wave_sum(s.e4k);
+        s.e5k = wave_sum(s.e5k);
+        if (lane == 0) {
+            double *o = partial + (int64_t)L * 4;
+            o[0] = s.e4i; o[1] = s.e5i; o[2] = s.e4k; o[3] = s.e5k;
+        }
+    }
+}
+#undef MF
+
+hipError_t launch_fused(const double *pan, const Geom &g, const Task *tasks, int ntasks,
+                        const short2 *pairs, const double *ev, const double *vec6,
+                        double *partial, double *fout, int64_t fstride, bool tengy,
+                        hipStream_t st)
+{
+    const int64_t total = (int64_t)ntasks * g.npairs;
+    if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
+    dim3 grid((unsigned)total), block(64);
+    if (tengy && fout)
+        hipLaunchKernelGGL((k_fused<true, true>), grid, block, 0, st, pan, g, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    else if (tengy)
+        hipLaunchKernelGGL((k_fused<true, false>), grid, block, 0, st, pan, g, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    else if (fout)
+        hipLaunchKernelGGL((k_fused<false, true>), grid, block, 0, st, pan, g, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ standalone energy pass
+// Reads the eight f matrices of each task from HBM (ld = nvir, body order
+// f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t) in 32x32 tile pairs (BI >= CI).  The direct
+// tile is read coalesced along b; the mirrored tile f(c,b) is read coalesced
+// along c into LDS and transposed there.
+__host__ __device__ inline int tengy_tiles(int nv) { return (nv + 31) / 32; }
+
+int tengy_npairs(const Geom &g)
+{
+    const int nt = tengy_tiles(g.nvir);
+    return nt * (nt + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int64_t fstride,
+                                               Geom g, const Task *__restrict__ tasks,
+                                               int ntasks, const double *__restrict__ ev,
+                                               const double *__restrict__ vec6,
+                                               double *__restrict__ partial)
+{
+    __shared__ double tile[32][33];
+    __shared__ double red[4][4];
+    const int nv = g.nvir, nvp = g.nvp;
+    const int nt = tengy_tiles(nv);
+    const int np = nt * (nt + 1) / 2;
+    const int L = xcd_remap(blockIdx.x, ntasks * np);
+    const int t = L / np;
+    int p = L - t * np;
+    int BI = 0;
+    while ((BI + 1) * (BI + 2) / 2 <= p) ++BI;
+    const int CI = p - BI * (BI + 1) / 2;
+    const Task tk = tasks[t];
+    const double *ft = f + (int64_t)t * fstride;
+    const int64_t vv = (int64_t)nv * nv;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int bx = BI * 32 + tx;
+
+    double dv[8][4], mv[8][4];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const double *fm = ft + m * vv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int cx = CI * 32 + ty + 8 * q;
+            dv[m][q] = (bx < nv && cx < nv) ? fm[bx + (int64_t)cx * nv] : 0.0;
+            const int cc = CI * 32 + tx, bb = BI * 32 + ty + 8 * q;
+            tile[ty + 8 * q][tx] = (cc < nv && bb < nv) ? fm[cc + (int64_t)bb * nv] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mv[m][q] = tile[tx][ty + 8 * q];
+        __syncthreads();
+    }
+    Sum4 s = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int cx = CI * 32 + ty + 8 * q;
+        if (bx < nv && cx < nv) {
+            const double d = -1.0 / (ev[bx] + ev[cx] + tk.eaijk);
+            // body order: 0 f1n, 1 f1t, 2 f2n, 3 f2t, 4 f3n, 5 f3t, 6 f4n, 7 f4t
+            tengy_point(dv[0][q], dv[2][q], dv[4][q], dv[6][q], dv[1][q], dv[3][q], dv[5][q], dv[7][q],
+                        mv[0][q], mv[2][q], mv[4][q], mv[6][q], mv[1][q], mv[3][q], mv[5][q], mv[7][q],
+                        d, vec6[bx], vec6[nvp + bx], vec6[2 * nvp + cx], vec6[3 * nvp + cx],
+                        vec6[4 * nvp + cx], vec6[5 * nvp + cx], s);
+            if (BI != CI)
+                tengy_point(mv[0][q], mv[2][q], mv[4][q], mv[6][q], mv[1][q], mv[3][q], mv[5][q], mv[7][q],
+                            dv[0][q], dv[2][q], dv[4][q], dv[6][q], dv[1][q], dv[3][q], dv[5][q], dv[7][q],
+                            d, vec6[cx], vec6[nvp + cx], vec6[2 * nvp + bx], vec6[3 * nvp + bx],
+                            vec6[4 * nvp + bx], vec6[5 * nvp + bx], s);
+        }
+    }
+    s.e4i = wave_sum(s.e4i);
+    s.e5i = wave_sum(s.e5i);
+    s.e4k = wave_sum(s.e4k);
+    s.e5k = wave_sum(s.e5k);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = s.e4i; red[w][1] = s.e5i; red[w][2] = s.e4k; red[w][3] = s.e5k;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        partial[(int64_t)L * 4 + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    }
+}
+
+hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
+                        int ntasks, const double *ev, const double *vec6, double *partial,
+                        hipStream_t st)
+{
+    const int64_t total = (int64_t)ntasks * tengy_npairs(g);
+    if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tengy, dim3((unsigned)total), dim3(256), 0, st, f, fstride, g, tasks,
+                       ntasks, ev, vec6, partial);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ reductions
+// Per task: sum of the tile partials in a fixed order (lane-strided, then a
+// butterfly), so the result does not depend on scheduling.
+__global__ __launch_bounds__(64) void k_reduce_terms(const double *__restrict__ partial,
+                                                     int npairs, double *__restrict__ terms)
+{
+    const int t = blockIdx.x, lane = threadIdx.x;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const double *pp = partial + (int64_t)t * npairs * 4;
+    for (int p = lane; p < npairs; p += 64) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] += pp[p * 4 + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) terms[(int64_t)t * 4 + q] = s[q];
+    }
+}
+
+hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,
+                               hipStream_t st)
+{
+    if (ntasks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reduce_terms, dim3(ntasks), dim3(64), 0, st, partial, npairs, terms);
+    return hipGetLastError();
+}
+
+// Neumaier compensated addition.
+__device__ __forceinline__ void neu_add(double &sum, double &comp, double x)
+{
+    const double t = sum + x;
+    comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+}
+
+// Totals over the list with the accumulation rule of
+// omp/ccsd_trpdrv_omp_cbody.c:133-139 (k terms only when i != k), compensated.
+__global__ __launch_bounds__(256) void k_reduce_list(const double *__restrict__ terms,
+                                                     const Task *__restrict__ tasks, int ntasks,
+                                                     double *__restrict__ out2)
+{
+    __shared__ double sh[256][4];
+    const int tid = threadIdx.x;
+    const int chunk = (ntasks + 255) / 256;
+    const int lo = tid * chunk, hi = min(ntasks, lo + chunk);
+    double s4 = 0.0, c4 = 0.0, s5 = 0.0, c5 = 0.0;
+    for (int t = lo; t < hi; ++t) {
+        const double *tm = terms + (int64_t)t * 4;
+        neu_add(s4, c4, tm[0]);
+        neu_add(s5, c5, tm[1]);
+        if (tasks[t].ik_differ) {
+            neu_add(s4, c4, tm[2]);
+            neu_add(s5, c5, tm[3]);
+        }
+    }
+    sh[tid][0] = s4; sh[tid][1] = c4; sh[tid][2] = s5; sh[tid][3] = c5;
+    __syncthreads();
+    if (tid == 0) {
+        double S4 = 0.0, C4 = 0.0, S5 = 0.0, C5 = 0.0;
+        for (int q = 0; q < 256; ++q) {
+            neu_add(S4, C4, sh[q][0]);
+            neu_add(S4, C4, sh[q][1]);
+            neu_add(S5, C5, sh[q][2]);
+            neu_add(S5, C5, sh[q][3]);
+        }
+        out2[0] = S4 + C4;
+        out2[1] = S5 + C5;
+    }
+}
+
+hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks, double *out2,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_list, dim3(1), dim3(256), 0, st, terms, tasks, ntasks, out2);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ layout probe
+// One v_mfma_f64_16x16x4_f64 on A (16x4, row major) and B (4x16, row major),
+// fed as lane l: A[l%16][l/16], B[l/16][l%16]; d[4*l + r] is the lane's result.
+__global__ __launch_bounds__(64) void k_mfma_probe(const double *a, const double *b, double *d)
+{
+    const int l = threadIdx.x;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(l & 15) * 4 + (l >> 4)], b[(l >> 4) * 16 + (l & 15)],
+                                              acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
+}
+
+hipError_t launch_mfma_probe(const double *a, const double *b, double *d, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, a, b, d);
+    return hipGetLastError();
+}
+
+}  // namespace ccsdt

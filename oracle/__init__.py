@@ -1,0 +1,199 @@
+"""ctypes bindings of the CPU oracle (liboracle.so) and of the reference-built
+checkers under oracle/_ref/.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg; never by the product package.
+Functions cite the reference lines they restate in ccsdt_oracle.c.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+ARRAY_NAMES = ["eorb", "tij", "tkj", "tia", "tka", "xia", "xka", "jia", "jka", "kia",
+               "kka", "jij", "jkj", "kij", "kkj", "dintc1", "dintx1", "t1v1", "dintc2",
+               "dintx2", "t1v2"]
+VEC_NAMES = ["dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2"]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_L = None
+
+
+def olib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_pcg_raw.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _vp, ctypes.c_long]
+        L.orc_pcg_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_double, _vp, ctypes.c_long]
+        L.orc_blas_select.argtypes = [ctypes.c_int]
+        L.orc_blas_select.restype = ctypes.c_int
+        L.orc_blas_name.restype = ctypes.c_char_p
+        L.orc_fmats.argtypes = [_dp] * 8 + [ctypes.c_int] * 4 + [_dp] * 14
+        L.orc_tengy.argtypes = [_dp] * 15 + [ctypes.c_double] + [_dp] * 4 + [ctypes.c_int] * 4
+        L.orc_tengy_abs.argtypes = [_dp] * 15 + [ctypes.c_double, _dp] + [ctypes.c_int] * 3
+        L.orc_eaijk.argtypes = [_dp] + [ctypes.c_int] * 5
+        L.orc_eaijk.restype = ctypes.c_double
+        L.orc_run_list.argtypes = [ctypes.c_int] * 3 + [_vp, _vp, ctypes.c_long, _vp, _vp, _vp,
+                                                       _vp, ctypes.c_double, ctypes.c_int]
+        L.orc_run_list.restype = ctypes.c_long
+        L.orc_max_threads.restype = ctypes.c_int
+        _L = L
+    return _L
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _v(a):
+    return a.ctypes.data_as(_vp)
+
+
+def blas_select(which=0):
+    """0 auto (MKL if the image has it), 1 the oracle's own loop, 2 MKL only."""
+    return olib().orc_blas_select(which)
+
+
+def blas_name():
+    return olib().orc_blas_name().decode()
+
+
+def max_threads():
+    return olib().orc_max_threads()
+
+
+def pcg_raw(seed, stream, n):
+    out = np.empty(n, dtype=np.uint32)
+    olib().orc_pcg_raw(seed, stream, _v(out), n)
+    return out
+
+
+def pcg_uniform(seed, stream, lo, hi, n):
+    out = np.empty(n)
+    olib().orc_pcg_uniform(seed, stream, lo, hi, _v(out), n)
+    return out
+
+
+def _arr(arrays, name):
+    return np.ascontiguousarray(arrays[name], dtype=np.float64)
+
+
+def fmats(arrays, nocc, nvir, k, klo):
+    """The eight f matrices of one triple, body order, each f[m][b, c] (row b)."""
+    f = np.zeros((8, nvir, nvir))  # stored column major: f[m] viewed as [c][b]
+    A = {n: _arr(arrays, n) for n in ARRAY_NAMES}
+    olib().orc_fmats(*[_p(f[m]) for m in range(8)], nocc, nvir, k, klo,
+                     *[_p(A[n]) for n in ARRAY_NAMES[1:15]])
+    return np.ascontiguousarray(np.swapaxes(f, 1, 2))
+
+
+def eaijk(eorb, ncor, a, i, j, k):
+    return olib().orc_eaijk(_p(np.ascontiguousarray(eorb)), ncor, a, i, j, k)
+
+
+def tengy(fcm, arrays, eaijk_, ncor, nocc, nvir, nthreads=1):
+    """Energy reduction; fcm = eight column-major f buffers (flat or [c][b])."""
+    out = [ctypes.c_double() for _ in range(4)]
+    fs = [np.ascontiguousarray(x, dtype=np.float64) for x in fcm]
+    v = [_arr(arrays, n) for n in VEC_NAMES]
+    olib().orc_tengy(*[_p(x) for x in fs], *[_p(x) for x in v], _p(_arr(arrays, "eorb")),
+                     eaijk_, *[ctypes.byref(o) for o in out], ncor, nocc, nvir, nthreads)
+    return tuple(o.value for o in out)
+
+
+def run_list(arrays, ncor, nocc, nvir, triples, with_abs=False, max_seconds=0.0, nthreads=0):
+    """Runs a triple list through the oracle body.  Returns a dict with
+    terms (n,4), abs4 (n,4) if requested, secs (n,), plain and compensated
+    totals, and done (triples run before max_seconds)."""
+    trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
+    n = trip.shape[0]
+    A = [_arr(arrays, nm) for nm in ARRAY_NAMES]
+    ptrs = (ctypes.c_void_p * len(A))(*[a.ctypes.data for a in A])
+    terms = np.zeros((n, 4))
+    abs4 = np.zeros((n, 4)) if with_abs else None
+    secs = np.zeros(n)
+    tot = np.zeros(4)
+    done = olib().orc_run_list(ncor, nocc, nvir, ctypes.cast(ptrs, _vp), _v(trip), n,
+                               _v(terms), _v(abs4) if with_abs else None, _v(secs), _v(tot),
+                               max_seconds, nthreads)
+    return {"terms": terms[:done], "abs4": abs4[:done] if with_abs else None,
+            "secs": secs[:done], "emp4_plain": tot[0], "emp5_plain": tot[1],
+            "emp4": tot[2], "emp5": tot[3], "done": done}
+
+
+# ------------------------------------------------------------------ reference builds
+def _ref(name):
+    path = os.path.join(REF_DIR, name)
+    return ctypes.CDLL(path) if os.path.exists(path) else None
+
+
+def ref_tengy_lib():
+    """omp/ccsd_tengy_c99.c compiled as is (ccsd_tengy_omp), or None."""
+    L = _ref("libref_tengy_c99.so")
+    if L is not None:
+        L.ccsd_tengy_omp.argtypes = [_dp] * 15 + [ctypes.c_double] + [_dp] * 4 + [ctypes.c_int] * 3
+        L.ccsd_tengy_omp.restype = None
+    return L
+
+
+def ref_cbody_skipdgemm_lib():
+    """omp/ccsd_trpdrv_omp_cbody.c with -DSKIP_DGEMM + tengy, or None."""
+    L = _ref("libref_cbody_skipdgemm.so")
+    if L is not None:
+        L.ccsd_trpdrv_omp_cbody_.argtypes = [_dp] * 9 + [_ip] * 3 + [_dp] * 2 + [_ip] * 5 + [_dp] * 20
+        L.ccsd_trpdrv_omp_cbody_.restype = None
+    return L
+
+
+def ref_tengy_f77_lib():
+    """omp/ccsd_tengy_f77.f (ccsd_tengy_omp_, Fortran order), or None."""
+    L = _ref("libref_tengy_f77.so")
+    if L is not None:
+        L.ccsd_tengy_omp_.argtypes = [_dp] * 20 + [_ip] * 3
+        L.ccsd_tengy_omp_.restype = None
+    return L
+
+
+def ref_tengy(L, fcm, arrays, eaijk_, ncor, nocc, nvir):
+    out = [ctypes.c_double() for _ in range(4)]
+    fs = [np.ascontiguousarray(x, dtype=np.float64) for x in fcm]
+    v = [_arr(arrays, n) for n in VEC_NAMES]
+    L.ccsd_tengy_omp(*[_p(x) for x in fs], *[_p(x) for x in v], _p(_arr(arrays, "eorb")),
+                     eaijk_, *[ctypes.byref(o) for o in out], ncor, nocc, nvir)
+    return tuple(o.value for o in out)
+
+
+def ref_tengy_f77(L, fcm, arrays, eaijk_, ncor, nocc, nvir):
+    out = [ctypes.c_double() for _ in range(4)]
+    fs = [np.ascontiguousarray(x, dtype=np.float64) for x in fcm]
+    g = {n: _arr(arrays, n) for n in VEC_NAMES}
+    ea = ctypes.c_double(eaijk_)
+    sz = [ctypes.c_int(x) for x in (ncor, nocc, nvir)]
+    L.ccsd_tengy_omp_(*[_p(x) for x in fs], _p(g["dintc1"]), _p(g["dintc2"]), _p(g["dintx1"]),
+                      _p(g["dintx2"]), _p(g["t1v1"]), _p(g["t1v2"]), _p(_arr(arrays, "eorb")),
+                      ctypes.byref(ea), *[ctypes.byref(o) for o in out],
+                      *[ctypes.byref(s) for s in sz])
+    return tuple(o.value for o in out)
+
+
+def ref_cbody_glue(L, fcm, arrays, ncor, nocc, nvir, emp4, emp5, a, i, j, k, klo):
+    """Runs the SKIP_DGEMM reference body on given f buffers (left untouched):
+    eaijk + tengy + the i!=k accumulation.  Returns (emp4, emp5)."""
+    fs = [np.ascontiguousarray(x, dtype=np.float64).copy() for x in fcm]
+    ints = [ctypes.c_int(v) for v in (ncor, nocc, nvir, a, i, j, k, klo)]
+    e4, e5 = ctypes.c_double(emp4), ctypes.c_double(emp5)
+    A = [_arr(arrays, n) for n in ARRAY_NAMES[1:]]
+    L.ccsd_trpdrv_omp_cbody_(*[_p(x) for x in fs], _p(_arr(arrays, "eorb")),
+                             ctypes.byref(ints[0]), ctypes.byref(ints[1]), ctypes.byref(ints[2]),
+                             ctypes.byref(e4), ctypes.byref(e5),
+                             *[ctypes.byref(v) for v in ints[3:]], *[_p(x) for x in A])
+    return e4.value, e5.value

@@ -1,0 +1,117 @@
+"""C-ABI surface and host logic, without a GPU.
+
+The library must load, export every function include/ccsdt.h declares, and
+fail loudly (an error code, never a silent CPU result) when no gfx950 device is
+present.  Also covers the input generator and the triple lists of the
+workload configurations.
+"""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "ccsdt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \t\*]*?\b(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "defined")))
+
+
+def test_header_symbols_exported(cc):
+    L = cc.lib()
+    names = header_functions()
+    assert "ccsd_trpdrv_omp_cbody_" in names and "ccsd_tengy_omp" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert len(names) >= 20
+
+
+def test_version(cc):
+    assert b"gfx950" in cc.lib().ccsdt_version()
+
+
+def _has_gpu(cc):
+    return cc.lib().ccsdt_init(0) == 0
+
+
+def test_no_device_is_an_error(cc):
+    if _has_gpu(cc):
+        pytest.skip("a GPU is present")
+    L = cc.lib()
+    assert L.ccsdt_init(0) == -2
+    assert "device" in L.ccsdt_last_error_string().decode().lower()
+
+
+def test_dropin_without_device_reports_nan(cc):
+    if _has_gpu(cc):
+        pytest.skip("a GPU is present")
+    L = cc.lib()
+    L.ccsdt_set_error_policy(0)
+    nocc, nvir = 3, 8
+    arrays = cc.make_inputs(nocc, nvir, 1, 1)
+    f = [np.zeros(nvir * nvir) for _ in range(8)]
+    e4, e5 = cc.cbody(f, arrays["eorb"], 0, nocc, nvir, 0.0, 0.0, nocc + 1, 1, 1, 1, 1, arrays)
+    assert math.isnan(e4) and math.isnan(e5)
+    assert L.ccsdt_last_error() == -2
+    vec = {n: arrays[n] for n in ("dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2")}
+    out = cc.tengy(f, vec, arrays["eorb"], 1.0, 0, nocc, nvir)
+    assert all(math.isnan(x) for x in out)
+    L.ccsdt_set_error_policy(1)
+
+
+def test_job_argument_errors(cc):
+    arrays = cc.make_inputs(3, 8, 1, 1)
+    with pytest.raises(cc.CcsdtError) as e:
+        cc.Job(arrays, nocc=3, nvir=8, kchunk=4)  # kchunk > nocc
+    assert e.value.code == -1
+    with pytest.raises(cc.CcsdtError) as e:
+        cc.Job(arrays, nocc=0, nvir=8, kchunk=1)
+    assert e.value.code == -1
+    L = cc.lib()
+    assert L.ccsdt_job_run(None, None, 0, None, None, None, None) == -5
+    assert L.ccsdt_job_destroy(None) == -5
+
+
+def test_footprint(cc):
+    small = cc.footprint(8, 40, 8, 120)
+    big = cc.footprint(30, 300, 16, 4960)
+    assert 0 < small < big
+    assert cc.footprint(0, 40, 1, 1) == -1
+
+
+def test_triple_lists(cc):
+    assert len(cc.triple_list(8, cc.blocks_uniform(8, 8), "ijk")) == 120
+    assert len(cc.triple_list(8, cc.blocks_uniform(8, 8), "ref")) == 288
+    t1 = cc.triple_list(30, cc.blocks_uniform(30, 16), "ijk")
+    assert len(t1) == 4960
+    assert cc.blocks_uniform(30, 16) == [(1, 16), (17, 30)]
+    i, j, k, klo = t1[:, 1], t1[:, 2], t1[:, 3], t1[:, 4]
+    assert np.all(i >= j) and np.all(j >= k) and np.all(k >= klo) and np.all(k - klo < 16)
+    assert np.all(t1[:, 0] == 31)
+    # every (i >= j >= k) triple exactly once
+    assert len({tuple(r) for r in t1[:, 1:4]}) == 4960
+    bl = cc.blocks_uneven(40, cc.config_seed(3))
+    assert bl[0][0] == 1 and bl[-1][1] == 40
+    assert all(b[1] + 1 == c[0] for b, c in zip(bl, bl[1:]))
+    assert len(cc.triple_list(40, bl, "ijk")) == 11480
+
+
+def test_inputs_shapes_and_ranges(cc):
+    nocc, nvir, kc = 8, 40, 8
+    a = cc.make_inputs(nocc, nvir, kc, cc.config_seed(0))
+    vv, ov = nvir * nvir, nocc * nvir
+    assert a["tkj"].size == kc * vv and a["tia"].size == ov * nocc and a["kkj"].size == kc * ov
+    e = a["eorb"]
+    assert np.all(np.diff(e[:nocc]) >= 0) and np.all(np.diff(e[nocc:]) >= 0)
+    assert e[:nocc].min() >= -1.5 and e[:nocc].max() < -0.3
+    assert e[nocc:].min() >= 0.1 and e[nocc:].max() < 4.0
+    for n in ("tij", "jia", "kkj", "t1v2"):
+        assert np.abs(a[n]).max() < 0.05
+    ill = cc.make_inputs(nocc, nvir, kc, cc.config_seed(2), illcond=True)
+    assert np.abs(ill["jia"]).min() < 1e-7 and np.abs(ill["tij"]).max() < 0.05
