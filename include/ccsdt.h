@@ -210,7 +210,8 @@ int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t nt
 
 /* ---------------------------------------------------------------- diagnostics */
 /* One v_mfma_f64_16x16x4_f64 on A (16x4 row major, 64 doubles) and B (4x16 row
- * major, 64 doubles); d256[4*lane + r] receives each lane's four results.  No
+ * major, 64 doubles); d256[4*lane + r] receives each lane's four results
+ * (D[lane/16 + 4r][lane%16] on gfx950).  No
  * counterpart in the reference: checks the lane layout the kernels assume. */
 int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256);
 

@@ -178,9 +178,10 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
         }
     }
 
-    // v_mfma_f64_16x16x4_f64 result: lane l, element r holds row 4*(l/16)+r, column l%16.
+    // v_mfma_f64_16x16x4_f64 result: lane l, element r holds row l/16 + 4*r,
+    // column l%16 (measured with ccsdt_selftest_mfma).
     const int nv = g.nvir, nvp = g.nvp;
-    const int col = lane & 15, row0 = 4 * (lane >> 4);
+    const int col = lane & 15, row0 = lane >> 4;
     const int cx = ci * 16 + col;
     if (WRITEF) {
         double *fo = fout + (int64_t)t * fstride;
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
         const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};  // product -> body order f1n,f1t,...
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int bx = bi * 16 + row0 + r;
+            const int bx = bi * 16 + row0 + 4 * r;
             if (bx < nv && cx < nv) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) fo[mat[q] * vv + bx + (int64_t)cx * nv] = acc[q][r];
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
         const double y1c = vec6[4 * nvp + cx], y2c = vec6[5 * nvp + cx];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int bx = bi * 16 + row0 + r;
+            const int bx = bi * 16 + row0 + 4 * r;
             if (bx < nv && cx < nv) {
                 const double d = -1.0 / (ev[bx] + ec + tk.eaijk);
                 const double u1b = vec6[bx], u2b = vec6[nvp + bx];
@@ -435,7 +436,8 @@ hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks
 
 // ------------------------------------------------------------------ layout probe
 // One v_mfma_f64_16x16x4_f64 on A (16x4, row major) and B (4x16, row major),
-// fed as lane l: A[l%16][l/16], B[l/16][l%16]; d[4*l + r] is the lane's result.
+// fed as lane l: A[l%16][l/16], B[l/16][l%16]; d[4*l + r] is the lane's result,
+// which is D[l/16 + 4*r][l%16].
 __global__ __launch_bounds__(64) void k_mfma_probe(const double *a, const double *b, double *d)
 {
     const int l = threadIdx.x;

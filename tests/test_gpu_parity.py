@@ -42,7 +42,7 @@ def test_mfma_lane_layout(cc):
     B = rng.standard_normal((4, 16))
     d = cc.selftest_mfma(A.ravel(), B.ravel())
     D = A @ B
-    want = np.array([[D[4 * (l // 16) + r, l % 16] for r in range(4)] for l in range(64)]).ravel()
+    want = np.array([[D[l // 16 + 4 * r, l % 16] for r in range(4)] for l in range(64)]).ravel()
     assert np.allclose(d, want, rtol=1e-14, atol=1e-14)
 
 
