@@ -23,6 +23,10 @@ FLAGS = [
     "-shared",
     # packed FP32 ops stay off (the host pass warns it does not know the feature)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: the fused energy epilogue reads them with VALU
+    # instructions, and the AGPR form made the compiler copy all 128 accumulator
+    # registers between the two files on every K step.
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-I" + os.path.join(ROOT, "include"),
 ]
 

@@ -116,8 +116,8 @@ Geom make_geom(int ncor, int nocc, int nvir)
     g.nvp = (int)round_up(nvir, 16);
     g.nb = g.nvp / 16;
     g.kdim = nvir + nocc;
-    g.kp = (int)round_up(g.kdim, 8);
-    g.nc = g.kp / 8;
+    g.kp = (int)round_up(g.kdim, 8);  // an even number of 4-wide MFMA steps
+    g.nks = g.kp / 4;
     g.npairs = g.nb * (g.nb + 1) / 2;
     g.vecsz = (int64_t)g.nvp * g.kp;
     return g;

@@ -22,7 +22,7 @@ struct Geom {
     int nb;      // nvp / 16
     int kdim;    // nvir + nocc: the concatenated contraction length
     int kp;      // kdim rounded up to 8
-    int nc;      // kp / 8 contraction chunks
+    int nks;     // kp / 4 MFMA contraction steps (even)
     int npairs;  // nb*(nb+1)/2 lower-triangle 16x16 block pairs
     int64_t vecsz;  // doubles in one packed panel: nvp * kp
 };

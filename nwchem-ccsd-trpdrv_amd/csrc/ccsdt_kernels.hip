@@ -16,10 +16,10 @@
 // the same lane and register, and finishes the energy terms in registers.
 //
 // Panels are stored pre-swizzled for v_mfma_f64_16x16x4_f64: per 16-row block
-// xb and 8-wide contraction chunk c, lane l holds a double2 with
-//   value(x = 16*xb + l%16, kk = 8*c + 2*(l/16) + s),  s = 0, 1
-// and the MFMA of step s consumes element s.  A and B use the same k mapping,
-// so every chunk contributes k = 8c..8c+7 exactly once.
+// xb and 4-wide contraction step ks, lane l holds
+//   value(x = 16*xb + l%16, kk = 4*ks + l/16),
+// the A (or B) operand of one MFMA, so a wave's operand load is one contiguous
+// 512-byte global_load_dwordx2.
 #include "ccsdt_internal.h"
 
 namespace ccsdt {
@@ -64,6 +64,22 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Raw buffer addressing: a 128-bit resource in scalar registers, a wave-uniform
+// byte offset (soffset) and one per-lane byte offset, so an operand fetch costs
+// no per-load vector address registers.  Word 3 = 0x00020000 is the gfx9-family
+// raw-buffer format (32-bit data format, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const double *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                  uint32_t soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // Spreads consecutive logical tiles over one XCD: the hardware hands block b to
 // XCD b % 8, so logical tile L = (b%8)*q + min(b%8, r) + b/8 keeps a task's
 // tiles (which share operand panels) in the same L2.
@@ -81,13 +97,12 @@ __global__ __launch_bounds__(256) void k_pack(const PanelDesc *__restrict__ desc
     const int nv = g.nvir, no = g.nocc;
     for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < g.vecsz;
          idx += (int64_t)gridDim.x * 256) {
-        const int s = (int)(idx & 1);
-        const int l = (int)((idx >> 1) & 63);
-        const int64_t rest = idx >> 7;
-        const int c = (int)(rest % g.nc);
-        const int xb = (int)(rest / g.nc);
+        const int l = (int)(idx & 63);
+        const int64_t rest = idx >> 6;
+        const int ks = (int)(rest % g.nks);
+        const int xb = (int)(rest / g.nks);
         const int x = xb * 16 + (l & 15);
-        const int kk = c * 8 + (l >> 4) * 2 + s;
+        const int kk = ks * 4 + (l >> 4);
         double v = 0.0;
         if (x < nv) {
             if (kk < nv) {
@@ -110,7 +125,7 @@ hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipS
 }
 
 // ------------------------------------------------------------------ fused GEMM + energy
-#define MF(p, ia, ib, s) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia][s], b[ib][s], acc[p], 0, 0, 0)
+#define MF(p, ia, ib) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], acc[p], 0, 0, 0)
 
 template <bool TENGY, bool WRITEF>
 __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Geom g,
@@ -130,51 +145,65 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
     const int bi = pr.x, ci = pr.y;
     const int lane = threadIdx.x;
 
-    const int64_t vs = g.vecsz;
-    const int64_t blk = (int64_t)g.nc * 128;
-    const double *sl = pan + (int64_t)(PAN_SLICE0 + 4 * tk.slice) * vs;
-    const double *ab = pan + bi * blk + lane * 2;   // panels at the b block
-    const double *sb = sl + bi * blk + lane * 2;
-    const double *ac = pan + ci * blk + lane * 2;   // panels at the c block
-    const double *sc = sl + ci * blk + lane * 2;
+    // Two buffer resources: the job-wide panels (LJ, LK, R1t, R3t) and the four
+    // panels of the task's k slice (R1, R3, LJt, LKt).  Byte offsets fit 32 bits
+    // (checked on the host: 4 panels < 4 GiB).
+    const uint32_t vsb = (uint32_t)(g.vecsz * 8);
+    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(pan, 4 * vsb);
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pan + (int64_t)(PAN_SLICE0 + 4 * tk.slice) * g.vecsz, 4 * vsb);
+    const uint32_t blkb = (uint32_t)g.nks * 512;  // bytes of one 16-row block of a panel
+    const uint32_t ob = bi * blkb, oc = ci * blkb;
+    const uint32_t voff = lane * 8;
     // A side, index b: LJ, LK, LJt, LKt, R1, R3, R1t, R3t
-    const double *A[8] = {ab + PAN_LJ * vs, ab + PAN_LK * vs, sb + SL_LJT * vs, sb + SL_LKT * vs,
-                          sb + SL_R1 * vs, sb + SL_R3 * vs, ab + PAN_R1T * vs, ab + PAN_R3T * vs};
+    const uint32_t AO[8] = {PAN_LJ * vsb + ob, PAN_LK * vsb + ob, SL_LJT * vsb + ob, SL_LKT * vsb + ob,
+                            SL_R1 * vsb + ob, SL_R3 * vsb + ob, (PAN_R1T) * vsb + ob, (PAN_R3T) * vsb + ob};
     // B side, index c: R1, R3, R1t, R3t, LJ, LK, LJt, LKt
-    const double *B[8] = {sc + SL_R1 * vs, sc + SL_R3 * vs, ac + PAN_R1T * vs, ac + PAN_R3T * vs,
-                          ac + PAN_LJ * vs, ac + PAN_LK * vs, sc + SL_LJT * vs, sc + SL_LKT * vs};
+    const uint32_t BO[8] = {SL_R1 * vsb + oc, SL_R3 * vsb + oc, PAN_R1T * vsb + oc, PAN_R3T * vsb + oc,
+                            PAN_LJ * vsb + oc, PAN_LK * vsb + oc, SL_LJT * vsb + oc, SL_LKT * vsb + oc};
+    // which resource each operand lives in (1 = slice panels)
+    constexpr bool AS[8] = {false, false, true, true, true, true, false, false};
+    constexpr bool BS[8] = {true, true, false, false, false, false, true, true};
 
     dbl4 acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    dbl2 a[8], b[8];
+    // Ping-pong operand buffers X and Y, two K steps per trip (nks is even):
+    // each buffer's loads are in flight while the other buffer's 16 MFMAs run.
+    double xa[8], xb[8], ya[8], yb[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-        a[v] = *reinterpret_cast<const dbl2 *>(A[v]);
-        b[v] = *reinterpret_cast<const dbl2 *>(B[v]);
+        xa[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v]);
+        xb[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v]);
     }
-    const int nc = g.nc;
-    for (int c = 0; c < nc; ++c) {
-        const int cn = (c + 1 < nc) ? c + 1 : c;  // clamped prefetch, always in bounds
-        const int64_t on = (int64_t)cn * 128;
-        dbl2 an[8], bn[8];
+    const int nks = g.nks;
+    for (int ks = 0; ks < nks; ks += 2) {
+        const uint32_t o1 = (uint32_t)(ks + 1) * 512;
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-            an[v] = *reinterpret_cast<const dbl2 *>(A[v] + on);
-            bn[v] = *reinterpret_cast<const dbl2 *>(B[v] + on);
+            ya[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v] + o1);
+            yb[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v] + o1);
         }
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            MF(0, 0, 0, s);  MF(1, 1, 0, s);  MF(2, 0, 1, s);  MF(3, 1, 1, s);    // f1n f2n f3n f4n (b,c)
-            MF(4, 2, 2, s);  MF(5, 3, 2, s);  MF(6, 2, 3, s);  MF(7, 3, 3, s);    // f1t f2t f3t f4t (b,c)
-            MF(8, 4, 4, s);  MF(9, 4, 5, s);  MF(10, 5, 4, s); MF(11, 5, 5, s);   // f1n..f4n (c,b)
-            MF(12, 6, 6, s); MF(13, 6, 7, s); MF(14, 7, 6, s); MF(15, 7, 7, s);   // f1t..f4t (c,b)
+        {
+            double *a = xa, *b = xb;
+            MF(0, 0, 0);  MF(1, 1, 0);  MF(2, 0, 1);  MF(3, 1, 1);    // f1n f2n f3n f4n (b,c)
+            MF(4, 2, 2);  MF(5, 3, 2);  MF(6, 2, 3);  MF(7, 3, 3);    // f1t f2t f3t f4t (b,c)
+            MF(8, 4, 4);  MF(9, 4, 5);  MF(10, 5, 4); MF(11, 5, 5);   // f1n..f4n (c,b)
+            MF(12, 6, 6); MF(13, 6, 7); MF(14, 7, 6); MF(15, 7, 7);   // f1t..f4t (c,b)
         }
+        const int k2 = (ks + 2 < nks) ? ks + 2 : ks;  // clamped prefetch, always in bounds
+        const uint32_t o2 = (uint32_t)k2 * 512;
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-            a[v] = an[v];
-            b[v] = bn[v];
+            xa[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v] + o2);
+            xb[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v] + o2);
+        }
+        {
+            double *a = ya, *b = yb;
+            MF(0, 0, 0);  MF(1, 1, 0);  MF(2, 0, 1);  MF(3, 1, 1);
+            MF(4, 2, 2);  MF(5, 3, 2);  MF(6, 2, 3);  MF(7, 3, 3);
+            MF(8, 4, 4);  MF(9, 4, 5);  MF(10, 5, 4); MF(11, 5, 5);
+            MF(12, 6, 6); MF(13, 6, 7); MF(14, 7, 6); MF(15, 7, 7);
         }
     }
 
