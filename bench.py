@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--mode", default="fp64", choices=["fp64", "fp64_split"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (reference point for the "
+                         "FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the triples path has no multi-GPU form in this "
@@ -140,6 +143,10 @@ def main():
     else:
         line["cpu_baseline"] = None
     job.close()
+    if args.copy_calibration > 0:
+        nbytes = int(args.copy_calibration * 2 ** 20)
+        ms = cc.lib().ccsdt_debug_copy(nbytes, 3)
+        line["copy_calibration"] = {"bytes": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}
     print(json.dumps(line), flush=True)
 
 

@@ -214,6 +214,15 @@ int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t nt
  * (D[lane/16 + 4r][lane%16] on gfx950).  No
  * counterpart in the reference: checks the lane layout the kernels assume. */
 int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256);
+/* One v_mfma_i32_16x16x64_i8 (shape 0; d receives 4 ints per lane) or
+ * v_mfma_i32_32x32x32_i8 (shape 1; 16 ints per lane) on raw operand registers:
+ * a256 and b256 hold each lane's 16 operand bytes as four int32 words.  No
+ * counterpart in the reference: checks the INT8 lane layout. */
+int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d, int shape);
+/* Streams `bytes` through a plain device copy kernel `reps` times and returns the
+ * average milliseconds per copy (or -1): the reference point for HBM counters.
+ * No counterpart in the reference. */
+double ccsdt_debug_copy(int64_t bytes, int reps);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,8 @@ def lib():
             "ccsdt_job_destroy": (ctypes.c_int, [vp]),
             "ccsdt_job_footprint": (i64, [ctypes.c_int] * 4 + [i64]),
             "ccsdt_selftest_mfma": (ctypes.c_int, [vp, vp, vp]),
+            "ccsdt_selftest_mfma_i8": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
+            "ccsdt_debug_copy": (ctypes.c_double, [i64, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -359,6 +361,15 @@ def tengy(f, vec, eorb, eaijk, ncor, nocc, nvir, symbol="ccsd_tengy_omp"):
         getattr(L, symbol)(*fs, v["dintc1"], v["dintx1"], v["t1v1"], v["dintc2"], v["dintx2"],
                            v["t1v2"], e, eaijk, *[ctypes.byref(o) for o in out], ncor, nocc, nvir)
     return tuple(o.value for o in out)
+
+
+def selftest_mfma_i8(a_bytes, b_bytes, shape):
+    """a_bytes, b_bytes: (64, 16) int8 per-lane operand bytes."""
+    a = np.ascontiguousarray(a_bytes, dtype=np.int8).view(np.int32).ravel()
+    b = np.ascontiguousarray(b_bytes, dtype=np.int8).view(np.int32).ravel()
+    d = np.empty(64 * (16 if shape else 4), dtype=np.int32)
+    _check(lib().ccsdt_selftest_mfma_i8(_ptr(a), _ptr(b), _ptr(d), shape))
+    return d.reshape(64, -1)
 
 
 def selftest_mfma(a, b):

@@ -783,6 +783,45 @@ int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256)
     return CCSDT_OK;
 }
 
+int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d, int shape)
+{
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    if (shape != 0 && shape != 1) return set_error(CCSDT_E_ARG, "shape must be 0 or 1");
+    int *buf = nullptr;
+    if ((rc = dev_alloc((void **)&buf, (256 + 256 + 1024) * sizeof(int)))) return rc;
+    HIP_TRY(hipMemcpy(buf, a256, 256 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(buf + 256, b256, 256 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(launch_mfma_i8_probe(buf, buf + 256, buf + 512, shape, nullptr));
+    HIP_TRY(hipMemcpy(d, buf + 512, (shape ? 1024 : 256) * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipFree(buf));
+    return CCSDT_OK;
+}
+
+double ccsdt_debug_copy(int64_t bytes, int reps)
+{
+    if (ensure_device(-1)) return -1.0;
+    const int64_t n = bytes / 16 * 2;
+    double *src = nullptr, *dst = nullptr;
+    if (dev_alloc((void **)&src, n * 8) || dev_alloc((void **)&dst, n * 8)) return -1.0;
+    hipEvent_t e0, e1;
+    float ms = 0.f;
+    if (hipMemset(src, 0, n * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        return -1.0;
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r)
+        if (launch_copy(src, dst, n, nullptr) != hipSuccess) return -1.0;
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return reps > 0 ? ms / reps : 0.0;
+}
+
 void ccsd_trpdrv_omp_cbody_(double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
                             double *f3t, double *f4n, double *f4t, double *eorb, int *ncor,
                             int *nocc, int *nvir, double *emp4, double *emp5, int *a, int *i,

@@ -59,5 +59,7 @@ hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, do
 hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks, double *out2,
                               hipStream_t st);
 hipError_t launch_mfma_probe(const double *a, const double *b, double *d, hipStream_t st);
+hipError_t launch_mfma_i8_probe(const int *a, const int *b, int *d, int shape, hipStream_t st);
+hipError_t launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
 
 }  // namespace ccsdt

@@ -483,4 +483,48 @@ hipError_t launch_mfma_probe(const double *a, const double *b, double *d, hipStr
     return hipGetLastError();
 }
 
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef int int16v __attribute__((ext_vector_type(16)));
+
+// One INT8 MFMA on raw per-lane operand registers: a, b hold 16 bytes per lane
+// (64 lanes), d receives the lane's 4 (16x16x64) or 16 (32x32x32) results.
+__global__ __launch_bounds__(64) void k_mfma_i8_probe(const int *a, const int *b, int *d, int shape)
+{
+    const int l = threadIdx.x;
+    const int4v av = {a[4 * l], a[4 * l + 1], a[4 * l + 2], a[4 * l + 3]};
+    const int4v bv = {b[4 * l], b[4 * l + 1], b[4 * l + 2], b[4 * l + 3]};
+    if (shape == 0) {
+        int4v acc = {0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
+    } else {
+        int16v acc = {};
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) d[l * 16 + r] = acc[r];
+    }
+}
+
+hipError_t launch_mfma_i8_probe(const int *a, const int *b, int *d, int shape, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mfma_i8_probe, dim3(1), dim3(64), 0, st, a, b, d, shape);
+    return hipGetLastError();
+}
+
+// Plain streaming copy of n doubles (HBM counter calibration: reads and writes
+// exactly 8*n bytes each).
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double *src, double *dst, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, reinterpret_cast<const double2 *>(src),
+                       reinterpret_cast<double2 *>(dst), n / 2);
+    return hipGetLastError();
+}
+
 }  // namespace ccsdt
