@@ -24,7 +24,8 @@ sys.path.insert(0, ROOT)
 METRIC = "(T) FP64-eq TFLOP/s nocc=30 nvir=300, split vs FP64 @rel dE<1e-8; job p99"
 FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 HBM_PEAK = 8000.0      # GB/s
-KERNEL = {"fp64": "k_fused<true,false>", "fp64_split": "k_fused<false,true>"}
+KERNEL = {"fp64": "k_fused<true,false>", "fp64_split": "k_fused<false,true>",
+          "ozaki_int8": "k_fused_split<S,true,false>"}
 
 
 def load_pkg():
@@ -69,7 +70,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp64_split"])
+    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp64_split", "ozaki_int8"])
+    ap.add_argument("--slices", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
@@ -89,14 +91,14 @@ def main():
     flops = cc.gemm_flops_per_triple(nocc, nvir)
     job = cc.Job(arrays, nocc, nvir, kchunk)
     for _ in range(args.warmup):
-        job.run(trip, mode=args.mode)
+        job.run(trip, mode=args.mode, slices=args.slices)
     print("bench: warmup done (%d steps), timing %d steps" % (args.warmup, args.steps),
           file=sys.stderr, flush=True)
     step_ms, main_ms, res = [], [], None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        res = job.run(trip, mode=args.mode, terms=True)
+        res = job.run(trip, mode=args.mode, slices=args.slices, terms=True)
         step_ms.append((time.perf_counter() - s0) * 1e3)
         main_ms.append(res["ms_main"])
     wall = time.perf_counter() - t0

@@ -123,6 +123,20 @@ Geom make_geom(int ncor, int nocc, int nvir)
     return g;
 }
 
+SplitGeom make_split_geom(const Geom &g, int slices, int npanels)
+{
+    SplitGeom sg;
+    sg.slices = slices;
+    sg.kv = (int)round_up(g.nvir, 64);
+    sg.nkv = sg.kv / 64;
+    sg.kt = (int)round_up(g.nocc, 8);
+    sg.nkt = sg.kt / 4;
+    sg.slice_bytes = (int64_t)g.nvp * sg.kv;
+    sg.tail_sz = (int64_t)g.nvp * sg.kt;
+    sg.npanels = npanels;
+    return sg;
+}
+
 // Element counts of the job arrays (omp/test.c:112-146 shapes; tia, xia, jij,
 // kij and the dint/t1v vectors at the lengths the body reads).
 void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n)
@@ -179,6 +193,11 @@ struct ccsdt_job_s {
     int64_t cap_f = 0;
     double *d_out2 = nullptr;
     double *h_out = nullptr;  // pinned: 2 totals + 4 terms
+    // Ozaki INT8 split form (allocated on the first split run)
+    int split_slices = 0;
+    int8_t *d_digits = nullptr;
+    double *d_scales = nullptr;
+    double *d_tail = nullptr;
     std::vector<double> eorb;
     hipEvent_t ev[12] = {};
 };
@@ -192,6 +211,9 @@ void job_free(ccsdt_job j)
     for (double *p : dd)
         if (p) (void)hipFree(p);
     if (j->d_desc) (void)hipFree(j->d_desc);
+    if (j->d_digits) (void)hipFree(j->d_digits);
+    if (j->d_scales) (void)hipFree(j->d_scales);
+    if (j->d_tail) (void)hipFree(j->d_tail);
     if (j->d_pairs) (void)hipFree(j->d_pairs);
     if (j->d_tasks) (void)hipFree(j->d_tasks);
     if (j->h_tasks) (void)hipHostFree(j->h_tasks);
@@ -409,8 +431,11 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     if (n < 1 || !trip) return set_error(CCSDT_E_ARG, "empty triple list");
     if (fouts && n != 1) return set_error(CCSDT_E_ARG, "f output needs a single triple");
     const int mode = opts ? opts->mode : CCSDT_MODE_FP64;
-    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT)
+    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT && mode != CCSDT_MODE_OZAKI_INT8)
         return set_error(CCSDT_E_ARG, "mode %d not available in this build", mode);
+    const int slices = (mode == CCSDT_MODE_OZAKI_INT8) ? (opts && opts->slices ? opts->slices : 4) : 0;
+    if (mode == CCSDT_MODE_OZAKI_INT8 && (slices < 3 || slices > 7))
+        return set_error(CCSDT_E_ARG, "Ozaki slice count %d outside 3..7", slices);
     int rc = ensure_device(-1);
     if (rc) return rc;
     const Geom &g = j->g;
@@ -419,7 +444,30 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     rc = job_tasks(j, trip, n);
     if (rc) return rc;
     const int tnp = tengy_npairs(g);
-    const int64_t npart = (int64_t)n * (mode == CCSDT_MODE_FP64 ? g.npairs : tnp) * 4;
+    const bool fused = (mode != CCSDT_MODE_FP64_SPLIT);
+    const int64_t npart = (int64_t)n * (fused ? g.npairs : tnp) * 4;
+    SplitGeom sg{};
+    if (mode == CCSDT_MODE_OZAKI_INT8) {
+        sg = make_split_geom(g, slices, j->npanels);
+        const int64_t dig = (int64_t)j->npanels * slices * sg.slice_bytes;
+        const int64_t tb = (int64_t)j->npanels * sg.tail_sz * 8;
+        if (dig >= (int64_t(1) << 32) || tb >= (int64_t(1) << 32))
+            return set_error(CCSDT_E_ARG, "split operands exceed 4 GiB (nvir=%d kchunk=%d slices=%d)",
+                             g.nvir, j->kchunk, slices);
+        if (j->split_slices != slices) {
+            if (j->d_digits) (void)hipFree(j->d_digits);
+            if (j->d_scales) (void)hipFree(j->d_scales);
+            if (j->d_tail) (void)hipFree(j->d_tail);
+            j->d_digits = nullptr;
+            j->d_scales = nullptr;
+            j->d_tail = nullptr;
+            j->split_slices = 0;
+            if ((rc = dev_alloc((void **)&j->d_digits, (size_t)dig))) return rc;
+            if ((rc = dev_alloc((void **)&j->d_scales, (size_t)j->npanels * g.nvp * sizeof(double)))) return rc;
+            if ((rc = dev_alloc((void **)&j->d_tail, (size_t)tb))) return rc;
+            j->split_slices = slices;
+        }
+    }
     if ((rc = grow(&j->d_partial, &j->cap_partial, npart))) return rc;
     if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * n))) return rc;
     int64_t chunk = n;
@@ -436,10 +484,20 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     double ms_main = 0.0, ms_tengy = 0.0;
     INJECT_POINT("launch");
     HIP_TRY(hipEventRecord(E[0], j->st));
-    HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+    if (mode == CCSDT_MODE_OZAKI_INT8)
+        HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
+    else
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
     ++launches;
     HIP_TRY(hipEventRecord(E[1], j->st));
-    if (mode == CCSDT_MODE_FP64) {
+    if (mode == CCSDT_MODE_OZAKI_INT8) {
+        HIP_TRY(launch_fused_split(j->d_digits, j->d_scales, j->d_tail, g, sg, j->d_tasks, (int)n,
+                                   j->d_pairs, j->d_ev, j->d_vec6, j->d_partial,
+                                   fouts ? j->d_f : nullptr, 8 * vv, true, j->st));
+        HIP_TRY(hipEventRecord(E[2], j->st));
+        HIP_TRY(launch_reduce_terms(j->d_partial, g.npairs, (int)n, j->d_terms, j->st));
+        launches += 2;
+    } else if (mode == CCSDT_MODE_FP64) {
         HIP_TRY(launch_fused(j->d_pan, g, j->d_tasks, (int)n, j->d_pairs, j->d_ev, j->d_vec6,
                              j->d_partial, fouts ? j->d_f : nullptr, 8 * vv, true, j->st));
         HIP_TRY(hipEventRecord(E[2], j->st));
@@ -483,7 +541,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         res->emp5 = j->h_out[1];
         res->ms_total = ev_ms(E[0], E[3]);
         res->ms_pack = ev_ms(E[0], E[1]);
-        res->ms_main = mode == CCSDT_MODE_FP64 ? ev_ms(E[1], E[2]) : ms_main;
+        res->ms_main = fused ? ev_ms(E[1], E[2]) : ms_main;
         res->ms_tengy = ms_tengy;
         res->ms_reduce = ev_ms(E[2], E[3]);
         res->ntriples = n;
@@ -688,7 +746,7 @@ void ccsdt_clear_error(void)
 }
 void ccsdt_set_error_policy(int abort_on_error) { g_abort_policy.store(abort_on_error ? 1 : 0); }
 void ccsdt_debug_inject_failure(int nth) { g_inject.store(nth > 0 ? nth : 0); }
-const char *ccsdt_version(void) { return "ccsdt-mi355x 0.1 (gfx950, FP64 MFMA)"; }
+const char *ccsdt_version(void) { return "ccsdt-mi355x 0.2 (gfx950, FP64 MFMA, Ozaki INT8 split)"; }
 int ccsdt_init(int device) { return ensure_device(device); }
 void ccsdt_set_f_writeback(int on) { g_f_writeback.store(on ? 1 : 0); }
 

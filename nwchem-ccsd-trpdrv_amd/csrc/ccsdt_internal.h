@@ -44,6 +44,19 @@ enum { PAN_LJ = 0, PAN_LK = 1, PAN_R1T = 2, PAN_R3T = 3, PAN_SLICE0 = 4 };
 // Per k-slice s the four panels start at PAN_SLICE0 + 4*s.
 enum { SL_R1 = 0, SL_R3 = 1, SL_LJT = 2, SL_LKT = 3 };
 
+// Geometry of the Ozaki INT8 split form: the K = nvir half of every panel as
+// `slices` int8 digit planes (kv = nvir rounded up to 64, one
+// v_mfma_i32_16x16x64_i8 K step per 64), and the K = nocc half as an FP64 tail
+// panel (kt = nocc rounded up to 8).
+struct SplitGeom {
+    int slices;
+    int kv, nkv;          // kv = round_up(nvir, 64), nkv = kv / 64
+    int kt, nkt;          // kt = round_up(nocc, 8), nkt = kt / 4 (even)
+    int64_t slice_bytes;  // bytes of one digit plane of one panel: nvp * kv
+    int64_t tail_sz;      // doubles of one FP64 tail panel: nvp * kt
+    int npanels;          // panels of the job (bounds of the buffer resources)
+};
+
 // Launchers (ccsdt_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st);
 hipError_t launch_fused(const double *pan, const Geom &g, const Task *tasks, int ntasks,
@@ -54,6 +67,14 @@ hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const T
                         int ntasks, const double *ev, const double *vec6, double *partial,
                         hipStream_t st);
 int tengy_npairs(const Geom &g);
+hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
+                             const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
+                             hipStream_t st);
+hipError_t launch_fused_split(const int8_t *digits, const double *scales, const double *tail,
+                              const Geom &g, const SplitGeom &sg, const Task *tasks, int ntasks,
+                              const short2 *pairs, const double *ev, const double *vec6,
+                              double *partial, double *fout, int64_t fstride, bool tengy,
+                              hipStream_t st);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,
                                hipStream_t st);
 hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks, double *out2,

@@ -288,6 +288,298 @@ hipError_t launch_fused(const double *pan, const Geom &g, const Task *tasks, int
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ Ozaki INT8 split form
+// The K = nvir half of every product runs as Ozaki-I split products (Ozaki,
+// Ogita, Oishi, Rump 2012; integer form after Ootomo, Ozaki, Yokota 2024): each
+// panel row x is scaled by 2^-e(x) so its largest entry is at most 127/128, then
+// cut into `S` signed 7-bit digit planes by round-to-nearest,
+//   P(x,k) = 2^e(x) * sum_s d_s(x,k) 2^(-7s) + r,  |d_1| <= 127, |d_s>1| <= 64,
+// every step exact in FP64.  Digit plane s of A times plane q of B is an exact
+// INT32 product (K*127^2 << 2^31); products with s+q <= S+1 are kept and summed
+// per level L = s+q in INT32, then recombined in FP64 with the power-of-two
+// scales.  The K = nocc half and the energy epilogue stay FP64.
+
+// One wave per panel row: row maximum, exponent, digits, and the FP64 tail.
+// Digit plane layout per 16-row block xb and 64-wide K step ks: lane l holds 16
+// bytes for x = 16*xb + l%16, k = 64*ks + 16*(l/16) + j (j = byte), which is the
+// operand layout of v_mfma_i32_16x16x64_i8 (row = lane%16, 16 K bytes per lane).
+template <int S>
+__global__ __launch_bounds__(64) void k_split_pack(const PanelDesc *__restrict__ descs, Geom g,
+                                                  SplitGeom sg, int8_t *__restrict__ digits,
+                                                  double *__restrict__ scales,
+                                                  double *__restrict__ tail)
+{
+    const int pnl = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
+    const PanelDesc d = descs[pnl];
+    const int nv = g.nvir, no = g.nocc;
+    auto val = [&](int k) -> double {
+        if (x >= nv || k >= nv) return 0.0;
+        return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
+    };
+    double m = 0.0;
+    for (int k = lane; k < nv; k += 64) m = fmax(m, fabs(val(k)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    int e = 0;
+    if (m > 0.0) (void)frexp(m * (128.0 / 127.0), &e);
+    const double inv = ldexp(1.0, -e);
+    if (lane == 0) scales[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
+    const int xb = x >> 4, xi = x & 15;
+    for (int k = lane; k < sg.kv; k += 64) {
+        double v = val(k) * inv;
+        const int64_t o = ((int64_t)(xb * sg.nkv + (k >> 6)) * 64 + ((k & 63) >> 4) * 16 + xi) * 16 + (k & 15);
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            v *= 128.0;
+            const double dg = rint(v);
+            v -= dg;
+            digits[((int64_t)pnl * S + q) * sg.slice_bytes + o] = (int8_t)dg;
+        }
+    }
+    for (int kk = lane; kk < sg.kt; kk += 64) {
+        double t = 0.0;
+        if (x < nv && kk < no)
+            t = d.sgn * (d.second_t ? d.Q[kk + (int64_t)x * no] : d.Q[x + (int64_t)kk * nv]);
+        tail[(int64_t)pnl * sg.tail_sz + ((int64_t)(xb * sg.nkt + (kk >> 2)) * 64 + xi + 16 * (kk & 3))] = t;
+    }
+}
+
+hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
+                             const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
+                             hipStream_t st)
+{
+    dim3 grid(g.nvp, npanels), block(64);
+    switch (sg.slices) {
+    case 3: hipLaunchKernelGGL(k_split_pack<3>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 4: hipLaunchKernelGGL(k_split_pack<4>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 5: hipLaunchKernelGGL(k_split_pack<5>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 6: hipLaunchKernelGGL(k_split_pack<6>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 7: hipLaunchKernelGGL(k_split_pack<7>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int4v buffer_load_i8x16(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(int4v, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+// Same work unit and epilogue as k_fused; products numbered as there.  Groups of
+// four products share two A and two B panels: g0 n (b,c), g1 t (b,c), g2 n (c,b),
+// g3 t (c,b).  After the INT8 groups the FP64 tail runs on v_mfma_f64_16x16x4_f64
+// with the A rows fed in the order 4*(i%4) + i/4, which makes its result layout
+// (lane l, element r -> row 4*(l/16)+r, column l%16) equal that of the INT8 MFMA.
+template <int S, bool TENGY, bool WRITEF>
+__global__ __launch_bounds__(64) void k_fused_split(const int8_t *__restrict__ digits,
+                                                    const double *__restrict__ scales,
+                                                    const double *__restrict__ tail, Geom g,
+                                                    SplitGeom sg, const Task *__restrict__ tasks,
+                                                    int ntasks, const short2 *__restrict__ pairs,
+                                                    const double *__restrict__ ev,
+                                                    const double *__restrict__ vec6,
+                                                    double *__restrict__ partial,
+                                                    double *__restrict__ fout, int64_t fstride)
+{
+    const int total = ntasks * g.npairs;
+    const int L = xcd_remap(blockIdx.x, total);
+    const int t = L / g.npairs;
+    const int p = L - t * g.npairs;
+    const Task tk = tasks[t];
+    const short2 pr = pairs[p];
+    const int bi = pr.x, ci = pr.y;
+    const int lane = threadIdx.x;
+    const int nvp = g.nvp;
+    const int ps = PAN_SLICE0 + 4 * tk.slice;
+    // panels of the four groups: {A0, A1, B0, B1}
+    const int GP[4][4] = {{PAN_LJ, PAN_LK, ps + SL_R1, ps + SL_R3},
+                          {ps + SL_LJT, ps + SL_LKT, PAN_R1T, PAN_R3T},
+                          {ps + SL_R1, ps + SL_R3, PAN_LJ, PAN_LK},
+                          {PAN_R1T, PAN_R3T, ps + SL_LJT, ps + SL_LKT}};
+    // product number of the combination (a, b) -> index a + 2*b
+    const int GPROD[4][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {8, 10, 9, 11}, {12, 14, 13, 15}};
+
+    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(digits),
+                                                  (uint32_t)(sg.npanels * S * sg.slice_bytes));
+    const uint32_t plane = (uint32_t)sg.slice_bytes;
+    const uint32_t blk8 = (uint32_t)sg.nkv * 1024;  // bytes of a 16-row block of one plane
+    const uint32_t voff8 = lane * 16;
+
+    const int row0 = 4 * (lane >> 4), col = lane & 15;
+    const int cx = ci * 16 + col;
+
+    dbl4 facc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+#pragma unroll
+    for (int grp = 0; grp < 4; ++grp) {
+        int4v iacc[4][S];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int q = 0; q < S; ++q) iacc[c][q] = int4v{0, 0, 0, 0};
+        const uint32_t oa0 = (uint32_t)GP[grp][0] * S * plane + bi * blk8;
+        const uint32_t oa1 = (uint32_t)GP[grp][1] * S * plane + bi * blk8;
+        const uint32_t ob0 = (uint32_t)GP[grp][2] * S * plane + ci * blk8;
+        const uint32_t ob1 = (uint32_t)GP[grp][3] * S * plane + ci * blk8;
+        for (int ks = 0; ks < sg.nkv; ++ks) {
+            const uint32_t ko = ks * 1024;
+            int4v a0[S], a1[S], b0[S], b1[S];
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                a0[q] = buffer_load_i8x16(rd, voff8, oa0 + q * plane + ko);
+                a1[q] = buffer_load_i8x16(rd, voff8, oa1 + q * plane + ko);
+                b0[q] = buffer_load_i8x16(rd, voff8, ob0 + q * plane + ko);
+                b1[q] = buffer_load_i8x16(rd, voff8, ob1 + q * plane + ko);
+            }
+#pragma unroll
+            for (int sa = 0; sa < S; ++sa) {
+#pragma unroll
+                for (int sb = 0; sb < S - sa; ++sb) {
+                    const int lv = sa + sb;
+                    iacc[0][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b0[sb], iacc[0][lv], 0, 0, 0);
+                    iacc[1][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[sa], b0[sb], iacc[1][lv], 0, 0, 0);
+                    iacc[2][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b1[sb], iacc[2][lv], 0, 0, 0);
+                    iacc[3][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[sa], b1[sb], iacc[3][lv], 0, 0, 0);
+                }
+            }
+        }
+        // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * 2^(-7(L+2))
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int pa = GP[grp][c & 1], pb = GP[grp][2 + (c >> 1)];
+            const double sB = scales[(int64_t)pb * nvp + cx];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * __builtin_ldexp(1.0, -7 * (q + 2));
+                const double sA = scales[(int64_t)pa * nvp + bi * 16 + row0 + r];
+                facc[GPROD[grp][c]][r] += v * sA * sB;
+            }
+        }
+    }
+
+    // FP64 tail (K = nocc)
+    {
+        const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
+        const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
+        const uint32_t blkt = (uint32_t)sg.nkt * 512;
+        const int li = lane & 15;
+        const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
+        const uint32_t voffB = lane * 8;
+        const int AP[8] = {PAN_LJ, PAN_LK, ps + SL_LJT, ps + SL_LKT, ps + SL_R1, ps + SL_R3, PAN_R1T, PAN_R3T};
+        const int BP[8] = {ps + SL_R1, ps + SL_R3, PAN_R1T, PAN_R3T, PAN_LJ, PAN_LK, ps + SL_LJT, ps + SL_LKT};
+        for (int ks = 0; ks < sg.nkt; ++ks) {
+            double a[8], b[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                a[v] = buffer_load_f64(rt, voffA, AP[v] * tsb + bi * blkt + ks * 512);
+                b[v] = buffer_load_f64(rt, voffB, BP[v] * tsb + ci * blkt + ks * 512);
+            }
+#define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
+            MS(0, 0, 0);  MS(1, 1, 0);  MS(2, 0, 1);  MS(3, 1, 1);
+            MS(4, 2, 2);  MS(5, 3, 2);  MS(6, 2, 3);  MS(7, 3, 3);
+            MS(8, 4, 4);  MS(9, 4, 5);  MS(10, 5, 4); MS(11, 5, 5);
+            MS(12, 6, 6); MS(13, 6, 7); MS(14, 7, 6); MS(15, 7, 7);
+#undef MS
+        }
+    }
+
+    const int nv = g.nvir;
+    if (WRITEF) {
+        double *fo = fout + (int64_t)t * fstride;
+        const int64_t vv = (int64_t)nv * nv;
+        const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int bx = bi * 16 + row0 + r;
+            if (bx < nv && cx < nv) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) fo[mat[q] * vv + bx + (int64_t)cx * nv] = facc[q][r];
+                if (bi != ci) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) fo[mat[q] * vv + cx + (int64_t)bx * nv] = facc[8 + q][r];
+                }
+            }
+        }
+    }
+    if (TENGY) {
+        Sum4 s = {0.0, 0.0, 0.0, 0.0};
+        const double ec = ev[cx];
+        const double u1c = vec6[cx], u2c = vec6[nvp + cx];
+        const double x1c = vec6[2 * nvp + cx], x2c = vec6[3 * nvp + cx];
+        const double y1c = vec6[4 * nvp + cx], y2c = vec6[5 * nvp + cx];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int bx = bi * 16 + row0 + r;
+            if (bx < nv && cx < nv) {
+                const double dd = -1.0 / (ev[bx] + ec + tk.eaijk);
+                tengy_point(facc[0][r], facc[1][r], facc[2][r], facc[3][r],
+                            facc[4][r], facc[5][r], facc[6][r], facc[7][r],
+                            facc[8][r], facc[9][r], facc[10][r], facc[11][r],
+                            facc[12][r], facc[13][r], facc[14][r], facc[15][r],
+                            dd, vec6[bx], vec6[nvp + bx], x1c, x2c, y1c, y2c, s);
+                if (bi != ci)
+                    tengy_point(facc[8][r], facc[9][r], facc[10][r], facc[11][r],
+                                facc[12][r], facc[13][r], facc[14][r], facc[15][r],
+                                facc[0][r], facc[1][r], facc[2][r], facc[3][r],
+                                facc[4][r], facc[5][r], facc[6][r], facc[7][r],
+                                dd, u1c, u2c, vec6[2 * nvp + bx], vec6[3 * nvp + bx],
+                                vec6[4 * nvp + bx], vec6[5 * nvp + bx], s);
+            }
+        }
+        s.e4i = wave_sum(s.e4i);
+        s.e5i = wave_sum(s.e5i);
+        s.e4k = wave_sum(s.e4k);
+        s.e5k = wave_sum(s.e5k);
+        if (lane == 0) {
+            double *o = partial + (int64_t)L * 4;
+            o[0] = s.e4i; o[1] = s.e5i; o[2] = s.e4k; o[3] = s.e5k;
+        }
+    }
+}
+
+template <int S>
+static hipError_t launch_split_s(const int8_t *digits, const double *scales, const double *tail,
+                                 const Geom &g, const SplitGeom &sg, const Task *tasks, int ntasks,
+                                 const short2 *pairs, const double *ev, const double *vec6,
+                                 double *partial, double *fout, int64_t fstride, bool tengy,
+                                 hipStream_t st)
+{
+    dim3 grid((unsigned)((int64_t)ntasks * g.npairs)), block(64);
+    if (tengy && fout)
+        hipLaunchKernelGGL((k_fused_split<S, true, true>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    else if (tengy)
+        hipLaunchKernelGGL((k_fused_split<S, true, false>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    else if (fout)
+        hipLaunchKernelGGL((k_fused_split<S, false, true>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_split(const int8_t *digits, const double *scales, const double *tail,
+                              const Geom &g, const SplitGeom &sg, const Task *tasks, int ntasks,
+                              const short2 *pairs, const double *ev, const double *vec6,
+                              double *partial, double *fout, int64_t fstride, bool tengy,
+                              hipStream_t st)
+{
+    const int64_t total = (int64_t)ntasks * g.npairs;
+    if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
+    switch (sg.slices) {
+    case 3: return launch_split_s<3>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
+    case 4: return launch_split_s<4>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
+    case 5: return launch_split_s<5>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
+    case 6: return launch_split_s<6>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
+    case 7: return launch_split_s<7>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // ------------------------------------------------------------------ standalone energy pass
 // Reads the eight f matrices of each task from HBM (ld = nvir, body order
 // f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t) in 32x32 tile pairs (BI >= CI).  The direct
