@@ -367,64 +367,47 @@ __device__ __forceinline__ int4v buffer_load_i8x16(__amdgpu_buffer_rsrc_t r, uin
     return __builtin_bit_cast(int4v, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-// Same work unit and epilogue as k_fused; products numbered as there.  Groups of
-// four products share two A and two B panels: g0 n (b,c), g1 t (b,c), g2 n (c,b),
-// g3 t (c,b).  After the INT8 groups the FP64 tail runs on v_mfma_f64_16x16x4_f64
-// with the A rows fed in the order 4*(i%4) + i/4, which makes its result layout
-// (lane l, element r -> row 4*(l/16)+r, column l%16) equal that of the INT8 MFMA.
-template <int S, bool TENGY, bool WRITEF>
-__global__ __launch_bounds__(64) void k_fused_split(const int8_t *__restrict__ digits,
-                                                    const double *__restrict__ scales,
-                                                    const double *__restrict__ tail, Geom g,
-                                                    SplitGeom sg, const Task *__restrict__ tasks,
-                                                    int ntasks, const short2 *__restrict__ pairs,
-                                                    const double *__restrict__ ev,
-                                                    const double *__restrict__ vec6,
-                                                    double *__restrict__ partial,
-                                                    double *__restrict__ fout, int64_t fstride)
+// Work unit of k_fused (one 16x16 block pair of one triple) split over two
+// waves: wave 0 computes the eight products at (b,c) (groups n and t), wave 1 the
+// eight mirrored ones f(c,b) (groups n' and t').  Each group shares two A and two
+// B panels.  After the INT8 groups each wave adds its FP64 tail (K = nocc) on
+// v_mfma_f64_16x16x4_f64 with the A rows fed in the order 4*(i%4) + i/4, which
+// makes its result layout (lane l, element r -> row 4*(l/16)+r, column l%16)
+// equal that of the INT8 MFMA.  The energy epilogue swaps half the values
+// through LDS and each wave finishes two of the four rows per lane.
+// The eight products of one wave role (W = 0: (b,c); W = 1: mirrored (c,b)),
+// panel ids resolved at compile time so every operand offset stays scalar.
+template <int S, int W>
+__device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
+                                           const __amdgpu_buffer_rsrc_t rt,
+                                           const double *__restrict__ scales, const Geom &g,
+                                           const SplitGeom &sg, int ps, int bi, int ci, int lane)
 {
-    const int total = ntasks * g.npairs;
-    const int L = xcd_remap(blockIdx.x, total);
-    const int t = L / g.npairs;
-    const int p = L - t * g.npairs;
-    const Task tk = tasks[t];
-    const short2 pr = pairs[p];
-    const int bi = pr.x, ci = pr.y;
-    const int lane = threadIdx.x;
     const int nvp = g.nvp;
-    const int ps = PAN_SLICE0 + 4 * tk.slice;
-    // panels of the four groups: {A0, A1, B0, B1}
-    const int GP[4][4] = {{PAN_LJ, PAN_LK, ps + SL_R1, ps + SL_R3},
-                          {ps + SL_LJT, ps + SL_LKT, PAN_R1T, PAN_R3T},
-                          {ps + SL_R1, ps + SL_R3, PAN_LJ, PAN_LK},
-                          {PAN_R1T, PAN_R3T, ps + SL_LJT, ps + SL_LKT}};
-    // product number of the combination (a, b) -> index a + 2*b
-    const int GPROD[4][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {8, 10, 9, 11}, {12, 14, 13, 15}};
-
-    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(digits),
-                                                  (uint32_t)(sg.npanels * S * sg.slice_bytes));
     const uint32_t plane = (uint32_t)sg.slice_bytes;
     const uint32_t blk8 = (uint32_t)sg.nkv * 1024;  // bytes of a 16-row block of one plane
     const uint32_t voff8 = lane * 16;
-
-    const int row0 = 4 * (lane >> 4), col = lane & 15;
-    const int cx = ci * 16 + col;
-
-    dbl4 facc[16];
+    const int row0 = 4 * (lane >> 4), cx = ci * 16 + (lane & 15);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-#pragma unroll
-    for (int grp = 0; grp < 4; ++grp) {
+    for (int gi = 0; gi < 2; ++gi) {
+        // panels {A0, A1, B0, B1}: W=0 groups n, t; W=1 groups n', t'
+        int A0, A1, B0, B1;
+        if (W == 0) {
+            A0 = gi == 0 ? PAN_LJ : ps + SL_LJT;  A1 = gi == 0 ? PAN_LK : ps + SL_LKT;
+            B0 = gi == 0 ? ps + SL_R1 : PAN_R1T;  B1 = gi == 0 ? ps + SL_R3 : PAN_R3T;
+        } else {
+            A0 = gi == 0 ? ps + SL_R1 : PAN_R1T;  A1 = gi == 0 ? ps + SL_R3 : PAN_R3T;
+            B0 = gi == 0 ? PAN_LJ : ps + SL_LJT;  B1 = gi == 0 ? PAN_LK : ps + SL_LKT;
+        }
         int4v iacc[4][S];
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int q = 0; q < S; ++q) iacc[c][q] = int4v{0, 0, 0, 0};
-        const uint32_t oa0 = (uint32_t)GP[grp][0] * S * plane + bi * blk8;
-        const uint32_t oa1 = (uint32_t)GP[grp][1] * S * plane + bi * blk8;
-        const uint32_t ob0 = (uint32_t)GP[grp][2] * S * plane + ci * blk8;
-        const uint32_t ob1 = (uint32_t)GP[grp][3] * S * plane + ci * blk8;
+        const uint32_t oa0 = (uint32_t)A0 * S * plane + bi * blk8;
+        const uint32_t oa1 = (uint32_t)A1 * S * plane + bi * blk8;
+        const uint32_t ob0 = (uint32_t)B0 * S * plane + ci * blk8;
+        const uint32_t ob1 = (uint32_t)B1 * S * plane + ci * blk8;
         for (int ks = 0; ks < sg.nkv; ++ks) {
             const uint32_t ko = ks * 1024;
             int4v a0[S], a1[S], b0[S], b1[S];
@@ -447,10 +430,13 @@ __global__ __launch_bounds__(64) void k_fused_split(const int8_t *__restrict__ d
                 }
             }
         }
-        // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * 2^(-7(L+2))
+        // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * 2^(-7(L+2)); the
+        // combination (a, b) is product 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int pa = GP[grp][c & 1], pb = GP[grp][2 + (c >> 1)];
+            const int ca = c & 1, cb = c >> 1;
+            const int pa = ca ? A1 : A0, pb = cb ? B1 : B0;
+            const int lp = 4 * gi + (W == 0 ? ca + 2 * cb : 2 * ca + cb);
             const double sB = scales[(int64_t)pb * nvp + cx];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -458,76 +444,142 @@ __global__ __launch_bounds__(64) void k_fused_split(const int8_t *__restrict__ d
 #pragma unroll
                 for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * __builtin_ldexp(1.0, -7 * (q + 2));
                 const double sA = scales[(int64_t)pa * nvp + bi * 16 + row0 + r];
-                facc[GPROD[grp][c]][r] += v * sA * sB;
+                facc[lp][r] += v * sA * sB;
             }
         }
     }
 
-    // FP64 tail (K = nocc)
-    {
-        const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
-        const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
-        const uint32_t blkt = (uint32_t)sg.nkt * 512;
-        const int li = lane & 15;
-        const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
-        const uint32_t voffB = lane * 8;
-        const int AP[8] = {PAN_LJ, PAN_LK, ps + SL_LJT, ps + SL_LKT, ps + SL_R1, ps + SL_R3, PAN_R1T, PAN_R3T};
-        const int BP[8] = {ps + SL_R1, ps + SL_R3, PAN_R1T, PAN_R3T, PAN_LJ, PAN_LK, ps + SL_LJT, ps + SL_LKT};
-        for (int ks = 0; ks < sg.nkt; ++ks) {
-            double a[8], b[8];
+    // FP64 tail (K = nocc) of the eight products
+    const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
+    const uint32_t blkt = (uint32_t)sg.nkt * 512;
+    const int li = lane & 15;
+    const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
+    const uint32_t voffB = lane * 8;
+    // W=0: A {LJ, LK, LJt, LKt} at b, B {R1, R3, R1t, R3t} at c; W=1 roles swapped
+    const int PL[4] = {PAN_LJ, PAN_LK, ps + SL_LJT, ps + SL_LKT};
+    const int PR[4] = {ps + SL_R1, ps + SL_R3, PAN_R1T, PAN_R3T};
+    uint32_t ao[4], bo[4];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                a[v] = buffer_load_f64(rt, voffA, AP[v] * tsb + bi * blkt + ks * 512);
-                b[v] = buffer_load_f64(rt, voffB, BP[v] * tsb + ci * blkt + ks * 512);
-            }
-#define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
-            MS(0, 0, 0);  MS(1, 1, 0);  MS(2, 0, 1);  MS(3, 1, 1);
-            MS(4, 2, 2);  MS(5, 3, 2);  MS(6, 2, 3);  MS(7, 3, 3);
-            MS(8, 4, 4);  MS(9, 4, 5);  MS(10, 5, 4); MS(11, 5, 5);
-            MS(12, 6, 6); MS(13, 6, 7); MS(14, 7, 6); MS(15, 7, 7);
-#undef MS
-        }
+    for (int v = 0; v < 4; ++v) {
+        ao[v] = (W == 0 ? PL[v] : PR[v]) * tsb + bi * blkt;
+        bo[v] = (W == 0 ? PR[v] : PL[v]) * tsb + ci * blkt;
     }
+    for (int ks = 0; ks < sg.nkt; ++ks) {
+        double a[4], b[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            a[v] = buffer_load_f64(rt, voffA, ao[v] + ks * 512);
+            b[v] = buffer_load_f64(rt, voffB, bo[v] + ks * 512);
+        }
+#define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
+        if (W == 0) {
+            MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);
+            MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);
+        } else {
+            MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);
+            MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);
+        }
+#undef MS
+    }
+}
+
+// Work unit of k_fused (one 16x16 block pair of one triple) split over two
+// waves: wave 0 computes the eight products at (b,c) (groups n and t), wave 1 the
+// eight mirrored ones f(c,b) (groups n' and t').  Each group shares two A and two
+// B panels.  After the INT8 groups each wave adds its FP64 tail (K = nocc) on
+// v_mfma_f64_16x16x4_f64 with the A rows fed in the order 4*(i%4) + i/4, which
+// makes its result layout (lane l, element r -> row 4*(l/16)+r, column l%16)
+// equal that of the INT8 MFMA.  The energy epilogue swaps half the values
+// through LDS and each wave finishes two of the four rows per lane.
+template <int S, bool TENGY, bool WRITEF>
+__global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ digits,
+                                                     const double *__restrict__ scales,
+                                                     const double *__restrict__ tail, Geom g,
+                                                     SplitGeom sg, const Task *__restrict__ tasks,
+                                                     int ntasks, const short2 *__restrict__ pairs,
+                                                     const double *__restrict__ ev,
+                                                     const double *__restrict__ vec6,
+                                                     double *__restrict__ partial,
+                                                     double *__restrict__ fout, int64_t fstride)
+{
+    __shared__ double xch[2][8][2][64];
+    __shared__ double red[2][4];
+    const int total = ntasks * g.npairs;
+    const int L = xcd_remap(blockIdx.x, total);
+    const int t = L / g.npairs;
+    const int p = L - t * g.npairs;
+    const Task tk = tasks[t];
+    const short2 pr = pairs[p];
+    const int bi = pr.x, ci = pr.y;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int nvp = g.nvp;
+    const int ps = PAN_SLICE0 + 4 * tk.slice;
+    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(digits),
+                                                  (uint32_t)(sg.npanels * S * sg.slice_bytes));
+    const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
+    const int row0 = 4 * (lane >> 4);
+    const int cx = ci * 16 + (lane & 15);
+
+    dbl4 facc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+    if (w == 0)
+        split_wave<S, 0>(facc, rd, rt, scales, g, sg, ps, bi, ci, lane);
+    else
+        split_wave<S, 1>(facc, rd, rt, scales, g, sg, ps, bi, ci, lane);
 
     const int nv = g.nvir;
     if (WRITEF) {
         double *fo = fout + (int64_t)t * fstride;
         const int64_t vv = (int64_t)nv * nv;
         const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};
+        if (w == 0 || bi != ci) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int bx = bi * 16 + row0 + r;
-            if (bx < nv && cx < nv) {
+            for (int r = 0; r < 4; ++r) {
+                const int bx = bi * 16 + row0 + r;
+                if (bx < nv && cx < nv) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) fo[mat[q] * vv + bx + (int64_t)cx * nv] = facc[q][r];
-                if (bi != ci) {
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) fo[mat[q] * vv + cx + (int64_t)bx * nv] = facc[8 + q][r];
+                    for (int q = 0; q < 8; ++q) {
+                        const int64_t o = (w == 0) ? bx + (int64_t)cx * nv : cx + (int64_t)bx * nv;
+                        fo[mat[q] * vv + o] = facc[q][r];
+                    }
                 }
             }
         }
     }
     if (TENGY) {
+        // wave 0 keeps rows r = 0,1 and receives the mirrored values for them;
+        // wave 1 keeps rows r = 2,3 and receives the direct values
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) xch[w][q][h][lane] = facc[q][(w == 0 ? 2 : 0) + h];
+        __syncthreads();
         Sum4 s = {0.0, 0.0, 0.0, 0.0};
         const double ec = ev[cx];
         const double u1c = vec6[cx], u2c = vec6[nvp + cx];
         const double x1c = vec6[2 * nvp + cx], x2c = vec6[3 * nvp + cx];
         const double y1c = vec6[4 * nvp + cx], y2c = vec6[5 * nvp + cx];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int h = 0; h < 2; ++h) {
+            const int r = (w == 0 ? 0 : 2) + h;
+            double dv[8], mv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double own = facc[q][r], other = xch[1 - w][q][h][lane];
+                dv[q] = (w == 0) ? own : other;
+                mv[q] = (w == 0) ? other : own;
+            }
             const int bx = bi * 16 + row0 + r;
             if (bx < nv && cx < nv) {
                 const double dd = -1.0 / (ev[bx] + ec + tk.eaijk);
-                tengy_point(facc[0][r], facc[1][r], facc[2][r], facc[3][r],
-                            facc[4][r], facc[5][r], facc[6][r], facc[7][r],
-                            facc[8][r], facc[9][r], facc[10][r], facc[11][r],
-                            facc[12][r], facc[13][r], facc[14][r], facc[15][r],
+                tengy_point(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
+                            mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
                             dd, vec6[bx], vec6[nvp + bx], x1c, x2c, y1c, y2c, s);
                 if (bi != ci)
-                    tengy_point(facc[8][r], facc[9][r], facc[10][r], facc[11][r],
-                                facc[12][r], facc[13][r], facc[14][r], facc[15][r],
-                                facc[0][r], facc[1][r], facc[2][r], facc[3][r],
-                                facc[4][r], facc[5][r], facc[6][r], facc[7][r],
+                    tengy_point(mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
+                                dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
                                 dd, u1c, u2c, vec6[2 * nvp + bx], vec6[3 * nvp + bx],
                                 vec6[4 * nvp + bx], vec6[5 * nvp + bx], s);
             }
@@ -537,9 +589,10 @@ __global__ __launch_bounds__(64) void k_fused_split(const int8_t *__restrict__ d
         s.e4k = wave_sum(s.e4k);
         s.e5k = wave_sum(s.e5k);
         if (lane == 0) {
-            double *o = partial + (int64_t)L * 4;
-            o[0] = s.e4i; o[1] = s.e5i; o[2] = s.e4k; o[3] = s.e5k;
+            red[w][0] = s.e4i; red[w][1] = s.e5i; red[w][2] = s.e4k; red[w][3] = s.e5k;
         }
+        __syncthreads();
+        if (threadIdx.x < 4) partial[(int64_t)L * 4 + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
     }
 }
 
@@ -550,7 +603,7 @@ static hipError_t launch_split_s(const int8_t *digits, const double *scales, con
                                  double *partial, double *fout, int64_t fstride, bool tengy,
                                  hipStream_t st)
 {
-    dim3 grid((unsigned)((int64_t)ntasks * g.npairs)), block(64);
+    dim3 grid((unsigned)((int64_t)ntasks * g.npairs)), block(128);
     if (tengy && fout)
         hipLaunchKernelGGL((k_fused_split<S, true, true>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
     else if (tengy)
