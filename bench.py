@@ -1,14 +1,21 @@
 """Benchmark of the CCSD(T) triples body on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--slices S]
 
 A step is one pass of the body over the whole triple list of the production
 case: nocc=30, nvir=300, k blocks of 16, all 4960 triples i >= j >= k, with the
-job's arrays already resident in HBM.  It prints one JSON line with the FP64-
-equivalent TFLOP/s (16*nvir^2*(nvir+nocc) flops per triple, 2 per multiply-add),
-the roofline of the dominant kernel measured with HIP events on its own
-stream, and the CPU oracle timed on the host's cores over a bounded sample of
-the same list.
+job's arrays already resident in HBM.  Both forms are timed in the same run:
+
+* the plain FP64 form (FP64 MFMA GEMMs with the fused energy epilogue), and
+* the split form (the K=nvir contractions as Ozaki INT8 split products with S
+  slices, FP64 recombination, FP64 K=nocc part and energy epilogue).
+
+`value` is the split form's FP64-equivalent TFLOP/s (16*nvir^2*(nvir+nocc) flops
+per triple, 2 per multiply-add) when its E[T] = emp4+emp5 agrees with the FP64
+form's to 1e-8 relative, else the FP64 form's.  The JSON line also carries the
+roofline of the dominant kernel (HIP events on the library's stream), per-step
+p50/p99 latency, the separate energy pass's HBM rate, and the CPU oracle timed
+on the host's cores over a bounded sample of the same list.
 """
 import argparse
 import json
@@ -22,10 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "(T) FP64-eq TFLOP/s nocc=30 nvir=300, split vs FP64 @rel dE<1e-8; job p99"
-FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
-HBM_PEAK = 8000.0      # GB/s
-KERNEL = {"fp64": "k_fused<true,false>", "fp64_split": "k_fused<false,true>",
-          "ozaki_int8": "k_fused_split<S,true,false>"}
+FP64_MFMA_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+INT8_MFMA_PEAK = 5000.0  # TOP/s, MI355X dense INT8 matrix
+HBM_PEAK = 8000.0        # GB/s
+ET_TOL = 1e-8
 
 
 def load_pkg():
@@ -40,8 +47,8 @@ def load_pkg():
 
 
 def traffic_from_profile(mode):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    """HBM-side bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_nocc30_nvir300.json" % mode)
     if not os.path.exists(path):
         return None
@@ -49,20 +56,35 @@ def traffic_from_profile(mode):
         return json.load(fh).get("bytes_per_launch")
 
 
-def cpu_baseline(cc, arrays, trip, flops, seconds):
+def time_form(job, trip, steps, warmup, **kw):
+    for _ in range(warmup):
+        job.run(trip, **kw)
+    step_ms, kern_ms, res = [], [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        res = job.run(trip, terms=True, **kw)
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        kern_ms.append(res["ms_main"])
+    wall = time.perf_counter() - t0
+    return {"ms_per_step": wall * 1e3 / steps, "kernel_ms": float(np.mean(kern_ms)),
+            "p50_ms": float(np.percentile(step_ms, 50)),
+            "p99_ms": float(np.percentile(step_ms, 99)), "res": res}
+
+
+def cpu_baseline(arrays, trip, flops, seconds):
     import oracle
-    provider = oracle.blas_select(0)
+    oracle.blas_select(0)
     cores = oracle.max_threads()
     out = oracle.run_list(arrays, 0, 30, 300, trip, max_seconds=seconds)
     done = int(out["done"])
     secs = float(np.sum(out["secs"]))
     return {"value": done * flops / secs / 1e12, "unit": "TFLOP/s", "cores": cores,
             "kind": "port",
-            "sample": "first %d of the %d triples, oracle body (%s DGEMM, %d OpenMP threads), "
-                      "per-call wall time summed as in omp/test.c:166-179"
+            "sample": "first %d of the %d triples through the oracle body (%s DGEMM, %d OpenMP "
+                      "threads), per-call wall time summed as in omp/test.c:166-179"
                       % (done, len(trip), oracle.blas_name(), cores),
-            "triples_per_s": done / secs, "provider": oracle.blas_name(), "_terms": out["terms"],
-            "_done": done}
+            "triples_per_s": done / secs}, out["terms"], done
 
 
 def main():
@@ -70,8 +92,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp64_split", "ozaki_int8"])
-    ap.add_argument("--slices", type=int, default=4)
+    ap.add_argument("--slices", type=int, default=4, help="Ozaki slices of the split form")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
@@ -85,66 +106,93 @@ def main():
 
     cc = load_pkg()
     cc.init(0)
-    nocc, nvir = 30, 300
+    nocc, nvir, S = 30, 300, args.slices
     arrays, blocks, trip, kchunk = cc.config_case(1)
     n = len(trip)
     flops = cc.gemm_flops_per_triple(nocc, nvir)
+    total_flops = n * flops
     job = cc.Job(arrays, nocc, nvir, kchunk)
-    for _ in range(args.warmup):
-        job.run(trip, mode=args.mode, slices=args.slices)
-    print("bench: warmup done (%d steps), timing %d steps" % (args.warmup, args.steps),
-          file=sys.stderr, flush=True)
-    step_ms, main_ms, res = [], [], None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        res = job.run(trip, mode=args.mode, slices=args.slices, terms=True)
-        step_ms.append((time.perf_counter() - s0) * 1e3)
-        main_ms.append(res["ms_main"])
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    value = n * flops / (ms_per_step * 1e-3) / 1e12
-    kern_ms = float(np.mean(main_ms))
-    achieved = n * flops / (kern_ms * 1e-3) / 1e12
 
-    # separate energy pass (GEMM writes f to HBM, tengy reads it back)
-    sp = job.run(trip, mode="fp64_split")
-    tengy_gbs = n * cc.tengy_bytes_per_triple(nvir) / (sp["ms_tengy"] * 1e-3) / 1e9
+    fp = time_form(job, trip, args.steps, args.warmup, mode="fp64")
+    print("bench: fp64 form %.2f ms/step" % fp["ms_per_step"], file=sys.stderr, flush=True)
+    sp = time_form(job, trip, args.steps, args.warmup, mode="ozaki_int8", slices=S)
+    print("bench: split form (s=%d) %.2f ms/step" % (S, sp["ms_per_step"]), file=sys.stderr,
+          flush=True)
+    sep = job.run(trip, mode="fp64_split")  # GEMM writes f to HBM, separate energy pass
+    job.close()
 
+    e_fp = fp["res"]["emp4"] + fp["res"]["emp5"]
+    e_sp = sp["res"]["emp4"] + sp["res"]["emp5"]
+    dev = abs(e_sp - e_fp) / abs(e_fp)
+    split_ok = dev <= ET_TOL
+    head = sp if split_ok else fp
+
+    def tf(ms):
+        return total_flops / (ms * 1e-3) / 1e12
+
+    # roofline of the dominant kernel of the headline form
+    if split_ok:
+        int8_ops = n * 2.0 * 8 * nvir ** 3 * S * (S + 1) / 2  # Ozaki products over K=nvir
+        fp64_part = n * 2.0 * 8 * nvir ** 2 * nocc           # K=nocc products in FP64
+        t = sp["kernel_ms"] * 1e-3
+        ideal = int8_ops / (INT8_MFMA_PEAK * 1e12) + fp64_part / (FP64_MFMA_PEAK * 1e12)
+        roof = {"bound": "mfma", "kernel": "k_fused_split<%d,true,false>" % S,
+                "achieved": int8_ops / t / 1e12, "peak": INT8_MFMA_PEAK, "unit": "TOP/s",
+                "frac": int8_ops / t / 1e12 / INT8_MFMA_PEAK,
+                "traffic": traffic_from_profile("split_s%d" % S),
+                "kernel_ms": sp["kernel_ms"], "int8_ops_per_launch": int8_ops,
+                "fp64_flops_per_launch": fp64_part,
+                "ideal_time_frac": ideal / t,
+                "fp64_eq_tflops": tf(sp["kernel_ms"])}
+    else:
+        roof = {"bound": "mfma", "kernel": "k_fused<true,false>",
+                "achieved": tf(fp["kernel_ms"]), "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
+                "frac": tf(fp["kernel_ms"]) / FP64_MFMA_PEAK,
+                "traffic": traffic_from_profile("fp64"), "kernel_ms": fp["kernel_ms"],
+                "flops_per_launch": total_flops}
+
+    tengy_gbs = n * cc.tengy_bytes_per_triple(nvir) / (sep["ms_tengy"] * 1e-3) / 1e9
     line = {
-        "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": tf(head["ms_per_step"]), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ("i8x%d+f64" % S) if split_ok else "f64",
         "data": "synthetic: seeded PCG inputs at the harness's array shapes",
-        "config": {"workload": "plain FP64 triples body, nocc=30 nvir=300, k blocks of 16, "
-                               "4960 triples (i>=j>=k), a = first virtual",
+        "config": {"workload": "triples body, nocc=30 nvir=300, k blocks of 16, 4960 triples "
+                               "(i>=j>=k), a = first virtual; headline = %s form"
+                               % ("split" if split_ok else "FP64"),
                    "nocc": nocc, "nvir": nvir, "kchunk": kchunk, "ntriples": n,
-                   "mode": args.mode},
-        "roofline": {"bound": "mfma", "kernel": KERNEL[args.mode], "achieved": achieved,
-                     "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_MFMA_PEAK,
-                     "traffic": traffic_from_profile(args.mode),
-                     "kernel_ms": kern_ms, "flops_per_launch": n * flops},
-        "job_p50_ms": float(np.percentile(step_ms, 50)),
-        "job_p99_ms": float(np.percentile(step_ms, 99)),
-        "tengy_pass": {"kernel": "k_tengy", "ms": sp["ms_tengy"], "GB_per_s": tengy_gbs,
+                   "mode": ("ozaki_int8 s=%d" % S) if split_ok else "fp64"},
+        "roofline": roof,
+        "job_p50_ms": head["p50_ms"], "job_p99_ms": head["p99_ms"],
+        "fp64_form": {"value": tf(fp["ms_per_step"]), "ms_per_step": fp["ms_per_step"],
+                      "kernel": "k_fused<true,false>", "kernel_ms": fp["kernel_ms"],
+                      "kernel_tflops": tf(fp["kernel_ms"]),
+                      "frac_fp64_peak": tf(fp["kernel_ms"]) / FP64_MFMA_PEAK,
+                      "traffic": traffic_from_profile("fp64"),
+                      "p50_ms": fp["p50_ms"], "p99_ms": fp["p99_ms"],
+                      "emp4": fp["res"]["emp4"], "emp5": fp["res"]["emp5"]},
+        "split_form": {"slices": S, "value": tf(sp["ms_per_step"]),
+                       "ms_per_step": sp["ms_per_step"], "kernel_ms": sp["kernel_ms"],
+                       "p50_ms": sp["p50_ms"], "p99_ms": sp["p99_ms"],
+                       "emp4": sp["res"]["emp4"], "emp5": sp["res"]["emp5"],
+                       "rel_dE_vs_fp64": dev, "within_tol": split_ok,
+                       "speedup_vs_fp64": fp["ms_per_step"] / sp["ms_per_step"]},
+        "tengy_pass": {"kernel": "k_tengy", "ms": sep["ms_tengy"], "GB_per_s": tengy_gbs,
                        "hbm_frac": tengy_gbs / HBM_PEAK,
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),
-                       "gemm_ms": sp["ms_main"]},
-        "emp4": res["emp4"], "emp5": res["emp5"],
+                       "gemm_only_ms": sep["ms_main"]},
     }
     if not args.no_cpu:
-        cb = cpu_baseline(cc, arrays, trip, flops, args.cpu_seconds)
-        done = cb.pop("_done")
-        ref_terms = cb.pop("_terms")
-        got = res["terms"][:done]
-        scale = np.maximum(np.abs(ref_terms), 1e-300)
+        cb, ref_terms, done = cpu_baseline(arrays, trip, flops, args.cpu_seconds)
         line["cpu_baseline"] = cb
-        line["parity_sample"] = {"triples": done,
-                                 "max_rel_err_terms": float(np.max(np.abs(got - ref_terms) / scale))}
+        got = fp["res"]["terms"][:done]
+        line["parity_sample"] = {
+            "triples": done,
+            "fp64_max_rel_err_terms": float(np.max(np.abs(got - ref_terms) /
+                                                   np.maximum(np.abs(ref_terms), 1e-300)))}
     else:
         line["cpu_baseline"] = None
-    job.close()
     if args.copy_calibration > 0:
         nbytes = int(args.copy_calibration * 2 ** 20)
         ms = cc.lib().ccsdt_debug_copy(nbytes, 3)
