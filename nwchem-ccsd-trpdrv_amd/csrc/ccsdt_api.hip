@@ -116,7 +116,7 @@ Geom make_geom(int ncor, int nocc, int nvir)
     g.nvp = (int)round_up(nvir, 16);
     g.nb = g.nvp / 16;
     g.kdim = nvir + nocc;
-    g.kp = (int)round_up(g.kdim, 8);  // an even number of 4-wide MFMA steps
+    g.kp = (int)round_up(g.kdim, 4);
     g.nks = g.kp / 4;
     g.npairs = g.nb * (g.nb + 1) / 2;
     g.vecsz = (int64_t)g.nvp * g.kp;

@@ -21,8 +21,8 @@ struct Geom {
     int nvp;     // nvir rounded up to 16 (MFMA block rows)
     int nb;      // nvp / 16
     int kdim;    // nvir + nocc: the concatenated contraction length
-    int kp;      // kdim rounded up to 8
-    int nks;     // kp / 4 MFMA contraction steps (even)
+    int kp;      // kdim rounded up to 4
+    int nks;     // kp / 4 MFMA contraction steps
     int npairs;  // nb*(nb+1)/2 lower-triangle 16x16 block pairs
     int64_t vecsz;  // doubles in one packed panel: nvp * kp
 };

@@ -127,6 +127,49 @@ hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipS
 // ------------------------------------------------------------------ fused GEMM + energy
 #define MF(p, ia, ib) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], acc[p], 0, 0, 0)
 
+// K loop of k_fused.  Ping-pong operand buffers X and Y, two K steps per trip:
+// each buffer's loads are in flight while the other buffer's MFMAs run; an odd
+// last step runs from X after the loop.  DIAG skips the eight mirrored products.
+template <bool DIAG>
+__device__ __forceinline__ void fused_mainloop(dbl4 (&acc)[16], const __amdgpu_buffer_rsrc_t rg,
+                                               const __amdgpu_buffer_rsrc_t rs,
+                                               const uint32_t (&AO)[8], const uint32_t (&BO)[8],
+                                               uint32_t voff, int nks)
+{
+    // which resource each operand lives in (true = the task's k-slice panels)
+    constexpr bool AS[8] = {false, false, true, true, true, true, false, false};
+    constexpr bool BS[8] = {true, true, false, false, false, false, true, true};
+    constexpr int NV = DIAG ? 4 : 8;
+    double xa[8], xb[8], ya[8], yb[8];
+#define STEP_LOAD(da, db, kstep)                                                              \
+    _Pragma("unroll") for (int v = 0; v < NV; ++v) {                                        \
+        da[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v] + (uint32_t)(kstep) * 512);     \
+        db[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v] + (uint32_t)(kstep) * 512);     \
+    }
+#define STEP_MFMA(da, db)                                                                     \
+    {                                                                                         \
+        double *a = da, *b = db;                                                              \
+        MF(0, 0, 0);  MF(1, 1, 0);  MF(2, 0, 1);  MF(3, 1, 1);                                \
+        MF(4, 2, 2);  MF(5, 3, 2);  MF(6, 2, 3);  MF(7, 3, 3);                                \
+        if (!DIAG) {                                                                          \
+            MF(8, 4, 4);  MF(9, 4, 5);  MF(10, 5, 4); MF(11, 5, 5);                           \
+            MF(12, 6, 6); MF(13, 6, 7); MF(14, 7, 6); MF(15, 7, 7);                           \
+        }                                                                                     \
+    }
+    STEP_LOAD(xa, xb, 0)
+    int ks = 0;
+    for (; ks + 1 < nks; ks += 2) {
+        STEP_LOAD(ya, yb, ks + 1)
+        STEP_MFMA(xa, xb)
+        const int k2 = (ks + 2 < nks) ? ks + 2 : nks - 1;  // clamped prefetch, in bounds
+        STEP_LOAD(xa, xb, k2)
+        STEP_MFMA(ya, yb)
+    }
+    if (nks & 1) STEP_MFMA(xa, xb)
+#undef STEP_LOAD
+#undef STEP_MFMA
+}
+
 template <bool TENGY, bool WRITEF>
 __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Geom g,
                                               const Task *__restrict__ tasks, int ntasks,
@@ -160,50 +203,29 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
     // B side, index c: R1, R3, R1t, R3t, LJ, LK, LJt, LKt
     const uint32_t BO[8] = {SL_R1 * vsb + oc, SL_R3 * vsb + oc, PAN_R1T * vsb + oc, PAN_R3T * vsb + oc,
                             PAN_LJ * vsb + oc, PAN_LK * vsb + oc, SL_LJT * vsb + oc, SL_LKT * vsb + oc};
-    // which resource each operand lives in (1 = slice panels)
-    constexpr bool AS[8] = {false, false, true, true, true, true, false, false};
-    constexpr bool BS[8] = {true, true, false, false, false, false, true, true};
-
     dbl4 acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    // Ping-pong operand buffers X and Y, two K steps per trip (nks is even):
-    // each buffer's loads are in flight while the other buffer's 16 MFMAs run.
-    double xa[8], xb[8], ya[8], yb[8];
+    // Diagonal pairs (B = C) compute only the eight products at (b,c): their
+    // mirrored values are the same products at (c,b), transposed through LDS below.
+    const bool diag = (bi == ci);
+    if (diag)
+        fused_mainloop<true>(acc, rg, rs, AO, BO, voff, g.nks);
+    else
+        fused_mainloop<false>(acc, rg, rs, AO, BO, voff, g.nks);
+    if (diag) {
+        // f(c,b) at (b,c) = product q at (c,b): element (row, col) sits in lane
+        // col + 16*(row%4), register row/4 (result layout below)
+        __shared__ double tr[16][17];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        xa[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v]);
-        xb[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v]);
-    }
-    const int nks = g.nks;
-    for (int ks = 0; ks < nks; ks += 2) {
-        const uint32_t o1 = (uint32_t)(ks + 1) * 512;
+        for (int q = 0; q < 8; ++q) {
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            ya[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v] + o1);
-            yb[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v] + o1);
-        }
-        {
-            double *a = xa, *b = xb;
-            MF(0, 0, 0);  MF(1, 1, 0);  MF(2, 0, 1);  MF(3, 1, 1);    // f1n f2n f3n f4n (b,c)
-            MF(4, 2, 2);  MF(5, 3, 2);  MF(6, 2, 3);  MF(7, 3, 3);    // f1t f2t f3t f4t (b,c)
-            MF(8, 4, 4);  MF(9, 4, 5);  MF(10, 5, 4); MF(11, 5, 5);   // f1n..f4n (c,b)
-            MF(12, 6, 6); MF(13, 6, 7); MF(14, 7, 6); MF(15, 7, 7);   // f1t..f4t (c,b)
-        }
-        const int k2 = (ks + 2 < nks) ? ks + 2 : ks;  // clamped prefetch, always in bounds
-        const uint32_t o2 = (uint32_t)k2 * 512;
+            for (int r = 0; r < 4; ++r) tr[(lane >> 4) + 4 * r][lane & 15] = acc[q][r];
+            __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            xa[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v] + o2);
-            xb[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v] + o2);
-        }
-        {
-            double *a = ya, *b = yb;
-            MF(0, 0, 0);  MF(1, 1, 0);  MF(2, 0, 1);  MF(3, 1, 1);
-            MF(4, 2, 2);  MF(5, 3, 2);  MF(6, 2, 3);  MF(7, 3, 3);
-            MF(8, 4, 4);  MF(9, 4, 5);  MF(10, 5, 4); MF(11, 5, 5);
-            MF(12, 6, 6); MF(13, 6, 7); MF(14, 7, 6); MF(15, 7, 7);
+            for (int r = 0; r < 4; ++r) acc[8 + q][r] = tr[lane & 15][(lane >> 4) + 4 * r];
+            __syncthreads();
         }
     }
 
@@ -524,9 +546,13 @@ __global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ 
     dbl4 facc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // On a diagonal pair (B = C) the mirrored products are the direct ones
+    // transposed: wave 1 skips its products and the epilogue reads both
+    // orientations from one LDS tile.
+    const bool diag = (bi == ci);
     if (w == 0)
         split_wave<S, 0>(facc, rd, rt, scales, g, sg, ps, bi, ci, lane);
-    else
+    else if (!diag)
         split_wave<S, 1>(facc, rd, rt, scales, g, sg, ps, bi, ci, lane);
 
     const int nv = g.nvir;
@@ -550,11 +576,23 @@ __global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ 
     }
     if (TENGY) {
         // wave 0 keeps rows r = 0,1 and receives the mirrored values for them;
-        // wave 1 keeps rows r = 2,3 and receives the direct values
+        // wave 1 keeps rows r = 2,3 and receives the direct values.  Diagonal
+        // pairs: wave 0's products go to an LDS tile T[q][row][col] that both
+        // waves read as (b,c) and transposed as (c,b).
+        double(*T)[16][16] = reinterpret_cast<double(*)[16][16]>(&xch[0][0][0][0]);
+        if (diag) {
+            if (w == 0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+                for (int q = 0; q < 8; ++q)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) xch[w][q][h][lane] = facc[q][(w == 0 ? 2 : 0) + h];
+                    for (int r = 0; r < 4; ++r) T[q][row0 + r][lane & 15] = facc[q][r];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) xch[w][q][h][lane] = facc[q][(w == 0 ? 2 : 0) + h];
+        }
         __syncthreads();
         Sum4 s = {0.0, 0.0, 0.0, 0.0};
         const double ec = ev[cx];
@@ -565,11 +603,19 @@ __global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ 
         for (int h = 0; h < 2; ++h) {
             const int r = (w == 0 ? 0 : 2) + h;
             double dv[8], mv[8];
+            if (diag) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const double own = facc[q][r], other = xch[1 - w][q][h][lane];
-                dv[q] = (w == 0) ? own : other;
-                mv[q] = (w == 0) ? other : own;
+                for (int q = 0; q < 8; ++q) {
+                    dv[q] = T[q][row0 + r][lane & 15];
+                    mv[q] = T[q][lane & 15][row0 + r];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const double own = facc[q][r], other = xch[1 - w][q][h][lane];
+                    dv[q] = (w == 0) ? own : other;
+                    mv[q] = (w == 0) ? other : own;
+                }
             }
             const int bx = bi * 16 + row0 + r;
             if (bx < nv && cx < nv) {
