@@ -178,10 +178,18 @@ enum {
     CCSDT_MODE_BF16X3 = 3       /* three-term BF16 split, FP32 accumulate */
 };
 
+/* Run flags. */
+enum {
+    CCSDT_RUN_PERSISTENT = 1,  /* fused kernel on a persistent grid (every workgroup slot
+                                  of the device, each looping over work items) */
+    CCSDT_RUN_GRAPH = 2        /* capture the run (uploads, launches, downloads) in a
+                                  hipGraph once per signature and replay it */
+};
+
 typedef struct {
     int mode;     /* CCSDT_MODE_* */
-    int slices;   /* Ozaki slice count (3..8), ignored otherwise */
-    int flags;    /* reserved, 0 */
+    int slices;   /* Ozaki slice count (3..7), ignored otherwise; 0 = 4 */
+    int flags;    /* CCSDT_RUN_* */
     int reserved;
 } ccsdt_run_opts;
 

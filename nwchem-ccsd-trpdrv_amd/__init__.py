@@ -31,6 +31,7 @@ ILLCOND_STREAM_BASE = 100   # exponent draws of the ill-conditioned variant
 JK_NAMES = ["jia", "jka", "kia", "kka", "jij", "jkj", "kij", "kkj"]
 
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3}
+RUN_PERSISTENT, RUN_GRAPH = 1, 2
 E_CODES = {0: "OK", -1: "E_ARG", -2: "E_DEVICE", -3: "E_NOMEM", -4: "E_BUDGET",
            -5: "E_STATE", -6: "E_INJECTED"}
 
@@ -281,10 +282,14 @@ class Job:
         self._keep = None
         self.handle = h
 
-    def run(self, triples, mode="fp64", slices=0, terms=False, fout=False):
+    def run(self, triples, mode="fp64", slices=0, terms=False, fout=False, persistent=False,
+            graph=False):
+        """Runs the triples; persistent / graph select CCSDT_RUN_PERSISTENT and
+        CCSDT_RUN_GRAPH."""
         trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
         n = trip.shape[0]
-        opts = RunOpts(MODES[mode], slices, 0, 0)
+        flags = (RUN_PERSISTENT if persistent else 0) | (RUN_GRAPH if graph else 0)
+        opts = RunOpts(MODES[mode], slices, flags, 0)
         res = RunResult()
         t = np.empty((n, 4)) if terms else None
         f = np.empty((8, self.nvir, self.nvir)) if fout else None

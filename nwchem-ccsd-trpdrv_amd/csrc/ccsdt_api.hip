@@ -78,6 +78,7 @@ int dev_alloc(void **p, size_t bytes)
 // ------------------------------------------------------------------ device
 std::mutex g_init_mu;
 int g_device = -1;
+int g_num_cu = 256;
 
 int ensure_device(int want)
 {
@@ -102,6 +103,7 @@ int ensure_device(int want)
                          prop.gcnArchName);
     HIP_TRY(hipSetDevice(dev));
     g_device = dev;
+    g_num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     return CCSDT_OK;
 }
 
@@ -193,6 +195,16 @@ struct ccsdt_job_s {
     int64_t cap_f = 0;
     double *d_out2 = nullptr;
     double *h_out = nullptr;  // pinned: 2 totals + 4 terms
+    // work-list tables of a run (pinned host copies feed the H2D copies)
+    JobDev *d_jobdev = nullptr, *h_jobdev = nullptr;
+    Segment *d_seg = nullptr, *h_seg = nullptr;
+    WorkHdr *d_hdr = nullptr, *h_hdr = nullptr;
+    double *h_terms = nullptr;
+    int64_t cap_hterms = 0;
+    // captured run (hipGraph) and the signature it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int64_t gkey[13] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // Ozaki INT8 split form (allocated on the first split run)
     int split_slices = 0;
     int8_t *d_digits = nullptr;
@@ -214,6 +226,15 @@ void job_free(ccsdt_job j)
     if (j->d_digits) (void)hipFree(j->d_digits);
     if (j->d_scales) (void)hipFree(j->d_scales);
     if (j->d_tail) (void)hipFree(j->d_tail);
+    if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+    if (j->graph) (void)hipGraphDestroy(j->graph);
+    if (j->d_jobdev) (void)hipFree(j->d_jobdev);
+    if (j->d_seg) (void)hipFree(j->d_seg);
+    if (j->d_hdr) (void)hipFree(j->d_hdr);
+    if (j->h_jobdev) (void)hipHostFree(j->h_jobdev);
+    if (j->h_seg) (void)hipHostFree(j->h_seg);
+    if (j->h_hdr) (void)hipHostFree(j->h_hdr);
+    if (j->h_terms) (void)hipHostFree(j->h_terms);
     if (j->d_pairs) (void)hipFree(j->d_pairs);
     if (j->d_tasks) (void)hipFree(j->d_tasks);
     if (j->h_tasks) (void)hipHostFree(j->h_tasks);
@@ -344,6 +365,12 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
     JOB_TRY(dev_alloc((void **)&j->d_ev, (size_t)g.nvp * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_vec6, (size_t)6 * g.nvp * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_out2, 8 * sizeof(double)));
+    JOB_TRY(dev_alloc((void **)&j->d_jobdev, sizeof(JobDev)));
+    JOB_TRY(dev_alloc((void **)&j->d_seg, sizeof(Segment)));
+    JOB_TRY(dev_alloc((void **)&j->d_hdr, sizeof(WorkHdr)));
+    JOB_HIP(hipHostMalloc((void **)&j->h_jobdev, sizeof(JobDev), 0));
+    JOB_HIP(hipHostMalloc((void **)&j->h_seg, sizeof(Segment), 0));
+    JOB_HIP(hipHostMalloc((void **)&j->h_hdr, sizeof(WorkHdr), 0));
     j->npanels = PAN_SLICE0 + 4 * j->kchunk;
     JOB_TRY(dev_alloc((void **)&j->d_pan, (size_t)j->npanels * g.vecsz * sizeof(double)));
 
@@ -407,8 +434,6 @@ int job_tasks(ccsdt_job j, const int *trip, int64_t n)
         // omp/ccsd_trpdrv_omp_cbody.c:126, same operation order
         tk.eaijk = e[a - 1] - (e[g.ncor + i - 1] + e[g.ncor + jj - 1] + e[g.ncor + k - 1]);
     }
-    HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice,
-                           j->st));
     return CCSDT_OK;
 }
 
@@ -422,6 +447,85 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Persistent grid of a fused kernel: every workgroup slot of the device, in a
+// multiple of 8 (one share per XCD).
+int persistent_grid(bool split, int slices)
+{
+    int g = g_num_cu * fused_blocks_per_cu(split, slices);
+    g = (g / 8) * 8;
+    return g < 8 ? 8 : g;
+}
+
+// Fills the one-job tables of a fused run (job 0, one segment of n tasks).
+void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f)
+{
+    JobDev &jd = *j->h_jobdev;
+    jd.g = j->g;
+    jd.sg = sg;
+    jd.pan = j->d_pan;
+    jd.digits = j->d_digits;
+    jd.scales = j->d_scales;
+    jd.tail = j->d_tail;
+    jd.ev = j->d_ev;
+    jd.vec6 = j->d_vec6;
+    jd.pairs = j->d_pairs;
+    Segment &sgm = *j->h_seg;
+    sgm.tasks = j->d_tasks;
+    sgm.partial = j->d_partial;
+    sgm.fout = with_f ? j->d_f : nullptr;
+    sgm.terms = j->d_terms;
+    sgm.job = 0;
+    sgm.item0 = 0;
+    sgm.ntasks = (int)n;
+    sgm.pad = 0;
+    WorkHdr &h = *j->h_hdr;
+    h.total = (int)(n * j->g.npairs);
+    h.nsegs = 1;
+    h.ntasks = (int)n;
+    h.pad = 0;
+}
+
+// The stream work of one fused run, from the task upload to the result
+// download (also what a captured graph holds).
+int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool persistent,
+                  bool with_f, double *terms_dst, bool timed, int64_t *launches)
+{
+    const Geom &g = j->g;
+    hipEvent_t *E = j->ev;
+    const bool split = (mode == CCSDT_MODE_OZAKI_INT8);
+    HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice, j->st));
+    HIP_TRY(hipMemcpyAsync(j->d_jobdev, j->h_jobdev, sizeof(JobDev), hipMemcpyHostToDevice, j->st));
+    HIP_TRY(hipMemcpyAsync(j->d_seg, j->h_seg, sizeof(Segment), hipMemcpyHostToDevice, j->st));
+    HIP_TRY(hipMemcpyAsync(j->d_hdr, j->h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, j->st));
+    if (timed) HIP_TRY(hipEventRecord(E[0], j->st));
+    if (split)
+        HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
+    else
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+    if (timed) HIP_TRY(hipEventRecord(E[1], j->st));
+    const int64_t items = n * g.npairs;
+    const int grid = persistent ? persistent_grid(split, sg.slices) : (int)items;
+    const int64_t vv = (int64_t)g.nvir * g.nvir;
+    const bool persist = persistent && !with_f;
+    if (split)
+        HIP_TRY(launch_fused_split(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
+                                   sg.slices, 8 * vv, true, with_f, j->st));
+    else
+        HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
+                             8 * vv, true, with_f, j->st));
+    if (timed) HIP_TRY(hipEventRecord(E[2], j->st));
+    const int rgrid = persistent ? (int)std::min<int64_t>(n, 8 * (int64_t)g_num_cu) : (int)n;
+    HIP_TRY(launch_reduce_segments(j->d_jobdev, j->d_seg, j->d_hdr, rgrid, j->st));
+    HIP_TRY(launch_reduce_list(j->d_terms, j->d_tasks, (int)n, j->d_out2, j->st));
+    if (timed) HIP_TRY(hipEventRecord(E[3], j->st));
+    HIP_TRY(hipMemcpyAsync(j->h_out, j->d_out2, 2 * sizeof(double), hipMemcpyDeviceToHost, j->st));
+    if (terms_dst)
+        HIP_TRY(hipMemcpyAsync(terms_dst, j->d_terms, (size_t)4 * n * sizeof(double),
+                               hipMemcpyDeviceToHost, j->st));
+    *launches = 4;
+    return CCSDT_OK;
+}
+
 // Runs n triples.  fouts (8 host pointers or null) receives the f matrices of a
 // single-triple run.
 int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
@@ -431,8 +535,11 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     if (n < 1 || !trip) return set_error(CCSDT_E_ARG, "empty triple list");
     if (fouts && n != 1) return set_error(CCSDT_E_ARG, "f output needs a single triple");
     const int mode = opts ? opts->mode : CCSDT_MODE_FP64;
+    const int flags = opts ? opts->flags : 0;
     if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT && mode != CCSDT_MODE_OZAKI_INT8)
         return set_error(CCSDT_E_ARG, "mode %d not available in this build", mode);
+    if (flags & ~(CCSDT_RUN_PERSISTENT | CCSDT_RUN_GRAPH))
+        return set_error(CCSDT_E_ARG, "unknown run flags 0x%x", flags);
     const int slices = (mode == CCSDT_MODE_OZAKI_INT8) ? (opts && opts->slices ? opts->slices : 4) : 0;
     if (mode == CCSDT_MODE_OZAKI_INT8 && (slices < 3 || slices > 7))
         return set_error(CCSDT_E_ARG, "Ozaki slice count %d outside 3..7", slices);
@@ -467,6 +574,8 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             if ((rc = dev_alloc((void **)&j->d_tail, (size_t)tb))) return rc;
             j->split_slices = slices;
         }
+    } else {
+        sg.slices = 0;
     }
     if ((rc = grow(&j->d_partial, &j->cap_partial, npart))) return rc;
     if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * n))) return rc;
@@ -482,53 +591,96 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     hipEvent_t *E = j->ev;
     int64_t launches = 0;
     double ms_main = 0.0, ms_tengy = 0.0;
+    const bool persistent = (flags & CCSDT_RUN_PERSISTENT) != 0;
+    const bool use_graph = (flags & CCSDT_RUN_GRAPH) != 0 && fused && !fouts;
     INJECT_POINT("launch");
-    HIP_TRY(hipEventRecord(E[0], j->st));
-    if (mode == CCSDT_MODE_OZAKI_INT8)
-        HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
-    else
-        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
-    ++launches;
-    HIP_TRY(hipEventRecord(E[1], j->st));
-    if (mode == CCSDT_MODE_OZAKI_INT8) {
-        HIP_TRY(launch_fused_split(j->d_digits, j->d_scales, j->d_tail, g, sg, j->d_tasks, (int)n,
-                                   j->d_pairs, j->d_ev, j->d_vec6, j->d_partial,
-                                   fouts ? j->d_f : nullptr, 8 * vv, true, j->st));
-        HIP_TRY(hipEventRecord(E[2], j->st));
-        HIP_TRY(launch_reduce_terms(j->d_partial, g.npairs, (int)n, j->d_terms, j->st));
-        launches += 2;
-    } else if (mode == CCSDT_MODE_FP64) {
-        HIP_TRY(launch_fused(j->d_pan, g, j->d_tasks, (int)n, j->d_pairs, j->d_ev, j->d_vec6,
-                             j->d_partial, fouts ? j->d_f : nullptr, 8 * vv, true, j->st));
-        HIP_TRY(hipEventRecord(E[2], j->st));
-        HIP_TRY(launch_reduce_terms(j->d_partial, g.npairs, (int)n, j->d_terms, j->st));
-        launches += 2;
+    if (fused) {
+        fill_tables(j, sg, n, fouts != nullptr);
+        if (use_graph) {
+            if (terms && n > j->cap_hterms) {
+                if (j->h_terms) (void)hipHostFree(j->h_terms);
+                j->h_terms = nullptr;
+                j->cap_hterms = 0;
+                HIP_TRY(hipHostMalloc((void **)&j->h_terms, (size_t)4 * n * sizeof(double), 0));
+                j->cap_hterms = n;
+            }
+            // the signature, and every buffer the captured nodes reference (a run
+            // of another size may have reallocated them)
+            const int64_t key[13] = {mode, slices, n, persistent, terms != nullptr,
+                                     (int64_t)(intptr_t)j->d_scales,
+                                     (int64_t)(intptr_t)j->d_tasks, (int64_t)(intptr_t)j->h_tasks,
+                                     (int64_t)(intptr_t)j->d_partial, (int64_t)(intptr_t)j->d_terms,
+                                     (int64_t)(intptr_t)j->h_terms, (int64_t)(intptr_t)j->d_digits,
+                                     (int64_t)(intptr_t)j->d_tail};
+            bool same = j->gexec != nullptr;
+            for (int q = 0; q < 13 && same; ++q) same = (key[q] == j->gkey[q]);
+            if (!same) {
+                if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+                if (j->graph) (void)hipGraphDestroy(j->graph);
+                j->gexec = nullptr;
+                j->graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(j->st, hipStreamCaptureModeThreadLocal));
+                int64_t l = 0;
+                rc = enqueue_fused(j, mode, sg, n, persistent, false, terms ? j->h_terms : nullptr,
+                                   false, &l);
+                hipGraph_t gr = nullptr;
+                hipError_t ec = hipStreamEndCapture(j->st, &gr);
+                if (rc) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return rc;
+                }
+                if (ec != hipSuccess)
+                    return set_error(CCSDT_E_DEVICE, "graph capture failed: %s", hipGetErrorString(ec));
+                j->graph = gr;
+                HIP_TRY(hipGraphInstantiate(&j->gexec, gr, nullptr, nullptr, 0));
+                for (int q = 0; q < 13; ++q) j->gkey[q] = key[q];
+            }
+            HIP_TRY(hipEventRecord(E[0], j->st));
+            HIP_TRY(hipGraphLaunch(j->gexec, j->st));
+            HIP_TRY(hipEventRecord(E[3], j->st));
+            launches = 1;
+        } else {
+            if ((rc = enqueue_fused(j, mode, sg, n, persistent, fouts != nullptr, terms, true, &launches)))
+                return rc;
+        }
     } else {
+        HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice, j->st));
+        HIP_TRY(hipEventRecord(E[0], j->st));
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+        HIP_TRY(hipEventRecord(E[1], j->st));
+        launches = 1;
+        fill_tables(j, sg, n, true);
         for (int64_t t0 = 0; t0 < n; t0 += chunk) {
             const int m = (int)std::min<int64_t>(chunk, n - t0);
+            j->h_seg->tasks = j->d_tasks + t0;
+            j->h_seg->ntasks = m;
+            j->h_hdr->total = m * g.npairs;
+            j->h_hdr->ntasks = m;
+            HIP_TRY(hipMemcpyAsync(j->d_jobdev, j->h_jobdev, sizeof(JobDev), hipMemcpyHostToDevice, j->st));
+            HIP_TRY(hipMemcpyAsync(j->d_seg, j->h_seg, sizeof(Segment), hipMemcpyHostToDevice, j->st));
+            HIP_TRY(hipMemcpyAsync(j->d_hdr, j->h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, j->st));
             HIP_TRY(hipEventRecord(E[4], j->st));
-            HIP_TRY(launch_fused(j->d_pan, g, j->d_tasks + t0, m, j->d_pairs, j->d_ev, j->d_vec6,
-                                 nullptr, j->d_f, 8 * vv, false, j->st));
+            HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, m * g.npairs, false, 8 * vv,
+                                 false, true, j->st));
             HIP_TRY(hipEventRecord(E[5], j->st));
             HIP_TRY(launch_tengy(j->d_f, 8 * vv, g, j->d_tasks + t0, m, j->d_ev, j->d_vec6,
                                  j->d_partial + t0 * tnp * 4, j->st));
             HIP_TRY(hipEventRecord(E[6], j->st));
             launches += 2;
-            HIP_TRY(hipEventSynchronize(E[6]));
+            HIP_TRY(hipEventSynchronize(E[6]));  // the pinned tables are rewritten next chunk
             ms_main += ev_ms(E[4], E[5]);
             ms_tengy += ev_ms(E[5], E[6]);
         }
         HIP_TRY(hipEventRecord(E[2], j->st));
         HIP_TRY(launch_reduce_terms(j->d_partial, tnp, (int)n, j->d_terms, j->st));
-        ++launches;
+        HIP_TRY(launch_reduce_list(j->d_terms, j->d_tasks, (int)n, j->d_out2, j->st));
+        launches += 2;
+        HIP_TRY(hipEventRecord(E[3], j->st));
+        HIP_TRY(hipMemcpyAsync(j->h_out, j->d_out2, 2 * sizeof(double), hipMemcpyDeviceToHost, j->st));
+        if (terms)
+            HIP_TRY(hipMemcpyAsync(terms, j->d_terms, (size_t)4 * n * sizeof(double),
+                                   hipMemcpyDeviceToHost, j->st));
     }
-    HIP_TRY(launch_reduce_list(j->d_terms, j->d_tasks, (int)n, j->d_out2, j->st));
-    ++launches;
-    HIP_TRY(hipEventRecord(E[3], j->st));
-    HIP_TRY(hipMemcpyAsync(j->h_out, j->d_out2, 2 * sizeof(double), hipMemcpyDeviceToHost, j->st));
-    if (terms)
-        HIP_TRY(hipMemcpyAsync(terms, j->d_terms, (size_t)4 * n * sizeof(double),
-                               hipMemcpyDeviceToHost, j->st));
     if (fouts) {
         for (int m = 0; m < 8; ++m)
             if (fouts[m])
@@ -536,14 +688,21 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
                                        hipMemcpyDeviceToHost, j->st));
     }
     HIP_TRY(hipStreamSynchronize(j->st));
+    if (use_graph && terms) memcpy(terms, j->h_terms, (size_t)4 * n * sizeof(double));
     if (res) {
         res->emp4 = j->h_out[0];
         res->emp5 = j->h_out[1];
         res->ms_total = ev_ms(E[0], E[3]);
-        res->ms_pack = ev_ms(E[0], E[1]);
-        res->ms_main = fused ? ev_ms(E[1], E[2]) : ms_main;
+        if (use_graph) {
+            res->ms_pack = 0.0;
+            res->ms_main = res->ms_total;
+            res->ms_reduce = 0.0;
+        } else {
+            res->ms_pack = ev_ms(E[0], E[1]);
+            res->ms_main = fused ? ev_ms(E[1], E[2]) : ms_main;
+            res->ms_reduce = ev_ms(E[2], E[3]);
+        }
         res->ms_tengy = ms_tengy;
-        res->ms_reduce = ev_ms(E[2], E[3]);
         res->ntriples = n;
         res->launches = launches;
     }

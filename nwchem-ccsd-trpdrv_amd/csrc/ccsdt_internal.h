@@ -57,12 +57,47 @@ struct SplitGeom {
     int npanels;          // panels of the job (bounds of the buffer resources)
 };
 
+// Everything a kernel needs to know about one job (device-resident table).
+struct JobDev {
+    Geom g;
+    SplitGeom sg;
+    const double *pan;      // FP64 packed panels
+    const int8_t *digits;   // split form: digit planes
+    const double *scales;   // split form: per-row power-of-two scales
+    const double *tail;     // split form: FP64 K = nocc panels
+    const double *ev;       // virtual orbital energies, padded to nvp
+    const double *vec6;     // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2, each nvp
+    const short2 *pairs;    // lower-triangle 16x16 block pairs
+};
+
+// A run of consecutive tasks of one job inside a launch.  Work items are
+// (task, block pair); the segment's items start at item0.
+struct Segment {
+    const Task *tasks;   // the segment's first task
+    double *partial;     // 4 doubles per (task, pair), task-major, from the first task
+    double *fout;        // f matrices of the first task (fstride apart) or null
+    double *terms;       // 4 doubles per task (emp4i, emp5i, emp4k, emp5k)
+    int job;             // index into the JobDev table
+    int item0;           // first work item of the segment
+    int ntasks;
+    int pad;
+};
+
+// Size of the current work list, read by the kernels (so a persistent launch
+// of fixed shape can be replayed on new lists).
+struct WorkHdr {
+    int total;    // work items
+    int nsegs;
+    int ntasks;   // tasks over all segments
+    int pad;
+};
+
 // Launchers (ccsdt_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st);
-hipError_t launch_fused(const double *pan, const Geom &g, const Task *tasks, int ntasks,
-                        const short2 *pairs, const double *ev, const double *vec6,
-                        double *partial, double *fout, int64_t fstride, bool tengy,
-                        hipStream_t st);
+// grid = number of workgroups: the item count for a plain launch, or a
+// multiple of 8 for a persistent one (each workgroup then loops over items).
+hipError_t launch_fused(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr, int grid,
+                        bool persist, int64_t fstride, bool tengy, bool writef, hipStream_t st);
 hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
                         int ntasks, const double *ev, const double *vec6, double *partial,
                         hipStream_t st);
@@ -70,11 +105,14 @@ int tengy_npairs(const Geom &g);
 hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
                              const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
                              hipStream_t st);
-hipError_t launch_fused_split(const int8_t *digits, const double *scales, const double *tail,
-                              const Geom &g, const SplitGeom &sg, const Task *tasks, int ntasks,
-                              const short2 *pairs, const double *ev, const double *vec6,
-                              double *partial, double *fout, int64_t fstride, bool tengy,
-                              hipStream_t st);
+hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+                              int grid, bool persist, int slices, int64_t fstride, bool tengy,
+                              bool writef, hipStream_t st);
+// Per-task sums of the pair partials over all segments (grid may be persistent).
+hipError_t launch_reduce_segments(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+                                  int grid, hipStream_t st);
+// Resident workgroups per CU of the fused kernels (for persistent grids).
+int fused_blocks_per_cu(bool split, int slices);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,
                                hipStream_t st);
 hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks, double *out2,

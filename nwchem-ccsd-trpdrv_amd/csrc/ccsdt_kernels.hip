@@ -90,6 +90,41 @@ __device__ __forceinline__ int xcd_remap(int bid, int total)
     return x * q + min(x, r) + idx;
 }
 
+// Work schedule shared by the plain and the persistent launches: the items are
+// split into 8 contiguous ranges, one per XCD (the hardware deals workgroup b to
+// XCD b % 8), and the workgroups of an XCD stride through its range.  With one
+// workgroup per item this is the same mapping as xcd_remap; with a persistent
+// grid (a multiple of 8 workgroups) each workgroup loops.
+struct ItemRange {
+    int first, end, stride;
+};
+
+__device__ __forceinline__ ItemRange item_range(int total, int grid, int b)
+{
+    const int x = b & 7, j = b >> 3;
+    const int q = total >> 3, r = total & 7;
+    const int start = x * q + min(x, r);
+    ItemRange ir;
+    ir.first = start + j;
+    ir.end = start + q + (x < r ? 1 : 0);
+    ir.stride = (grid >> 3) + (x < (grid & 7) ? 1 : 0);
+    return ir;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Segment holding work item `item` (segments sorted by item0; wave-uniform).
+__device__ __forceinline__ int find_segment(const Segment *__restrict__ segs, int nsegs, int item)
+{
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].item0 <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------------ packing
 __global__ __launch_bounds__(256) void k_pack(const PanelDesc *__restrict__ descs, Geom g)
 {
@@ -170,30 +205,22 @@ __device__ __forceinline__ void fused_mainloop(dbl4 (&acc)[16], const __amdgpu_b
 #undef STEP_MFMA
 }
 
+// One work item of k_fused: the 16x16 block pair (bi, ci) of one task.
 template <bool TENGY, bool WRITEF>
-__global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Geom g,
-                                              const Task *__restrict__ tasks, int ntasks,
-                                              const short2 *__restrict__ pairs,
-                                              const double *__restrict__ ev,
-                                              const double *__restrict__ vec6,
-                                              double *__restrict__ partial,
-                                              double *__restrict__ fout, int64_t fstride)
+__device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, int slice, int bi,
+                                           int ci, int lane, double *__restrict__ part4,
+                                           double *__restrict__ fo)
 {
-    const int total = ntasks * g.npairs;
-    const int L = xcd_remap(blockIdx.x, total);
-    const int t = L / g.npairs;
-    const int p = L - t * g.npairs;
-    const Task tk = tasks[t];
-    const short2 pr = pairs[p];
-    const int bi = pr.x, ci = pr.y;
-    const int lane = threadIdx.x;
-
+    const Geom &g = jd.g;
+    const double *__restrict__ pan = jd.pan;
+    const double *__restrict__ ev = jd.ev;
+    const double *__restrict__ vec6 = jd.vec6;
     // Two buffer resources: the job-wide panels (LJ, LK, R1t, R3t) and the four
     // panels of the task's k slice (R1, R3, LJt, LKt).  Byte offsets fit 32 bits
     // (checked on the host: 4 panels < 4 GiB).
     const uint32_t vsb = (uint32_t)(g.vecsz * 8);
     const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(pan, 4 * vsb);
-    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pan + (int64_t)(PAN_SLICE0 + 4 * tk.slice) * g.vecsz, 4 * vsb);
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pan + (int64_t)(PAN_SLICE0 + 4 * slice) * g.vecsz, 4 * vsb);
     const uint32_t blkb = (uint32_t)g.nks * 512;  // bytes of one 16-row block of a panel
     const uint32_t ob = bi * blkb, oc = ci * blkb;
     const uint32_t voff = lane * 8;
@@ -235,7 +262,6 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
     const int col = lane & 15, row0 = lane >> 4;
     const int cx = ci * 16 + col;
     if (WRITEF) {
-        double *fo = fout + (int64_t)t * fstride;
         const int64_t vv = (int64_t)nv * nv;
         const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};  // product -> body order f1n,f1t,...
 #pragma unroll
@@ -284,27 +310,50 @@ __global__ __launch_bounds__(64) void k_fused(const double *__restrict__ pan, Ge
         s.e4k = wave_sum(s.e4k);
         s.e5k = wave_sum(s.e5k);
         if (lane == 0) {
-            double *o = partial + (int64_t)L * 4;
-            o[0] = s.e4i; o[1] = s.e5i; o[2] = s.e4k; o[3] = s.e5k;
+            part4[0] = s.e4i; part4[1] = s.e5i; part4[2] = s.e4k; part4[3] = s.e5k;
         }
     }
 }
 #undef MF
 
-hipError_t launch_fused(const double *pan, const Geom &g, const Task *tasks, int ntasks,
-                        const short2 *pairs, const double *ev, const double *vec6,
-                        double *partial, double *fout, int64_t fstride, bool tengy,
-                        hipStream_t st)
+// PERSIST: the workgroup loops over its items (persistent grid); otherwise the
+// grid has one workgroup per item and no loop (the loop costs registers).
+template <bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
+                                              const Segment *__restrict__ segs,
+                                              const WorkHdr *__restrict__ hdr, int64_t fstride)
 {
-    const int64_t total = (int64_t)ntasks * g.npairs;
-    if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
-    dim3 grid((unsigned)total), block(64);
-    if (tengy && fout)
-        hipLaunchKernelGGL((k_fused<true, true>), grid, block, 0, st, pan, g, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    const WorkHdr h = *hdr;
+    const ItemRange ir = item_range(h.total, gridDim.x, blockIdx.x);
+    for (int item = uniform(ir.first); item < ir.end; item = PERSIST ? item + ir.stride : ir.end) {
+        const int si = uniform(find_segment(segs, h.nsegs, item));
+        const Segment sg = segs[si];
+        const JobDev jd = jobs[uniform(sg.job)];
+        const int L = uniform(item - sg.item0);
+        const int t = uniform(L / jd.g.npairs);
+        const int p = uniform(L - t * jd.g.npairs);
+        const Task tk = sg.tasks[t];
+        const short2 pr = jd.pairs[p];
+        fused_item<TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), threadIdx.x, sg.partial + (int64_t)L * 4,
+                                  WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
+    }
+}
+
+hipError_t launch_fused(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr, int grid,
+                        bool persist, int64_t fstride, bool tengy, bool writef, hipStream_t st)
+{
+    if (grid <= 0) return hipErrorInvalidValue;
+    dim3 g3((unsigned)grid), block(64);
+    if (persist && tengy && !writef)
+        hipLaunchKernelGGL((k_fused<true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (persist)
+        return hipErrorInvalidValue;
+    else if (tengy && writef)
+        hipLaunchKernelGGL((k_fused<true, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
     else if (tengy)
-        hipLaunchKernelGGL((k_fused<true, false>), grid, block, 0, st, pan, g, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
-    else if (fout)
-        hipLaunchKernelGGL((k_fused<false, true>), grid, block, 0, st, pan, g, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+        hipLaunchKernelGGL((k_fused<true, false, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (writef)
+        hipLaunchKernelGGL((k_fused<false, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -514,32 +563,22 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
 // equal that of the INT8 MFMA.  The energy epilogue swaps half the values
 // through LDS and each wave finishes two of the four rows per lane.
 template <int S, bool TENGY, bool WRITEF>
-__global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ digits,
-                                                     const double *__restrict__ scales,
-                                                     const double *__restrict__ tail, Geom g,
-                                                     SplitGeom sg, const Task *__restrict__ tasks,
-                                                     int ntasks, const short2 *__restrict__ pairs,
-                                                     const double *__restrict__ ev,
-                                                     const double *__restrict__ vec6,
-                                                     double *__restrict__ partial,
-                                                     double *__restrict__ fout, int64_t fstride)
+__device__ __forceinline__ void split_item(const JobDev &jd, const Task &tk, int slice, int bi,
+                                           int ci, int w, int lane, double *__restrict__ part4,
+                                           double *__restrict__ fo)
 {
     __shared__ double xch[2][8][2][64];
     __shared__ double red[2][4];
-    const int total = ntasks * g.npairs;
-    const int L = xcd_remap(blockIdx.x, total);
-    const int t = L / g.npairs;
-    const int p = L - t * g.npairs;
-    const Task tk = tasks[t];
-    const short2 pr = pairs[p];
-    const int bi = pr.x, ci = pr.y;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const Geom &g = jd.g;
+    const SplitGeom &sg = jd.sg;
+    const double *__restrict__ scales = jd.scales;
+    const double *__restrict__ ev = jd.ev;
+    const double *__restrict__ vec6 = jd.vec6;
     const int nvp = g.nvp;
-    const int ps = PAN_SLICE0 + 4 * tk.slice;
-    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(digits),
+    const int ps = PAN_SLICE0 + 4 * slice;
+    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
                                                   (uint32_t)(sg.npanels * S * sg.slice_bytes));
-    const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
+    const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
     const int row0 = 4 * (lane >> 4);
     const int cx = ci * 16 + (lane & 15);
 
@@ -557,7 +596,6 @@ __global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ 
 
     const int nv = g.nvir;
     if (WRITEF) {
-        double *fo = fout + (int64_t)t * fstride;
         const int64_t vv = (int64_t)nv * nv;
         const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};
         if (w == 0 || bi != ci) {
@@ -638,45 +676,92 @@ __global__ __launch_bounds__(128) void k_fused_split(const int8_t *__restrict__ 
             red[w][0] = s.e4i; red[w][1] = s.e5i; red[w][2] = s.e4k; red[w][3] = s.e5k;
         }
         __syncthreads();
-        if (threadIdx.x < 4) partial[(int64_t)L * 4 + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
+        if (threadIdx.x < 4) part4[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
+    }
+    __syncthreads();  // the LDS buffers are reused by the workgroup's next item
+}
+
+// The split form's work item (one 16x16 block pair of one task) on two waves.
+template <int S, bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(128) void k_fused_split(const JobDev *__restrict__ jobs,
+                                                     const Segment *__restrict__ segs,
+                                                     const WorkHdr *__restrict__ hdr, int64_t fstride)
+{
+    const WorkHdr h = *hdr;
+    const ItemRange ir = item_range(h.total, gridDim.x, blockIdx.x);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    for (int item = uniform(ir.first); item < ir.end; item = PERSIST ? item + ir.stride : ir.end) {
+        const int si = uniform(find_segment(segs, h.nsegs, item));
+        const Segment sg = segs[si];
+        const JobDev jd = jobs[uniform(sg.job)];
+        const int L = uniform(item - sg.item0);
+        const int t = uniform(L / jd.g.npairs);
+        const int p = uniform(L - t * jd.g.npairs);
+        const Task tk = sg.tasks[t];
+        const short2 pr = jd.pairs[p];
+        split_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane, sg.partial + (int64_t)L * 4,
+                                     WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }
 }
 
 template <int S>
-static hipError_t launch_split_s(const int8_t *digits, const double *scales, const double *tail,
-                                 const Geom &g, const SplitGeom &sg, const Task *tasks, int ntasks,
-                                 const short2 *pairs, const double *ev, const double *vec6,
-                                 double *partial, double *fout, int64_t fstride, bool tengy,
+static hipError_t launch_split_s(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+                                 int grid, bool persist, int64_t fstride, bool tengy, bool writef,
                                  hipStream_t st)
 {
-    dim3 grid((unsigned)((int64_t)ntasks * g.npairs)), block(128);
-    if (tengy && fout)
-        hipLaunchKernelGGL((k_fused_split<S, true, true>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+    dim3 g3((unsigned)grid), block(128);
+    if (persist && tengy && !writef)
+        hipLaunchKernelGGL((k_fused_split<S, true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (persist)
+        return hipErrorInvalidValue;
+    else if (tengy && writef)
+        hipLaunchKernelGGL((k_fused_split<S, true, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
     else if (tengy)
-        hipLaunchKernelGGL((k_fused_split<S, true, false>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
-    else if (fout)
-        hipLaunchKernelGGL((k_fused_split<S, false, true>), grid, block, 0, st, digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride);
+        hipLaunchKernelGGL((k_fused_split<S, true, false, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (writef)
+        hipLaunchKernelGGL((k_fused_split<S, false, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_fused_split(const int8_t *digits, const double *scales, const double *tail,
-                              const Geom &g, const SplitGeom &sg, const Task *tasks, int ntasks,
-                              const short2 *pairs, const double *ev, const double *vec6,
-                              double *partial, double *fout, int64_t fstride, bool tengy,
-                              hipStream_t st)
+hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+                              int grid, bool persist, int slices, int64_t fstride, bool tengy,
+                              bool writef, hipStream_t st)
 {
-    const int64_t total = (int64_t)ntasks * g.npairs;
-    if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
-    switch (sg.slices) {
-    case 3: return launch_split_s<3>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
-    case 4: return launch_split_s<4>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
-    case 5: return launch_split_s<5>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
-    case 6: return launch_split_s<6>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
-    case 7: return launch_split_s<7>(digits, scales, tail, g, sg, tasks, ntasks, pairs, ev, vec6, partial, fout, fstride, tengy, st);
+    if (grid <= 0) return hipErrorInvalidValue;
+    switch (slices) {
+    case 3: return launch_split_s<3>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+    case 4: return launch_split_s<4>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+    case 5: return launch_split_s<5>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+    case 6: return launch_split_s<6>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+    case 7: return launch_split_s<7>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// Resident workgroups per CU of each fused kernel, from the occupancy API.
+int fused_blocks_per_cu(bool split, int slices)
+{
+    int n = 0;
+    hipError_t e;
+    if (!split) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<true, false, true>, 64, 0);
+    } else {
+        switch (slices) {
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<3, true, false, true>, 128, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<4, true, false, true>, 128, 0); break;
+        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<5, true, false, true>, 128, 0); break;
+        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<6, true, false, true>, 128, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<7, true, false, true>, 128, 0); break;
+        }
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
 }
 
 // ------------------------------------------------------------------ standalone energy pass
@@ -801,6 +886,48 @@ hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, do
 {
     if (ntasks <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_reduce_terms, dim3(ntasks), dim3(64), 0, st, partial, npairs, terms);
+    return hipGetLastError();
+}
+
+// Per-task sums over the segments of a work list (fixed order, as above); the
+// terms go to each segment's terms array.  grid may be smaller than the task
+// count (the workgroups then stride).
+__global__ __launch_bounds__(64) void k_reduce_segments(const JobDev *__restrict__ jobs,
+                                                        const Segment *__restrict__ segs,
+                                                        const WorkHdr *__restrict__ hdr)
+{
+    const WorkHdr h = *hdr;
+    const int lane = threadIdx.x;
+    int sgi = 0, before = 0;
+    for (int gt = blockIdx.x; gt < h.ntasks; gt += gridDim.x) {
+        // segment of global task gt (segments hold consecutive task ranges)
+        while (sgi + 1 < h.nsegs && gt >= before + segs[sgi].ntasks) {
+            before += segs[sgi].ntasks;
+            ++sgi;
+        }
+        const Segment sg = segs[sgi];
+        const int npairs = jobs[sg.job].g.npairs;
+        const int t = gt - before;
+        const double *pp = sg.partial + (int64_t)t * npairs * 4;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int p = lane; p < npairs; p += 64) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[q] += pp[p * 4 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sg.terms[(int64_t)t * 4 + q] = s[q];
+        }
+    }
+}
+
+hipError_t launch_reduce_segments(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+                                  int grid, hipStream_t st)
+{
+    if (grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reduce_segments, dim3(grid), dim3(64), 0, st, jobs, segs, hdr);
     return hipGetLastError();
 }
 
