@@ -72,6 +72,53 @@ def time_form(job, trip, steps, warmup, **kw):
             "p99_ms": float(np.percentile(step_ms, 99)), "res": res}
 
 
+def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536):
+    """Feeds the jobs to a server at their arrival times (seconds after start)
+    and returns per-job latency and results."""
+    srv = cc.Server(mode=mode, slices=slices, items_per_tick=items_per_tick)
+    ids = [None] * len(jobs)
+    nxt = 0
+    start = srv.clock()
+    while True:
+        now = srv.clock() - start
+        while nxt < len(jobs) and arrivals[nxt] <= now:
+            jb = jobs[nxt]
+            ids[nxt] = srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+            nxt += 1
+        srv.progress()
+        c = srv.counts()
+        if nxt == len(jobs) and c["waiting"] == 0 and c["running"] == 0:
+            break
+        if c["running"] == 0 and nxt < len(jobs):
+            time.sleep(max(0.0, min(arrivals[nxt] - now, 0.002)))
+    res = [srv.result(i) for i in ids]
+    srv.close()
+    lat = np.array([(r["t_finish"] - r["t_submit"]) * 1e3 for r in res])
+    span = max(r["t_finish"] for r in res) - min(r["t_submit"] for r in res)
+    return res, lat, span
+
+
+def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True):
+    jobs = cc.serving_jobs(njobs, seed)
+    arrivals = cc.poisson_arrivals(njobs, rate, seed)
+    res, lat, span = run_serving(cc, jobs, arrivals, mode, slices)
+    flops = sum(len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) for jb in jobs)
+    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": mode if mode == "fp64" else "%s s=%d" % (mode, slices),
+           "jobs_per_s": njobs / span, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+           "fp64_eq_tflops_over_span": flops / span / 1e12}
+    if check:
+        # every job's E[T] against the same job run alone in the FP64 form
+        worst = 0.0
+        for jb, r in zip(jobs, res):
+            with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+                a = job.run(jb["triples"], mode="fp64")
+            E = a["emp4"] + a["emp5"]
+            worst = max(worst, abs(r["emp4"] + r["emp5"] - E) / abs(E))
+        out["max_rel_dE_vs_fp64_alone"] = worst
+    return out
+
+
 def cpu_baseline(arrays, trip, flops, seconds):
     import oracle
     oracle.blas_select(0)
@@ -95,6 +142,9 @@ def main():
     ap.add_argument("--slices", type=int, default=4, help="Ozaki slices of the split form")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving", default="", metavar="RATES",
+                    help="only run the serving loop at these Poisson rates (jobs/s, comma list)")
+    ap.add_argument("--serving-jobs", type=int, default=24)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (reference point for the "
                          "FETCH_SIZE/WRITE_SIZE counters)")
@@ -106,6 +156,13 @@ def main():
 
     cc = load_pkg()
     cc.init(0)
+    if args.serving:
+        rates = [float(x) for x in args.serving.split(",")]
+        reps = [serving_report(cc, r, args.serving_jobs, "ozaki_int8", args.slices) for r in rates]
+        reps += [serving_report(cc, rates[-1], args.serving_jobs, "fp64", 0)]
+        print(json.dumps({"metric": "serving: Poisson stream of triples jobs, nocc log-U[5,40], "
+                                    "nvir log-U[50,400], one GPU", "runs": reps}), flush=True)
+        return
     nocc, nvir, S = 30, 300, args.slices
     arrays, blocks, trip, kchunk = cc.config_case(1)
     n = len(trip)
@@ -183,6 +240,7 @@ def main():
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),
                        "gemm_only_ms": sep["ms_main"]},
     }
+    line["serving"] = serving_report(cc, 40.0, 16, "ozaki_int8", S, check=False)
     if not args.no_cpu:
         cb, ref_terms, done = cpu_baseline(arrays, trip, flops, args.cpu_seconds)
         line["cpu_baseline"] = cb

@@ -216,6 +216,51 @@ int ccsdt_job_destroy(ccsdt_job job);
 /* Device bytes the job holds (arrays + packed operands + workspace for n triples). */
 int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples);
 
+/* ---------------------------------------------------------------- serving */
+/* No counterpart in the reference (whose harness runs one job, omp/test.c).  A
+ * server keeps admitted jobs device-resident and interleaves their triples in
+ * shared launches ("ticks"): every tick takes a round-robin share of the
+ * remaining triples of every admitted job, so jobs that arrive while others run
+ * join the next tick.  Jobs are admitted in arrival order while the sum of their
+ * device footprints stays under the budget; the others wait.  Each job's totals
+ * are bitwise the same alone and in any mix (fixed per-job reduction order). */
+typedef struct ccsdt_server_s *ccsdt_server;
+
+typedef struct {
+    int64_t budget_bytes;    /* HBM budget for admitted jobs (0 = 200 GB) */
+    int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = 65536 */
+    int mode;                /* CCSDT_MODE_FP64 or CCSDT_MODE_OZAKI_INT8 */
+    int slices;              /* Ozaki slices (0 = 4) */
+    int reserved;
+} ccsdt_server_opts;
+
+typedef struct {
+    double emp4, emp5;       /* totals with the i != k rule, compensated */
+    double t_submit, t_admit, t_finish;  /* seconds on the server's clock */
+    int64_t ntriples;
+    int64_t footprint;       /* device bytes the job held */
+    int status;              /* CCSDT_OK or the error that failed the job */
+} ccsdt_job_result;
+
+int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out);
+/* Submits a job (arrays as for ccsdt_job_create, copied before return) and its
+ * triples; *id receives the job id.  CCSDT_E_BUDGET if the job alone exceeds
+ * the budget. */
+int ccsdt_server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples,
+                        int64_t n, int64_t *id);
+/* Retires finished ticks, finishes jobs, admits waiting jobs and launches ticks.
+ * block = 1 waits for the oldest tick in flight first.  Returns the number of
+ * jobs finished by this call, or a negative error. */
+int ccsdt_server_progress(ccsdt_server s, int block);
+/* 1 and *out filled when job id has finished, 0 while it is pending, negative
+ * for an unknown id. */
+int ccsdt_server_result(ccsdt_server s, int64_t id, ccsdt_job_result *out);
+/* Jobs waiting, admitted (running), finished; device bytes in use. */
+int ccsdt_server_counts(ccsdt_server s, int64_t *waiting, int64_t *running, int64_t *finished,
+                        int64_t *bytes_in_use);
+double ccsdt_server_clock(ccsdt_server s);  /* seconds since creation */
+int ccsdt_server_destroy(ccsdt_server s);
+
 /* ---------------------------------------------------------------- diagnostics */
 /* One v_mfma_f64_16x16x4_f64 on A (16x4 row major, 64 doubles) and B (4x16 row
  * major, 64 doubles); d256[4*lane + r] receives each lane's four results

@@ -63,6 +63,21 @@ class RunResult(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("items_per_tick", ctypes.c_int),
+                ("mode", ctypes.c_int), ("slices", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("emp4", ctypes.c_double), ("emp5", ctypes.c_double),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_finish", ctypes.c_double), ("ntriples", ctypes.c_int64),
+                ("footprint", ctypes.c_int64), ("status", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _LIB = None
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -96,6 +111,14 @@ def lib():
             "ccsdt_selftest_mfma": (ctypes.c_int, [vp, vp, vp]),
             "ccsdt_selftest_mfma_i8": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
             "ccsdt_debug_copy": (ctypes.c_double, [i64, ctypes.c_int]),
+            "ccsdt_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
+            "ccsdt_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(JobDesc), vp, i64,
+                                                   ctypes.POINTER(i64)]),
+            "ccsdt_server_progress": (ctypes.c_int, [vp, ctypes.c_int]),
+            "ccsdt_server_result": (ctypes.c_int, [vp, i64, ctypes.POINTER(JobResult)]),
+            "ccsdt_server_counts": (ctypes.c_int, [vp] + [ctypes.POINTER(i64)] * 4),
+            "ccsdt_server_clock": (ctypes.c_double, [vp]),
+            "ccsdt_server_destroy": (ctypes.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -320,6 +343,93 @@ class Job:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching server over the device (see ccsdt_server_* in ccsdt.h)."""
+
+    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0):
+        o = ServerOpts(int(budget_bytes), int(items_per_tick), MODES[mode], int(slices), 0)
+        h = ctypes.c_void_p()
+        _check(lib().ccsdt_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+
+    def submit(self, arrays, nocc, nvir, kchunk, triples, ncor=0):
+        d = JobDesc(ncor, nocc, nvir, kchunk)
+        keep = []
+        for q, name in enumerate(ARRAY_NAMES):
+            a = np.ascontiguousarray(arrays[name], dtype=np.float64)
+            keep.append(a)
+            d.arrays[q] = a.ctypes.data
+        trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
+        jid = ctypes.c_int64()
+        _check(lib().ccsdt_server_submit(self.handle, ctypes.byref(d), _ptr(trip), trip.shape[0],
+                                         ctypes.byref(jid)))
+        return jid.value
+
+    def progress(self, block=False):
+        rc = lib().ccsdt_server_progress(self.handle, 1 if block else 0)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def result(self, jid):
+        r = JobResult()
+        rc = lib().ccsdt_server_result(self.handle, jid, ctypes.byref(r))
+        if rc < 0:
+            _check(rc)
+        return r.as_dict() if rc == 1 else None
+
+    def counts(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(lib().ccsdt_server_counts(self.handle, *[ctypes.byref(x) for x in v]))
+        return {"waiting": v[0].value, "running": v[1].value, "finished": v[2].value,
+                "bytes_in_use": v[3].value}
+
+    def clock(self):
+        return lib().ccsdt_server_clock(self.handle)
+
+    def drain(self):
+        """Progresses until every submitted job has finished."""
+        while True:
+            c = self.counts()
+            if c["waiting"] == 0 and c["running"] == 0:
+                return
+            self.progress(block=True)
+
+    def close(self):
+        if self.handle:
+            lib().ccsdt_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def serving_jobs(njobs, seed, nocc_range=(5, 40), nvir_range=(50, 400)):
+    """Independent jobs of the serving workload: nocc and nvir log-uniform in
+    their ranges, inputs seeded per job as in the 8/40 case (one k block, all
+    i >= j >= k triples)."""
+    u = pcg_uniform(seed, 90, 0.0, 1.0, 2 * njobs)
+    jobs = []
+    for q in range(njobs):
+        nocc = int(round(np.exp(np.log(nocc_range[0]) + u[2 * q] *
+                                (np.log(nocc_range[1]) - np.log(nocc_range[0])))))
+        nvir = int(round(np.exp(np.log(nvir_range[0]) + u[2 * q + 1] *
+                                (np.log(nvir_range[1]) - np.log(nvir_range[0])))))
+        arrays = make_inputs(nocc, nvir, nocc, seed + 1000 + q)
+        trip = triple_list(nocc, blocks_uniform(nocc, nocc))
+        jobs.append({"nocc": nocc, "nvir": nvir, "kchunk": nocc, "arrays": arrays, "triples": trip})
+    return jobs
+
+
+def poisson_arrivals(njobs, rate, seed):
+    """Arrival times (seconds) of a Poisson stream of the given rate."""
+    u = pcg_uniform(seed, 91, 0.0, 1.0, njobs)
+    return np.cumsum(-np.log1p(-u) / rate)
 
 
 def footprint(nocc, nvir, kchunk, ntriples, ncor=0):

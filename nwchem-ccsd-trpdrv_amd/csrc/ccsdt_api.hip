@@ -3,8 +3,12 @@
 // error reporting and the seeded input generator).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <deque>
+#include <map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -447,6 +451,25 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Digit planes, scales and FP64 tail panels of the split form for sg.slices.
+int ensure_split(ccsdt_job j, const SplitGeom &sg)
+{
+    if (j->split_slices == sg.slices) return CCSDT_OK;
+    if (j->d_digits) (void)hipFree(j->d_digits);
+    if (j->d_scales) (void)hipFree(j->d_scales);
+    if (j->d_tail) (void)hipFree(j->d_tail);
+    j->d_digits = nullptr;
+    j->d_scales = nullptr;
+    j->d_tail = nullptr;
+    j->split_slices = 0;
+    int rc;
+    if ((rc = dev_alloc((void **)&j->d_digits, (size_t)j->npanels * sg.slices * sg.slice_bytes))) return rc;
+    if ((rc = dev_alloc((void **)&j->d_scales, (size_t)j->npanels * j->g.nvp * sizeof(double)))) return rc;
+    if ((rc = dev_alloc((void **)&j->d_tail, (size_t)j->npanels * sg.tail_sz * sizeof(double)))) return rc;
+    j->split_slices = sg.slices;
+    return CCSDT_OK;
+}
+
 // Persistent grid of a fused kernel: every workgroup slot of the device, in a
 // multiple of 8 (one share per XCD).
 int persistent_grid(bool split, int slices)
@@ -561,19 +584,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         if (dig >= (int64_t(1) << 32) || tb >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands exceed 4 GiB (nvir=%d kchunk=%d slices=%d)",
                              g.nvir, j->kchunk, slices);
-        if (j->split_slices != slices) {
-            if (j->d_digits) (void)hipFree(j->d_digits);
-            if (j->d_scales) (void)hipFree(j->d_scales);
-            if (j->d_tail) (void)hipFree(j->d_tail);
-            j->d_digits = nullptr;
-            j->d_scales = nullptr;
-            j->d_tail = nullptr;
-            j->split_slices = 0;
-            if ((rc = dev_alloc((void **)&j->d_digits, (size_t)dig))) return rc;
-            if ((rc = dev_alloc((void **)&j->d_scales, (size_t)j->npanels * g.nvp * sizeof(double)))) return rc;
-            if ((rc = dev_alloc((void **)&j->d_tail, (size_t)tb))) return rc;
-            j->split_slices = slices;
-        }
+        if ((rc = ensure_split(j, sg))) return rc;
     } else {
         sg.slices = 0;
     }
@@ -893,6 +904,266 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
 }
 }  // namespace
 
+
+// ------------------------------------------------------------------ serving
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct SJob {
+    int64_t id = 0;
+    ccsdt_job job = nullptr;
+    ccsdt_job_desc desc{};
+    std::vector<std::vector<double>> host;  // owned copies of the arrays while waiting
+    std::vector<int> trip;
+    int64_t n = 0, next = 0, done = 0;
+    int slot = -1;
+    int64_t footprint = 0;
+    double t_submit = 0.0, t_admit = 0.0, t_finish = 0.0;
+    int state = 0;  // 0 waiting, 1 running, 2 reducing, 3 finished, -1 failed
+    int status = CCSDT_OK;
+    hipEvent_t fin = nullptr;
+    double emp4 = 0.0, emp5 = 0.0;
+};
+
+struct TickBuf {
+    Segment *h_seg = nullptr, *d_seg = nullptr;
+    WorkHdr *h_hdr = nullptr, *d_hdr = nullptr;
+    hipEvent_t done = nullptr;
+    bool inflight = false;
+    std::vector<std::pair<int64_t, int64_t>> spans;  // (job id, tasks) of the tick
+};
+constexpr int kMaxSlots = 512;
+}  // namespace
+
+struct ccsdt_server_s {
+    ccsdt_server_opts o{};
+    hipStream_t st = nullptr;
+    JobDev *d_jobs = nullptr, *h_jobs = nullptr;
+    std::vector<int> free_slots;
+    std::map<int64_t, SJob *> jobs;
+    std::deque<int64_t> waiting;
+    std::vector<int64_t> running;
+    size_t rr = 0;
+    int64_t used = 0, nfinished = 0;
+    TickBuf tick[2];
+    std::deque<int> order;  // ticks in flight, launch order
+    int64_t next_id = 1;
+    Clock::time_point t0;
+    std::mutex mu;
+};
+
+namespace {
+double server_now(ccsdt_server s)
+{
+    return std::chrono::duration<double>(Clock::now() - s->t0).count();
+}
+
+int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n)
+{
+    int64_t b = ccsdt_job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n);
+    if (o.mode == CCSDT_MODE_OZAKI_INT8) {
+        const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
+        const int npanels = PAN_SLICE0 + 4 * d->kchunk;
+        const SplitGeom sg = make_split_geom(g, o.slices, npanels);
+        b += npanels * (o.slices * sg.slice_bytes + sg.tail_sz * 8 + (int64_t)g.nvp * 8);
+    }
+    return b;
+}
+
+// Uploads a job, packs its operands and publishes it in the job table.
+int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
+{
+    ccsdt_job_desc d = sj->desc;
+    for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q];
+    int rc = job_create(&d, &sj->job);
+    if (rc) return rc;
+    ccsdt_job j = sj->job;
+    const Geom &g = j->g;
+    if ((rc = job_tasks(j, sj->trip.data(), sj->n))) return rc;
+    if ((rc = grow(&j->d_partial, &j->cap_partial, sj->n * g.npairs * 4))) return rc;
+    if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * sj->n))) return rc;
+    SplitGeom sg{};
+    if (s->o.mode == CCSDT_MODE_OZAKI_INT8) {
+        sg = make_split_geom(g, s->o.slices, j->npanels);
+        if ((rc = ensure_split(j, sg))) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)sj->n * sizeof(Task), hipMemcpyHostToDevice, j->st));
+    if (s->o.mode == CCSDT_MODE_OZAKI_INT8)
+        HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
+    else
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+    sj->slot = s->free_slots.back();
+    s->free_slots.pop_back();
+    JobDev &jd = s->h_jobs[sj->slot];
+    jd.g = g;
+    jd.sg = sg;
+    jd.pan = j->d_pan;
+    jd.digits = j->d_digits;
+    jd.scales = j->d_scales;
+    jd.tail = j->d_tail;
+    jd.ev = j->d_ev;
+    jd.vec6 = j->d_vec6;
+    jd.pairs = j->d_pairs;
+    HIP_TRY(hipMemcpyAsync(s->d_jobs + sj->slot, &jd, sizeof(JobDev), hipMemcpyHostToDevice, j->st));
+    HIP_TRY(hipEventCreate(&sj->fin));
+    HIP_TRY(hipStreamSynchronize(j->st));
+    sj->state = 1;
+    sj->t_admit = server_now(s);
+    s->running.push_back(sj->id);
+    return CCSDT_OK;
+}
+
+void server_fail(ccsdt_server s, SJob *sj, int rc)
+{
+    sj->state = -1;
+    sj->status = rc;
+    sj->t_finish = server_now(s);
+    if (sj->job) {
+        job_free(sj->job);
+        sj->job = nullptr;
+    }
+    if (sj->slot >= 0) s->free_slots.push_back(sj->slot);
+    sj->slot = -1;
+    s->used -= sj->footprint;
+    s->running.erase(std::remove(s->running.begin(), s->running.end(), sj->id), s->running.end());
+    ++s->nfinished;
+}
+
+// Admits waiting jobs in arrival order while they fit.
+void server_admit_waiting(ccsdt_server s)
+{
+    while (!s->waiting.empty() && !s->free_slots.empty()) {
+        SJob *sj = s->jobs[s->waiting.front()];
+        if (s->used + sj->footprint > s->o.budget_bytes) break;
+        s->waiting.pop_front();
+        s->used += sj->footprint;
+        const double *arr[CCSDT_NARRAYS];
+        for (int q = 0; q < CCSDT_NARRAYS; ++q) arr[q] = sj->host[q].data();
+        const int rc = server_admit(s, sj, arr);
+        sj->host.clear();
+        sj->host.shrink_to_fit();
+        if (rc) server_fail(s, sj, rc);
+    }
+}
+
+// Builds and launches one tick from a free tick buffer; false when there is
+// nothing to run or no free buffer.
+int server_launch_tick(ccsdt_server s, bool *launched)
+{
+    *launched = false;
+    int tb = -1;
+    for (int q = 0; q < 2; ++q)
+        if (!s->tick[q].inflight) tb = q;
+    if (tb < 0 || s->running.empty()) return CCSDT_OK;
+    TickBuf &T = s->tick[tb];
+    T.spans.clear();
+    const int64_t budget = s->o.items_per_tick;
+    const size_t nr = s->running.size();
+    int64_t items = 0, tasks = 0;
+    int nseg = 0;
+    for (size_t q = 0; q < nr && items < budget && nseg < kMaxSlots; ++q) {
+        SJob *sj = s->jobs[s->running[(s->rr + q) % nr]];
+        const int64_t rem = sj->n - sj->next;
+        if (rem <= 0) continue;
+        const int np = sj->job->g.npairs;
+        const int64_t share = std::max<int64_t>(1, budget / (int64_t)nr / np);
+        const int64_t m = std::min(rem, share);
+        Segment &sg = T.h_seg[nseg++];
+        sg.tasks = sj->job->d_tasks + sj->next;
+        sg.partial = sj->job->d_partial + sj->next * np * 4;
+        sg.fout = nullptr;
+        sg.terms = sj->job->d_terms + sj->next * 4;
+        sg.job = sj->slot;
+        sg.item0 = (int)items;
+        sg.ntasks = (int)m;
+        sg.pad = 0;
+        items += m * np;
+        tasks += m;
+        sj->next += m;
+        T.spans.emplace_back(sj->id, m);
+    }
+    s->rr = (s->rr + 1) % nr;
+    if (nseg == 0) return CCSDT_OK;
+    T.h_hdr->total = (int)items;
+    T.h_hdr->nsegs = nseg;
+    T.h_hdr->ntasks = (int)tasks;
+    T.h_hdr->pad = 0;
+    HIP_TRY(hipMemcpyAsync(T.d_seg, T.h_seg, (size_t)nseg * sizeof(Segment), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(T.d_hdr, T.h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, s->st));
+    if (s->o.mode == CCSDT_MODE_OZAKI_INT8)
+        HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.slices, 0,
+                                   true, false, s->st));
+    else
+        HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, 0, true, false, s->st));
+    HIP_TRY(launch_reduce_segments(s->d_jobs, T.d_seg, T.d_hdr, (int)tasks, s->st));
+    HIP_TRY(hipEventRecord(T.done, s->st));
+    T.inflight = true;
+    s->order.push_back(tb);
+    *launched = true;
+    return CCSDT_OK;
+}
+
+// Retires finished ticks; jobs whose triples are all done get their totals
+// reduced on their own stream.
+int server_retire(ccsdt_server s, bool block)
+{
+    while (!s->order.empty()) {
+        TickBuf &T = s->tick[s->order.front()];
+        if (block) {
+            HIP_TRY(hipEventSynchronize(T.done));
+            block = false;
+        } else {
+            const hipError_t e = hipEventQuery(T.done);
+            if (e == hipErrorNotReady) break;
+            if (e != hipSuccess) return set_error(CCSDT_E_DEVICE, "tick failed: %s", hipGetErrorString(e));
+        }
+        for (auto &sp : T.spans) {
+            SJob *sj = s->jobs[sp.first];
+            sj->done += sp.second;
+            if (sj->done == sj->n && sj->state == 1) {
+                ccsdt_job j = sj->job;
+                HIP_TRY(launch_reduce_list(j->d_terms, j->d_tasks, (int)sj->n, j->d_out2, j->st));
+                HIP_TRY(hipMemcpyAsync(j->h_out, j->d_out2, 2 * sizeof(double), hipMemcpyDeviceToHost, j->st));
+                HIP_TRY(hipEventRecord(sj->fin, j->st));
+                sj->state = 2;
+                s->running.erase(std::remove(s->running.begin(), s->running.end(), sj->id), s->running.end());
+            }
+        }
+        T.inflight = false;
+        s->order.pop_front();
+    }
+    if (!s->running.empty()) s->rr %= s->running.size();
+    return CCSDT_OK;
+}
+
+// Completes jobs whose totals have arrived and releases their memory.
+int server_finish(ccsdt_server s)
+{
+    int nf = 0;
+    for (auto &kv : s->jobs) {
+        SJob *sj = kv.second;
+        if (sj->state != 2) continue;
+        const hipError_t e = hipEventQuery(sj->fin);
+        if (e == hipErrorNotReady) continue;
+        if (e != hipSuccess) return set_error(CCSDT_E_DEVICE, "job %lld failed: %s", (long long)sj->id, hipGetErrorString(e));
+        sj->emp4 = sj->job->h_out[0];
+        sj->emp5 = sj->job->h_out[1];
+        sj->t_finish = server_now(s);
+        sj->state = 3;
+        (void)hipEventDestroy(sj->fin);
+        sj->fin = nullptr;
+        job_free(sj->job);
+        sj->job = nullptr;
+        s->free_slots.push_back(sj->slot);
+        sj->slot = -1;
+        s->used -= sj->footprint;
+        ++s->nfinished;
+        ++nf;
+    }
+    return nf;
+}
+}  // namespace
+
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
@@ -984,6 +1255,176 @@ int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t nt
     const int64_t pan = (int64_t)(PAN_SLICE0 + 4 * kchunk) * g.vecsz;
     const int64_t per_task = (int64_t)g.npairs * 4 + 4 + (int64_t)sizeof(Task) / 8;
     return (raw + pan + 7 * g.nvp + ntriples * per_task) * (int64_t)sizeof(double);
+}
+
+
+int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
+{
+    if (!out) return set_error(CCSDT_E_ARG, "null output handle");
+    *out = nullptr;
+    ccsdt_server_opts o = opts ? *opts : ccsdt_server_opts{};
+    if (o.budget_bytes <= 0) o.budget_bytes = int64_t(200) * 1000 * 1000 * 1000;
+    if (o.items_per_tick <= 0) o.items_per_tick = 65536;
+    if (o.mode != CCSDT_MODE_FP64 && o.mode != CCSDT_MODE_OZAKI_INT8)
+        return set_error(CCSDT_E_ARG, "server mode %d must be FP64 or OZAKI_INT8", o.mode);
+    if (o.mode == CCSDT_MODE_OZAKI_INT8) {
+        if (o.slices == 0) o.slices = 4;
+        if (o.slices < 3 || o.slices > 7) return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
+    }
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    ccsdt_server s = new ccsdt_server_s();
+    s->o = o;
+    s->t0 = Clock::now();
+    auto fail = [&](int r) {
+        ccsdt_server_destroy(s);
+        return r;
+    };
+    if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_error(CCSDT_E_DEVICE, "stream creation failed"));
+    if ((rc = dev_alloc((void **)&s->d_jobs, kMaxSlots * sizeof(JobDev)))) return fail(rc);
+    if (hipHostMalloc((void **)&s->h_jobs, kMaxSlots * sizeof(JobDev), 0) != hipSuccess)
+        return fail(set_error(CCSDT_E_NOMEM, "pinned allocation failed"));
+    for (int q = 0; q < 2; ++q) {
+        TickBuf &T = s->tick[q];
+        if ((rc = dev_alloc((void **)&T.d_seg, kMaxSlots * sizeof(Segment)))) return fail(rc);
+        if ((rc = dev_alloc((void **)&T.d_hdr, sizeof(WorkHdr)))) return fail(rc);
+        if (hipHostMalloc((void **)&T.h_seg, kMaxSlots * sizeof(Segment), 0) != hipSuccess ||
+            hipHostMalloc((void **)&T.h_hdr, sizeof(WorkHdr), 0) != hipSuccess ||
+            hipEventCreateWithFlags(&T.done, hipEventDisableTiming) != hipSuccess)
+            return fail(set_error(CCSDT_E_NOMEM, "tick buffers failed"));
+    }
+    for (int q = kMaxSlots - 1; q >= 0; --q) s->free_slots.push_back(q);
+    *out = s;
+    t_err = CCSDT_OK;
+    return CCSDT_OK;
+}
+
+int ccsdt_server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples, int64_t n,
+                        int64_t *id)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (n < 1 || !triples) return set_error(CCSDT_E_ARG, "empty triple list");
+    const Geom g = make_geom(desc->ncor, desc->nocc, desc->nvir);
+    if (n * g.npairs > 0x7fffffff) return set_error(CCSDT_E_ARG, "triple list too long");
+    const int nbf = desc->ncor + desc->nocc + desc->nvir;
+    for (int64_t t = 0; t < n; ++t) {
+        const int *q = triples + 5 * t;
+        if (q[0] < 1 || q[0] > nbf || q[1] < 1 || q[1] > desc->nocc || q[2] < 1 || q[2] > desc->nocc ||
+            q[3] < 1 || q[3] > desc->nocc || q[4] < 1 || q[4] > q[3] || q[3] - q[4] >= desc->kchunk)
+            return set_error(CCSDT_E_ARG, "triple %lld outside the job's shape", (long long)t);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    SJob *sj = new SJob();
+    sj->id = s->next_id++;
+    sj->desc = *desc;
+    sj->n = n;
+    sj->trip.assign(triples, triples + 5 * n);
+    sj->footprint = server_footprint(s->o, desc, n);
+    sj->t_submit = server_now(s);
+    if (sj->footprint > s->o.budget_bytes) {
+        delete sj;
+        return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
+                         (long long)server_footprint(s->o, desc, n), (long long)s->o.budget_bytes);
+    }
+    s->jobs[sj->id] = sj;
+    if (id) *id = sj->id;
+    if (s->waiting.empty() && !s->free_slots.empty() && s->used + sj->footprint <= s->o.budget_bytes) {
+        s->used += sj->footprint;
+        rc = server_admit(s, sj, desc->arrays);  // straight from the caller's arrays
+        if (rc) server_fail(s, sj, rc);
+    } else {
+        int64_t cnt[CCSDT_NARRAYS];
+        array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt);
+        sj->host.resize(CCSDT_NARRAYS);
+        for (int q = 0; q < CCSDT_NARRAYS; ++q) sj->host[q].assign(desc->arrays[q], desc->arrays[q] + cnt[q]);
+        s->waiting.push_back(sj->id);
+    }
+    t_err = CCSDT_OK;
+    return CCSDT_OK;
+}
+
+int ccsdt_server_progress(ccsdt_server s, int block)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    int rc = server_retire(s, block != 0 && !s->order.empty());
+    if (rc) return rc;
+    const int nf = server_finish(s);
+    if (nf < 0) return nf;
+    server_admit_waiting(s);
+    bool launched = true;
+    while (launched) {
+        if ((rc = server_launch_tick(s, &launched))) return rc;
+    }
+    return nf;
+}
+
+int ccsdt_server_result(ccsdt_server s, int64_t id, ccsdt_job_result *out)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return set_error(CCSDT_E_ARG, "unknown job id %lld", (long long)id);
+    SJob *sj = it->second;
+    if (sj->state != 3 && sj->state != -1) return 0;
+    if (out) {
+        out->emp4 = sj->emp4;
+        out->emp5 = sj->emp5;
+        out->t_submit = sj->t_submit;
+        out->t_admit = sj->t_admit;
+        out->t_finish = sj->t_finish;
+        out->ntriples = sj->n;
+        out->footprint = sj->footprint;
+        out->status = sj->status;
+    }
+    return 1;
+}
+
+int ccsdt_server_counts(ccsdt_server s, int64_t *waiting, int64_t *running, int64_t *finished,
+                        int64_t *bytes_in_use)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    int64_t run = 0;
+    for (auto &kv : s->jobs)
+        if (kv.second->state == 1 || kv.second->state == 2) ++run;
+    if (waiting) *waiting = (int64_t)s->waiting.size();
+    if (running) *running = run;
+    if (finished) *finished = s->nfinished;
+    if (bytes_in_use) *bytes_in_use = s->used;
+    return CCSDT_OK;
+}
+
+double ccsdt_server_clock(ccsdt_server s) { return s ? server_now(s) : 0.0; }
+
+int ccsdt_server_destroy(ccsdt_server s)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    for (auto &kv : s->jobs) {
+        SJob *sj = kv.second;
+        if (sj->job) {
+            (void)hipStreamSynchronize(sj->job->st);
+            job_free(sj->job);
+        }
+        if (sj->fin) (void)hipEventDestroy(sj->fin);
+        delete sj;
+    }
+    for (int q = 0; q < 2; ++q) {
+        TickBuf &T = s->tick[q];
+        if (T.d_seg) (void)hipFree(T.d_seg);
+        if (T.d_hdr) (void)hipFree(T.d_hdr);
+        if (T.h_seg) (void)hipHostFree(T.h_seg);
+        if (T.h_hdr) (void)hipHostFree(T.h_hdr);
+        if (T.done) (void)hipEventDestroy(T.done);
+    }
+    if (s->d_jobs) (void)hipFree(s->d_jobs);
+    if (s->h_jobs) (void)hipHostFree(s->h_jobs);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+    return CCSDT_OK;
 }
 
 int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256)

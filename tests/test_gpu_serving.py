@@ -1,0 +1,79 @@
+"""Serving: continuous batching of independent jobs (needs a GPU).
+
+Each job's totals must be bitwise the same alone (a plain job run) and in the
+mix; jobs beyond the HBM budget wait and still finish; totals agree with the
+oracle at the FP64 tolerance (FP64 form) or within 1e-8 in E[T] (split form)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device(cc):
+    cc.init(0)
+
+
+def small_jobs(cc, n=5):
+    return cc.serving_jobs(n, 4242, nocc_range=(3, 9), nvir_range=(20, 70))
+
+
+@pytest.mark.parametrize("mode,slices", [("fp64", 0), ("ozaki_int8", 4)])
+def test_alone_equals_mixed(cc, orc, mode, slices):
+    jobs = small_jobs(cc)
+    alone = []
+    for jb in jobs:
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            alone.append(job.run(jb["triples"], mode=mode, slices=slices))
+    with cc.Server(mode=mode, slices=slices, items_per_tick=512) as srv:
+        ids = [srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+               for jb in jobs]
+        srv.drain()
+        mixed = [srv.result(i) for i in ids]
+    for jb, a, m in zip(jobs, alone, mixed):
+        assert m is not None and m["status"] == 0
+        assert (a["emp4"], a["emp5"]) == (m["emp4"], m["emp5"])
+        ref = orc.run_list(jb["arrays"], 0, jb["nocc"], jb["nvir"], jb["triples"])
+        E = ref["emp4"] + ref["emp5"]
+        tol = 1e-11 if mode == "fp64" else 1e-8
+        assert abs(m["emp4"] + m["emp5"] - E) <= tol * abs(E)
+
+
+def test_budget_makes_jobs_wait(cc):
+    jobs = small_jobs(cc, 4)
+    fps = [cc.footprint(jb["nocc"], jb["nvir"], jb["kchunk"], len(jb["triples"])) for jb in jobs]
+    with cc.Server(budget_bytes=max(fps) + 1, items_per_tick=256) as srv:
+        ids = [srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+               for jb in jobs]
+        c = srv.counts()
+        assert c["waiting"] >= 1 and c["bytes_in_use"] <= max(fps) + 1
+        srv.drain()
+        res = [srv.result(i) for i in ids]
+    assert all(r is not None and r["status"] == 0 for r in res)
+    # admitted in arrival order
+    admits = [r["t_admit"] for r in res]
+    assert admits == sorted(admits)
+
+
+def test_job_over_budget_rejected(cc):
+    jb = small_jobs(cc, 1)[0]
+    with cc.Server(budget_bytes=1000) as srv:
+        with pytest.raises(cc.CcsdtError) as e:
+            srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+        assert e.value.code == -4
+
+
+def test_arrivals_while_running(cc):
+    """Jobs submitted while earlier ticks are in flight join later ticks."""
+    jobs = cc.serving_jobs(6, 99, nocc_range=(6, 14), nvir_range=(60, 120))
+    with cc.Server(items_per_tick=2048) as srv:
+        ids = []
+        for jb in jobs:
+            ids.append(srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"]))
+            srv.progress()
+        srv.drain()
+        res = [srv.result(i) for i in ids]
+    for jb, r in zip(jobs, res):
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            a = job.run(jb["triples"])
+        assert (a["emp4"], a["emp5"]) == (r["emp4"], r["emp5"])
