@@ -66,17 +66,81 @@ bool injected()
         if (injected()) return set_error(CCSDT_E_INJECTED, "injected failure at %s", what); \
     } while (0)
 
+// Device memory comes from a caching pool: freed blocks are kept (up to
+// kPoolCacheMax bytes) and handed out again for requests of a similar size, so
+// jobs created and destroyed in a serving loop do not go through hipMalloc and
+// hipFree (which can stall the device) each time.
+std::mutex g_pool_mu;
+std::multimap<size_t, void *> g_pool_free;
+std::map<void *, size_t> g_pool_live;
+size_t g_pool_cached = 0;
+constexpr size_t kPoolCacheMax = size_t(96) << 30;
+
+size_t pool_size(size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    const size_t q = bytes <= (size_t(1) << 20) ? 4096 : (size_t(1) << 20);
+    return (bytes + q - 1) / q * q;
+}
+
+void pool_release_cached()
+{
+    for (auto &kv : g_pool_free) (void)hipFree(kv.second);
+    g_pool_free.clear();
+    g_pool_cached = 0;
+}
+
 int dev_alloc(void **p, size_t bytes)
 {
     INJECT_POINT("device allocation");
-    if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(p, bytes);
+    const size_t sz = pool_size(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_pool_free.lower_bound(sz);
+        if (it != g_pool_free.end() && it->first <= sz + sz / 4) {
+            *p = it->second;
+            g_pool_live[*p] = it->first;
+            g_pool_cached -= it->first;
+            g_pool_free.erase(it);
+            return CCSDT_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, sz);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        pool_release_cached();
+        e = hipMalloc(p, sz);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
-        return set_error(CCSDT_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return set_error(CCSDT_E_NOMEM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
     }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_live[*p] = sz;
     return CCSDT_OK;
+}
+
+// Returns a block from dev_alloc to the pool.  The caller guarantees no queued
+// or running work still uses it.
+void dev_free(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_pool_live.find(p);
+    if (it == g_pool_live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const size_t sz = it->second;
+    g_pool_live.erase(it);
+    if (g_pool_cached + sz <= kPoolCacheMax) {
+        g_pool_free.emplace(sz, p);
+        g_pool_cached += sz;
+    } else {
+        (void)hipFree(p);
+    }
 }
 
 // ------------------------------------------------------------------ device
@@ -225,22 +289,22 @@ void job_free(ccsdt_job j)
     if (j->st) (void)hipStreamSynchronize(j->st);
     double *dd[] = {j->d_raw, j->d_ev, j->d_vec6, j->d_pan, j->d_partial, j->d_terms, j->d_f, j->d_out2};
     for (double *p : dd)
-        if (p) (void)hipFree(p);
-    if (j->d_desc) (void)hipFree(j->d_desc);
-    if (j->d_digits) (void)hipFree(j->d_digits);
-    if (j->d_scales) (void)hipFree(j->d_scales);
-    if (j->d_tail) (void)hipFree(j->d_tail);
+        if (p) dev_free(p);
+    if (j->d_desc) dev_free(j->d_desc);
+    if (j->d_digits) dev_free(j->d_digits);
+    if (j->d_scales) dev_free(j->d_scales);
+    if (j->d_tail) dev_free(j->d_tail);
     if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
     if (j->graph) (void)hipGraphDestroy(j->graph);
-    if (j->d_jobdev) (void)hipFree(j->d_jobdev);
-    if (j->d_seg) (void)hipFree(j->d_seg);
-    if (j->d_hdr) (void)hipFree(j->d_hdr);
+    if (j->d_jobdev) dev_free(j->d_jobdev);
+    if (j->d_seg) dev_free(j->d_seg);
+    if (j->d_hdr) dev_free(j->d_hdr);
     if (j->h_jobdev) (void)hipHostFree(j->h_jobdev);
     if (j->h_seg) (void)hipHostFree(j->h_seg);
     if (j->h_hdr) (void)hipHostFree(j->h_hdr);
     if (j->h_terms) (void)hipHostFree(j->h_terms);
-    if (j->d_pairs) (void)hipFree(j->d_pairs);
-    if (j->d_tasks) (void)hipFree(j->d_tasks);
+    if (j->d_pairs) dev_free(j->d_pairs);
+    if (j->d_tasks) dev_free(j->d_tasks);
     if (j->h_tasks) (void)hipHostFree(j->h_tasks);
     if (j->h_stage) (void)hipHostFree(j->h_stage);
     if (j->h_vec) (void)hipHostFree(j->h_vec);
@@ -255,7 +319,7 @@ void job_free(ccsdt_job j)
 int grow(double **p, int64_t *cap, int64_t need)
 {
     if (need <= *cap) return CCSDT_OK;
-    if (*p) (void)hipFree(*p);
+    if (*p) dev_free(*p);
     *p = nullptr;
     *cap = 0;
     int rc = dev_alloc((void **)p, (size_t)need * sizeof(double));
@@ -298,12 +362,20 @@ int job_upload_vectors(ccsdt_job j, const double *eorb, const double *const *vec
 // derived vectors.  `slice_off[q]` (elements) selects where array q is read.
 int job_upload(ccsdt_job j, const double *const *arrays, const int64_t *slice_off)
 {
-    for (int q = 0; q < CCSDT_NARRAYS; ++q)
-        memcpy(j->h_stage + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
-               (size_t)j->cnt[q] * sizeof(double));
     INJECT_POINT("upload");
-    HIP_TRY(hipMemcpyAsync(j->d_raw, j->h_stage, (size_t)j->nraw * sizeof(double),
-                           hipMemcpyHostToDevice, j->st));
+    if (j->h_stage) {
+        // small, repeated uploads (the drop-in context): one copy from pinned memory
+        for (int q = 0; q < CCSDT_NARRAYS; ++q)
+            memcpy(j->h_stage + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
+                   (size_t)j->cnt[q] * sizeof(double));
+        HIP_TRY(hipMemcpyAsync(j->d_raw, j->h_stage, (size_t)j->nraw * sizeof(double),
+                               hipMemcpyHostToDevice, j->st));
+    } else {
+        // one-time job uploads: straight from the caller's memory
+        for (int q = 0; q < CCSDT_NARRAYS; ++q)
+            HIP_TRY(hipMemcpyAsync(j->d_raw + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
+                                   (size_t)j->cnt[q] * sizeof(double), hipMemcpyHostToDevice, j->st));
+    }
     // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2: the order the kernels read
     const double *v6[6] = {arrays[CCSDT_T1V1], arrays[CCSDT_T1V2], arrays[CCSDT_DINTX1],
                            arrays[CCSDT_DINTX2], arrays[CCSDT_DINTC1], arrays[CCSDT_DINTC2]};
@@ -314,7 +386,7 @@ int job_upload(ccsdt_job j, const double *const *arrays, const int64_t *slice_of
 int ensure_tasks(ccsdt_job j, int64_t n)
 {
     if (n <= j->cap_tasks) return CCSDT_OK;
-    if (j->d_tasks) (void)hipFree(j->d_tasks);
+    if (j->d_tasks) dev_free(j->d_tasks);
     if (j->h_tasks) (void)hipHostFree(j->h_tasks);
     j->d_tasks = nullptr;
     j->h_tasks = nullptr;
@@ -326,7 +398,7 @@ int ensure_tasks(ccsdt_job j, int64_t n)
     return CCSDT_OK;
 }
 
-int job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
+int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged)
 {
     *out = nullptr;
     int rc = check_desc(desc);
@@ -363,7 +435,7 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
     JOB_HIP(hipStreamCreate(&j->st));
     for (hipEvent_t &e : j->ev) JOB_HIP(hipEventCreate(&e));
     JOB_TRY(dev_alloc((void **)&j->d_raw, (size_t)j->nraw * sizeof(double)));
-    JOB_HIP(hipHostMalloc((void **)&j->h_stage, (size_t)j->nraw * sizeof(double), 0));
+    if (staged) JOB_HIP(hipHostMalloc((void **)&j->h_stage, (size_t)j->nraw * sizeof(double), 0));
     JOB_HIP(hipHostMalloc((void **)&j->h_vec, (size_t)7 * g.nvp * sizeof(double), 0));
     JOB_HIP(hipHostMalloc((void **)&j->h_out, 64 * sizeof(double), 0));
     JOB_TRY(dev_alloc((void **)&j->d_ev, (size_t)g.nvp * sizeof(double)));
@@ -455,9 +527,9 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
 int ensure_split(ccsdt_job j, const SplitGeom &sg)
 {
     if (j->split_slices == sg.slices) return CCSDT_OK;
-    if (j->d_digits) (void)hipFree(j->d_digits);
-    if (j->d_scales) (void)hipFree(j->d_scales);
-    if (j->d_tail) (void)hipFree(j->d_tail);
+    if (j->d_digits) dev_free(j->d_digits);
+    if (j->d_scales) dev_free(j->d_scales);
+    if (j->d_tail) dev_free(j->d_tail);
     j->d_digits = nullptr;
     j->d_scales = nullptr;
     j->d_tail = nullptr;
@@ -739,7 +811,7 @@ int drop_context(int ncor, int nocc, int nvir, const double *const *arrays, cons
         // the first upload happens inside job_create from offset 0: point the
         // descriptor at the slices directly
         for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q] + (offs ? offs[q] : 0);
-        return job_create(&d, &g_drop);
+        return job_create(&d, &g_drop, true);
     }
     return job_upload(g_drop, arrays, offs);
 }
@@ -975,7 +1047,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
 {
     ccsdt_job_desc d = sj->desc;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q];
-    int rc = job_create(&d, &sj->job);
+    int rc = job_create(&d, &sj->job, false);
     if (rc) return rc;
     ccsdt_job j = sj->job;
     const Geom &g = j->g;
@@ -1219,7 +1291,7 @@ void ccsdt_pcg_uniform(uint64_t seed, uint64_t stream, double lo, double hi, dou
 int ccsdt_job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
 {
     if (!out) return set_error(CCSDT_E_ARG, "null output handle");
-    int rc = job_create(desc, out);
+    int rc = job_create(desc, out, false);
     if (!rc) t_err = CCSDT_OK;
     return rc;
 }
@@ -1414,13 +1486,13 @@ int ccsdt_server_destroy(ccsdt_server s)
     }
     for (int q = 0; q < 2; ++q) {
         TickBuf &T = s->tick[q];
-        if (T.d_seg) (void)hipFree(T.d_seg);
-        if (T.d_hdr) (void)hipFree(T.d_hdr);
+        if (T.d_seg) dev_free(T.d_seg);
+        if (T.d_hdr) dev_free(T.d_hdr);
         if (T.h_seg) (void)hipHostFree(T.h_seg);
         if (T.h_hdr) (void)hipHostFree(T.h_hdr);
         if (T.done) (void)hipEventDestroy(T.done);
     }
-    if (s->d_jobs) (void)hipFree(s->d_jobs);
+    if (s->d_jobs) dev_free(s->d_jobs);
     if (s->h_jobs) (void)hipHostFree(s->h_jobs);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
@@ -1437,7 +1509,7 @@ int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256)
     HIP_TRY(hipMemcpy(d + 64, b64, 64 * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(launch_mfma_probe(d, d + 64, d + 128, nullptr));
     HIP_TRY(hipMemcpy(d256, d + 128, 256 * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipFree(d));
+    dev_free(d);
     return CCSDT_OK;
 }
 
@@ -1452,7 +1524,7 @@ int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d,
     HIP_TRY(hipMemcpy(buf + 256, b256, 256 * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(launch_mfma_i8_probe(buf, buf + 256, buf + 512, shape, nullptr));
     HIP_TRY(hipMemcpy(d, buf + 512, (shape ? 1024 : 256) * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_TRY(hipFree(buf));
+    dev_free(buf);
     return CCSDT_OK;
 }
 
@@ -1475,8 +1547,8 @@ double ccsdt_debug_copy(int64_t bytes, int reps)
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(src);
-    (void)hipFree(dst);
+    dev_free(src);
+    dev_free(dst);
     return reps > 0 ? ms / reps : 0.0;
 }
 
