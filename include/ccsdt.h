@@ -186,11 +186,19 @@ enum {
                                   hipGraph once per signature and replay it */
 };
 
+/* Tile variants of the FP64 fused kernel (the split form has one). */
+enum {
+    CCSDT_TILE_ONE_WAVE = 0,  /* one wave computes all 16 products of a 16x16 block pair */
+    CCSDT_TILE_TWO_WAVE = 1   /* two waves share the pair, eight products each */
+};
+
 typedef struct {
-    int mode;     /* CCSDT_MODE_* */
-    int slices;   /* Ozaki slice count (3..7), ignored otherwise; 0 = 4 */
-    int flags;    /* CCSDT_RUN_* */
-    int reserved;
+    int mode;       /* CCSDT_MODE_* */
+    int slices;     /* Ozaki slice count (3..7), ignored otherwise; 0 = 4 */
+    int flags;      /* CCSDT_RUN_* */
+    int variant;    /* CCSDT_TILE_* (FP64 modes) */
+    int wg_per_cu;  /* persistent grid: workgroups per CU (0 = as many as fit) */
+    int reserved[3];
 } ccsdt_run_opts;
 
 typedef struct {
@@ -213,6 +221,26 @@ int ccsdt_job_run(ccsdt_job job, const int *triples, int64_t n,
                   const ccsdt_run_opts *opts, ccsdt_run_result *res,
                   double *terms, double *fout);
 int ccsdt_job_destroy(ccsdt_job job);
+
+/* Autotuner.  No counterpart in the reference (one DGEMM call per product,
+ * omp/ccsd_trpdrv_omp_cbody.c:71-116, with MKL's own blocking).  Times every
+ * launch configuration of the job's triple list -- tile variant x schedule
+ * (one workgroup per item, or persistent grids of 1..max workgroups per CU) --
+ * with base's mode, slices and graph flag, and writes the fastest to *best.
+ * Each candidate runs once untimed, then `reps` times; its time is the median
+ * of ms_total.  table (may be NULL) receives up to cap entries, *count the
+ * number of candidates tried.  Candidates the device rejects are recorded with
+ * their status and skipped. */
+typedef struct {
+    ccsdt_run_opts opts;
+    double ms;      /* median ms_total, or -1 if the candidate failed */
+    int status;
+    int pad;
+} ccsdt_tune_entry;
+
+int ccsdt_job_autotune(ccsdt_job job, const int *triples, int64_t n, const ccsdt_run_opts *base,
+                       int reps, ccsdt_run_opts *best, ccsdt_tune_entry *table, int cap,
+                       int *count);
 /* Device bytes the job holds (arrays + packed operands + workspace for n triples). */
 int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples);
 
@@ -231,7 +259,7 @@ typedef struct {
     int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = 65536 */
     int mode;                /* CCSDT_MODE_FP64 or CCSDT_MODE_OZAKI_INT8 */
     int slices;              /* Ozaki slices (0 = 4) */
-    int reserved;
+    int variant;             /* CCSDT_TILE_* of the FP64 kernel */
 } ccsdt_server_opts;
 
 typedef struct {

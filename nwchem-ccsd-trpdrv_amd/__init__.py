@@ -31,6 +31,8 @@ ILLCOND_STREAM_BASE = 100   # exponent draws of the ill-conditioned variant
 JK_NAMES = ["jia", "jka", "kia", "kka", "jij", "jkj", "kij", "kkj"]
 
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3}
+MODE_NAMES = {v: k for k, v in MODES.items()}
+TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
 RUN_PERSISTENT, RUN_GRAPH = 1, 2
 E_CODES = {0: "OK", -1: "E_ARG", -2: "E_DEVICE", -3: "E_NOMEM", -4: "E_BUDGET",
            -5: "E_STATE", -6: "E_INJECTED"}
@@ -49,7 +51,19 @@ class JobDesc(ctypes.Structure):
 
 class RunOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slices", ctypes.c_int), ("flags", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("variant", ctypes.c_int), ("wg_per_cu", ctypes.c_int),
+                ("reserved", ctypes.c_int * 3)]
+
+    def as_dict(self):
+        return {"mode": MODE_NAMES[self.mode], "slices": self.slices,
+                "persistent": bool(self.flags & RUN_PERSISTENT),
+                "graph": bool(self.flags & RUN_GRAPH), "variant": self.variant,
+                "wg_per_cu": self.wg_per_cu}
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("opts", RunOpts), ("ms", ctypes.c_double), ("status", ctypes.c_int),
+                ("pad", ctypes.c_int)]
 
 
 class RunResult(ctypes.Structure):
@@ -65,7 +79,7 @@ class RunResult(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_int64), ("items_per_tick", ctypes.c_int),
-                ("mode", ctypes.c_int), ("slices", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("slices", ctypes.c_int), ("variant", ctypes.c_int)]
 
 
 class JobResult(ctypes.Structure):
@@ -107,6 +121,9 @@ def lib():
             "ccsdt_job_run": (ctypes.c_int, [vp, vp, i64, ctypes.POINTER(RunOpts),
                                              ctypes.POINTER(RunResult), vp, vp]),
             "ccsdt_job_destroy": (ctypes.c_int, [vp]),
+            "ccsdt_job_autotune": (ctypes.c_int, [vp, vp, i64, ctypes.POINTER(RunOpts), ctypes.c_int,
+                                                  ctypes.POINTER(RunOpts), ctypes.POINTER(TuneEntry),
+                                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
             "ccsdt_job_footprint": (i64, [ctypes.c_int] * 4 + [i64]),
             "ccsdt_selftest_mfma": (ctypes.c_int, [vp, vp, vp]),
             "ccsdt_selftest_mfma_i8": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
@@ -306,13 +323,14 @@ class Job:
         self.handle = h
 
     def run(self, triples, mode="fp64", slices=0, terms=False, fout=False, persistent=False,
-            graph=False):
+            graph=False, variant=0, wg_per_cu=0):
         """Runs the triples; persistent / graph select CCSDT_RUN_PERSISTENT and
-        CCSDT_RUN_GRAPH."""
+        CCSDT_RUN_GRAPH, variant the FP64 tile (CCSDT_TILE_*), wg_per_cu the
+        persistent grid (0 = as many workgroups per CU as fit)."""
         trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
         n = trip.shape[0]
         flags = (RUN_PERSISTENT if persistent else 0) | (RUN_GRAPH if graph else 0)
-        opts = RunOpts(MODES[mode], slices, flags, 0)
+        opts = RunOpts(MODES[mode], slices, flags, variant, wg_per_cu)
         res = RunResult()
         t = np.empty((n, 4)) if terms else None
         f = np.empty((8, self.nvir, self.nvir)) if fout else None
@@ -326,6 +344,30 @@ class Job:
             # f[m] is column major (nvir x nvir): f[m][c, b] holds element (b, c)
             out["f"] = np.ascontiguousarray(np.swapaxes(f, 1, 2))
         return out
+
+    def autotune(self, triples, mode="fp64", slices=0, graph=True, reps=3):
+        """Times every launch configuration (tile variant x schedule) on these
+        triples (ccsdt_job_autotune).  Returns (best, table): best is a dict of
+        run() keyword arguments, table one dict per candidate with its median
+        ms (-1 when it failed) and status."""
+        trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
+        base = RunOpts(MODES[mode], slices, RUN_GRAPH if graph else 0, 0, 0)
+        best = RunOpts()
+        cap = 16
+        table = (TuneEntry * cap)()
+        count = ctypes.c_int()
+        _check(lib().ccsdt_job_autotune(self.handle, _ptr(trip), trip.shape[0], ctypes.byref(base),
+                                        int(reps), ctypes.byref(best), table, cap,
+                                        ctypes.byref(count)))
+        rows = []
+        for q in range(min(count.value, cap)):
+            d = table[q].opts.as_dict()
+            d.update(ms=table[q].ms, status=table[q].status)
+            rows.append(d)
+        b = best.as_dict()
+        del b["mode"]
+        b["mode"] = mode
+        return b, rows
 
     def close(self):
         if self.handle:
@@ -348,8 +390,9 @@ class Job:
 class Server:
     """Continuous-batching server over the device (see ccsdt_server_* in ccsdt.h)."""
 
-    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0):
-        o = ServerOpts(int(budget_bytes), int(items_per_tick), MODES[mode], int(slices), 0)
+    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0, variant=0):
+        o = ServerOpts(int(budget_bytes), int(items_per_tick), MODES[mode], int(slices),
+                       int(variant))
         h = ctypes.c_void_p()
         _check(lib().ccsdt_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.handle = h

@@ -235,6 +235,9 @@ constexpr int64_t kFWorkspaceBytes = int64_t(4) << 30;
 }  // namespace
 
 // ------------------------------------------------------------------ job
+// entries of a captured graph's signature (job_run)
+constexpr int kGraphKey = 15;
+
 struct ccsdt_job_s {
     Geom g{};
     int kchunk = 0;
@@ -272,7 +275,7 @@ struct ccsdt_job_s {
     // captured run (hipGraph) and the signature it was captured for
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int64_t gkey[13] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    int64_t gkey[kGraphKey] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // Ozaki INT8 split form (allocated on the first split run)
     int split_slices = 0;
     int8_t *d_digits = nullptr;
@@ -542,11 +545,14 @@ int ensure_split(ccsdt_job j, const SplitGeom &sg)
     return CCSDT_OK;
 }
 
-// Persistent grid of a fused kernel: every workgroup slot of the device, in a
-// multiple of 8 (one share per XCD).
-int persistent_grid(bool split, int slices)
+// Persistent grid of a fused kernel: wg_per_cu workgroups per CU (0 or more
+// than fit: every workgroup slot of the device), in a multiple of 8 (one share
+// per XCD).
+int persistent_grid(bool split, int slices, int variant, int wg_per_cu)
 {
-    int g = g_num_cu * fused_blocks_per_cu(split, slices);
+    const int fit = fused_blocks_per_cu(split, slices, variant);
+    const int per = (wg_per_cu > 0 && wg_per_cu < fit) ? wg_per_cu : fit;
+    int g = g_num_cu * per;
     g = (g / 8) * 8;
     return g < 8 ? 8 : g;
 }
@@ -574,6 +580,7 @@ void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f)
     sgm.ntasks = (int)n;
     sgm.pad = 0;
     WorkHdr &h = *j->h_hdr;
+    memset(&h, 0, sizeof(h));  // also zeroes the persistent item counters
     h.total = (int)(n * j->g.npairs);
     h.nsegs = 1;
     h.ntasks = (int)n;
@@ -583,7 +590,8 @@ void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f)
 // The stream work of one fused run, from the task upload to the result
 // download (also what a captured graph holds).
 int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool persistent,
-                  bool with_f, double *terms_dst, bool timed, int64_t *launches)
+                  int variant, int wg_per_cu, bool with_f, double *terms_dst, bool timed,
+                  int64_t *launches)
 {
     const Geom &g = j->g;
     hipEvent_t *E = j->ev;
@@ -599,7 +607,7 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
     if (timed) HIP_TRY(hipEventRecord(E[1], j->st));
     const int64_t items = n * g.npairs;
-    const int grid = persistent ? persistent_grid(split, sg.slices) : (int)items;
+    const int grid = persistent ? persistent_grid(split, sg.slices, variant, wg_per_cu) : (int)items;
     const int64_t vv = (int64_t)g.nvir * g.nvir;
     const bool persist = persistent && !with_f;
     if (split)
@@ -607,7 +615,7 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
                                    sg.slices, 8 * vv, true, with_f, j->st));
     else
         HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
-                             8 * vv, true, with_f, j->st));
+                             variant, 8 * vv, true, with_f, j->st));
     if (timed) HIP_TRY(hipEventRecord(E[2], j->st));
     const int rgrid = persistent ? (int)std::min<int64_t>(n, 8 * (int64_t)g_num_cu) : (int)n;
     HIP_TRY(launch_reduce_segments(j->d_jobdev, j->d_seg, j->d_hdr, rgrid, j->st));
@@ -638,6 +646,12 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const int slices = (mode == CCSDT_MODE_OZAKI_INT8) ? (opts && opts->slices ? opts->slices : 4) : 0;
     if (mode == CCSDT_MODE_OZAKI_INT8 && (slices < 3 || slices > 7))
         return set_error(CCSDT_E_ARG, "Ozaki slice count %d outside 3..7", slices);
+    const int variant = (mode == CCSDT_MODE_OZAKI_INT8 || !opts) ? 0 : opts->variant;
+    if (variant != CCSDT_TILE_ONE_WAVE && variant != CCSDT_TILE_TWO_WAVE)
+        return set_error(CCSDT_E_ARG, "unknown tile variant %d", variant);
+    const int wg_per_cu = opts ? opts->wg_per_cu : 0;
+    if (wg_per_cu < 0 || wg_per_cu > 64)
+        return set_error(CCSDT_E_ARG, "workgroups per CU %d outside 0..64", wg_per_cu);
     int rc = ensure_device(-1);
     if (rc) return rc;
     const Geom &g = j->g;
@@ -689,14 +703,15 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             }
             // the signature, and every buffer the captured nodes reference (a run
             // of another size may have reallocated them)
-            const int64_t key[13] = {mode, slices, n, persistent, terms != nullptr,
+            const int64_t key[kGraphKey] = {mode, slices, n, persistent, terms != nullptr, variant,
+                                     persistent ? wg_per_cu : 0,
                                      (int64_t)(intptr_t)j->d_scales,
                                      (int64_t)(intptr_t)j->d_tasks, (int64_t)(intptr_t)j->h_tasks,
                                      (int64_t)(intptr_t)j->d_partial, (int64_t)(intptr_t)j->d_terms,
                                      (int64_t)(intptr_t)j->h_terms, (int64_t)(intptr_t)j->d_digits,
                                      (int64_t)(intptr_t)j->d_tail};
             bool same = j->gexec != nullptr;
-            for (int q = 0; q < 13 && same; ++q) same = (key[q] == j->gkey[q]);
+            for (int q = 0; q < kGraphKey && same; ++q) same = (key[q] == j->gkey[q]);
             if (!same) {
                 if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
                 if (j->graph) (void)hipGraphDestroy(j->graph);
@@ -704,8 +719,8 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
                 j->graph = nullptr;
                 HIP_TRY(hipStreamBeginCapture(j->st, hipStreamCaptureModeThreadLocal));
                 int64_t l = 0;
-                rc = enqueue_fused(j, mode, sg, n, persistent, false, terms ? j->h_terms : nullptr,
-                                   false, &l);
+                rc = enqueue_fused(j, mode, sg, n, persistent, variant, wg_per_cu, false,
+                                   terms ? j->h_terms : nullptr, false, &l);
                 hipGraph_t gr = nullptr;
                 hipError_t ec = hipStreamEndCapture(j->st, &gr);
                 if (rc) {
@@ -716,14 +731,15 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
                     return set_error(CCSDT_E_DEVICE, "graph capture failed: %s", hipGetErrorString(ec));
                 j->graph = gr;
                 HIP_TRY(hipGraphInstantiate(&j->gexec, gr, nullptr, nullptr, 0));
-                for (int q = 0; q < 13; ++q) j->gkey[q] = key[q];
+                for (int q = 0; q < kGraphKey; ++q) j->gkey[q] = key[q];
             }
             HIP_TRY(hipEventRecord(E[0], j->st));
             HIP_TRY(hipGraphLaunch(j->gexec, j->st));
             HIP_TRY(hipEventRecord(E[3], j->st));
             launches = 1;
         } else {
-            if ((rc = enqueue_fused(j, mode, sg, n, persistent, fouts != nullptr, terms, true, &launches)))
+            if ((rc = enqueue_fused(j, mode, sg, n, persistent, variant, wg_per_cu, fouts != nullptr,
+                                    terms, true, &launches)))
                 return rc;
         }
     } else {
@@ -743,8 +759,8 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             HIP_TRY(hipMemcpyAsync(j->d_seg, j->h_seg, sizeof(Segment), hipMemcpyHostToDevice, j->st));
             HIP_TRY(hipMemcpyAsync(j->d_hdr, j->h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, j->st));
             HIP_TRY(hipEventRecord(E[4], j->st));
-            HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, m * g.npairs, false, 8 * vv,
-                                 false, true, j->st));
+            HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, m * g.npairs, false, variant,
+                                 8 * vv, false, true, j->st));
             HIP_TRY(hipEventRecord(E[5], j->st));
             HIP_TRY(launch_tengy(j->d_f, 8 * vv, g, j->d_tasks + t0, m, j->d_ev, j->d_vec6,
                                  j->d_partial + t0 * tnp * 4, j->st));
@@ -789,6 +805,73 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         res->ntriples = n;
         res->launches = launches;
     }
+    return CCSDT_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times every launch configuration of one triple list (tile variant x schedule)
+// and keeps the fastest.  The schedules are one workgroup per work item, and
+// persistent grids at the occupancy limit and at half of it (fewer resident
+// workgroups trade latency hiding for L2 reuse).
+int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *base, int reps,
+                 ccsdt_run_opts *best, ccsdt_tune_entry *table, int cap, int *count)
+{
+    if (!j) return set_error(CCSDT_E_STATE, "null job");
+    if (!best) return set_error(CCSDT_E_ARG, "null output options");
+    if (reps < 1 || reps > 100) return set_error(CCSDT_E_ARG, "reps %d outside 1..100", reps);
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    ccsdt_run_opts b0 = base ? *base : ccsdt_run_opts{};
+    const bool split = (b0.mode == CCSDT_MODE_OZAKI_INT8);
+    const bool fused = (b0.mode != CCSDT_MODE_FP64_SPLIT);
+    const int slices = split ? (b0.slices ? b0.slices : 4) : 0;
+    std::vector<ccsdt_run_opts> cand;
+    for (int v = 0; v < (split ? 1 : 2); ++v) {
+        ccsdt_run_opts o = b0;
+        o.variant = v;
+        o.flags = b0.flags & ~CCSDT_RUN_PERSISTENT;
+        o.wg_per_cu = 0;
+        cand.push_back(o);
+        if (!fused) continue;  // the f-writing kernel has no persistent form
+        const int fit = fused_blocks_per_cu(split, slices, v);
+        for (int w : {fit, fit / 2}) {
+            if (w < 1 || (w == fit / 2 && w == fit)) continue;
+            o.flags = b0.flags | CCSDT_RUN_PERSISTENT;
+            o.wg_per_cu = w;
+            cand.push_back(o);
+        }
+    }
+    int nbest = -1;
+    double tbest = 0.0;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        std::vector<double> ms;
+        ccsdt_run_result r{};
+        rc = job_run(j, trip, n, &cand[c], &r, nullptr, nullptr);
+        for (int q = 0; q < reps && rc == CCSDT_OK; ++q) {
+            rc = job_run(j, trip, n, &cand[c], &r, nullptr, nullptr);
+            ms.push_back(r.ms_total);
+        }
+        double med = -1.0;
+        if (rc == CCSDT_OK) {
+            std::sort(ms.begin(), ms.end());
+            med = ms[ms.size() / 2];
+            if (nbest < 0 || med < tbest) {
+                nbest = (int)c;
+                tbest = med;
+            }
+        } else if (rc == CCSDT_E_ARG || rc == CCSDT_E_STATE) {
+            return rc;  // the list or the options are wrong, not the candidate
+        }
+        if (table && (int)c < cap) {
+            table[c].opts = cand[c];
+            table[c].ms = med;
+            table[c].status = rc;
+            table[c].pad = 0;
+        }
+    }
+    if (count) *count = (int)cand.size();
+    if (nbest < 0) return rc ? rc : set_error(CCSDT_E_DEVICE, "no launch configuration ran");
+    *best = cand[nbest];
     return CCSDT_OK;
 }
 
@@ -1156,6 +1239,7 @@ int server_launch_tick(ccsdt_server s, bool *launched)
     }
     s->rr = (s->rr + 1) % nr;
     if (nseg == 0) return CCSDT_OK;
+    memset(T.h_hdr, 0, sizeof(WorkHdr));
     T.h_hdr->total = (int)items;
     T.h_hdr->nsegs = nseg;
     T.h_hdr->ntasks = (int)tasks;
@@ -1166,7 +1250,8 @@ int server_launch_tick(ccsdt_server s, bool *launched)
         HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.slices, 0,
                                    true, false, s->st));
     else
-        HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, 0, true, false, s->st));
+        HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.variant, 0, true,
+                             false, s->st));
     HIP_TRY(launch_reduce_segments(s->d_jobs, T.d_seg, T.d_hdr, (int)tasks, s->st));
     HIP_TRY(hipEventRecord(T.done, s->st));
     T.inflight = true;
@@ -1309,6 +1394,14 @@ int ccsdt_job_run(ccsdt_job job, const int *triples, int64_t n, const ccsdt_run_
     return rc;
 }
 
+int ccsdt_job_autotune(ccsdt_job job, const int *triples, int64_t n, const ccsdt_run_opts *base,
+                       int reps, ccsdt_run_opts *best, ccsdt_tune_entry *table, int cap, int *count)
+{
+    int rc = job_autotune(job, triples, n, base, reps, best, table, cap, count);
+    if (!rc) t_err = CCSDT_OK;
+    return rc;
+}
+
 int ccsdt_job_destroy(ccsdt_job job)
 {
     if (!job) return set_error(CCSDT_E_STATE, "null job");
@@ -1342,7 +1435,10 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
     if (o.mode == CCSDT_MODE_OZAKI_INT8) {
         if (o.slices == 0) o.slices = 4;
         if (o.slices < 3 || o.slices > 7) return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
+        o.variant = 0;
     }
+    if (o.variant != CCSDT_TILE_ONE_WAVE && o.variant != CCSDT_TILE_TWO_WAVE)
+        return set_error(CCSDT_E_ARG, "unknown tile variant %d", o.variant);
     int rc = ensure_device(-1);
     if (rc) return rc;
     ccsdt_server s = new ccsdt_server_s();

@@ -90,14 +90,21 @@ struct WorkHdr {
     int nsegs;
     int ntasks;   // tasks over all segments
     int pad;
+    // Persistent launches: items handed out so far from each of the 8 item
+    // ranges (one per XCD).  The host writes zeros with every upload of the
+    // header, so each launch (and each graph replay) starts from zero.
+    int next[8];
 };
 
 // Launchers (ccsdt_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st);
 // grid = number of workgroups: the item count for a plain launch, or a
 // multiple of 8 for a persistent one (each workgroup then loops over items).
-hipError_t launch_fused(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr, int grid,
-                        bool persist, int64_t fstride, bool tengy, bool writef, hipStream_t st);
+// variant 0: one wave computes all 16 products of a block pair (k_fused);
+// variant 1: two waves share the pair, eight products each (k_fused2).
+hipError_t launch_fused(const JobDev *jobs, const Segment *segs, WorkHdr *hdr, int grid,
+                        bool persist, int variant, int64_t fstride, bool tengy, bool writef,
+                        hipStream_t st);
 hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
                         int ntasks, const double *ev, const double *vec6, double *partial,
                         hipStream_t st);
@@ -105,14 +112,14 @@ int tengy_npairs(const Geom &g);
 hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
                              const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
                              hipStream_t st);
-hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
                               int grid, bool persist, int slices, int64_t fstride, bool tengy,
                               bool writef, hipStream_t st);
 // Per-task sums of the pair partials over all segments (grid may be persistent).
 hipError_t launch_reduce_segments(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
                                   int grid, hipStream_t st);
 // Resident workgroups per CU of the fused kernels (for persistent grids).
-int fused_blocks_per_cu(bool split, int slices);
+int fused_blocks_per_cu(bool split, int slices, int variant);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,
                                hipStream_t st);
 hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks, double *out2,

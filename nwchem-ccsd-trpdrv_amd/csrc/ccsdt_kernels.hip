@@ -92,26 +92,53 @@ __device__ __forceinline__ int xcd_remap(int bid, int total)
 
 // Work schedule shared by the plain and the persistent launches: the items are
 // split into 8 contiguous ranges, one per XCD (the hardware deals workgroup b to
-// XCD b % 8), and the workgroups of an XCD stride through its range.  With one
-// workgroup per item this is the same mapping as xcd_remap; with a persistent
-// grid (a multiple of 8 workgroups) each workgroup loops.
+// XCD b % 8).  A plain launch (one workgroup per item) maps workgroup b to item
+// start(b%8) + b/8, the same mapping as xcd_remap, so every XCD works through
+// its range in dispatch order.  A persistent grid (a multiple of 8 workgroups)
+// keeps that order with a counter per range: each workgroup takes the next
+// item of range b%8 with an atomic add until the range is used up.  (A static
+// stride let the workgroups drift apart over thousands of items, widening the
+// set of tasks in flight past what one XCD's L2 holds.)
 struct ItemRange {
-    int first, end, stride;
+    int start, len;  // the range of XCD b % 8
 };
 
-__device__ __forceinline__ ItemRange item_range(int total, int grid, int b)
+__device__ __forceinline__ ItemRange item_range(int total, int b)
 {
-    const int x = b & 7, j = b >> 3;
+    const int x = b & 7;
     const int q = total >> 3, r = total & 7;
-    const int start = x * q + min(x, r);
-    ItemRange ir;
-    ir.first = start + j;
-    ir.end = start + q + (x < r ? 1 : 0);
-    ir.stride = (grid >> 3) + (x < (grid & 7) ? 1 : 0);
-    return ir;
+    return ItemRange{x * q + min(x, r), q + (x < r ? 1 : 0)};
 }
 
+// Next item of this workgroup's range for a persistent launch, or -1 when the
+// range is used up (wave-uniform; one atomic per workgroup, lane 0 of wave 0).
+// NW = waves per workgroup; with two waves the value passes through LDS, and
+// the caller's item work must end with a barrier before the next call.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Item of workgroup b in a plain launch (grid = item count), or -1.
+__device__ __forceinline__ int plain_item(const ItemRange &ir)
+{
+    const int j = blockIdx.x >> 3;
+    return uniform(j < ir.len ? ir.start + j : -1);
+}
+
+template <int NW>
+__device__ __forceinline__ int next_item(WorkHdr *hdr, const ItemRange &ir)
+{
+    int c = 0;
+    if (threadIdx.x == 0) c = atomicAdd(&hdr->next[blockIdx.x & 7], 1);
+    if (NW == 1) {
+        c = __builtin_amdgcn_readlane(c, 0);
+    } else {
+        __shared__ int s_next;
+        if (threadIdx.x == 0) s_next = c;
+        __syncthreads();
+        c = __builtin_amdgcn_readfirstlane(s_next);
+    }
+    return c < ir.len ? ir.start + c : -1;
+}
+
 
 // Segment holding work item `item` (segments sorted by item0; wave-uniform).
 __device__ __forceinline__ int find_segment(const Segment *__restrict__ segs, int nsegs, int item)
@@ -321,12 +348,13 @@ __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, int
 template <bool TENGY, bool WRITEF, bool PERSIST>
 __global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
                                               const Segment *__restrict__ segs,
-                                              const WorkHdr *__restrict__ hdr, int64_t fstride)
+                                              WorkHdr *hdr, int64_t fstride)
 {
-    const WorkHdr h = *hdr;
-    const ItemRange ir = item_range(h.total, gridDim.x, blockIdx.x);
-    for (int item = uniform(ir.first); item < ir.end; item = PERSIST ? item + ir.stride : ir.end) {
-        const int si = uniform(find_segment(segs, h.nsegs, item));
+    const int nsegs = hdr->nsegs;
+    const ItemRange ir = item_range(hdr->total, blockIdx.x);
+    for (int item = PERSIST ? next_item<1>(hdr, ir) : plain_item(ir); item >= 0;
+         item = PERSIST ? next_item<1>(hdr, ir) : -1) {
+        const int si = uniform(find_segment(segs, nsegs, item));
         const Segment sg = segs[si];
         const JobDev jd = jobs[uniform(sg.job)];
         const int L = uniform(item - sg.item0);
@@ -339,10 +367,17 @@ __global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
     }
 }
 
-hipError_t launch_fused(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr, int grid,
-                        bool persist, int64_t fstride, bool tengy, bool writef, hipStream_t st)
+static hipError_t launch_fused2(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
+                                int grid, bool persist, int64_t fstride, bool tengy, bool writef,
+                                hipStream_t st);
+
+hipError_t launch_fused(const JobDev *jobs, const Segment *segs, WorkHdr *hdr, int grid,
+                        bool persist, int variant, int64_t fstride, bool tengy, bool writef,
+                        hipStream_t st)
 {
     if (grid <= 0) return hipErrorInvalidValue;
+    if (variant == 1) return launch_fused2(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+    if (variant != 0) return hipErrorInvalidValue;
     dim3 g3((unsigned)grid), block(64);
     if (persist && tengy && !writef)
         hipLaunchKernelGGL((k_fused<true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
@@ -554,46 +589,89 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
     }
 }
 
-// Work unit of k_fused (one 16x16 block pair of one triple) split over two
-// waves: wave 0 computes the eight products at (b,c) (groups n and t), wave 1 the
-// eight mirrored ones f(c,b) (groups n' and t').  Each group shares two A and two
-// B panels.  After the INT8 groups each wave adds its FP64 tail (K = nocc) on
-// v_mfma_f64_16x16x4_f64 with the A rows fed in the order 4*(i%4) + i/4, which
-// makes its result layout (lane l, element r -> row 4*(l/16)+r, column l%16)
-// equal that of the INT8 MFMA.  The energy epilogue swaps half the values
-// through LDS and each wave finishes two of the four rows per lane.
-template <int S, bool TENGY, bool WRITEF>
-__device__ __forceinline__ void split_item(const JobDev &jd, const Task &tk, int slice, int bi,
-                                           int ci, int w, int lane, double *__restrict__ part4,
-                                           double *__restrict__ fo)
+// FP64 form of the two-wave item (k_fused2): the same eight products per wave
+// role as split_wave, over the full K = nvir + nocc of the FP64 panels.  Each K
+// step loads four A and four B panels and runs eight v_mfma_f64_16x16x4_f64;
+// ping-pong buffers as in fused_mainloop.  The A rows are fed in the order
+// 4*(i%4) + i/4 so the result layout is the one pair_epilogue expects.
+template <int W>
+__device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restrict__ pan,
+                                          const Geom &g, int ps, int bi, int ci, int lane)
+{
+    const uint32_t vsb = (uint32_t)(g.vecsz * 8);
+    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(pan, 4 * vsb);
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pan + (int64_t)ps * g.vecsz, 4 * vsb);
+    const uint32_t blkb = (uint32_t)g.nks * 512;
+    const uint32_t ob = bi * blkb, oc = ci * blkb;
+    const int li = lane & 15;
+    const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
+    const uint32_t voffB = lane * 8;
+    // L panels {LJ, LK, LJt, LKt} and R panels {R1, R3, R1t, R3t}: offsets inside
+    // their resource, and whether that resource is the task's k slice.
+    const uint32_t LO[4] = {PAN_LJ * vsb, PAN_LK * vsb, SL_LJT * vsb, SL_LKT * vsb};
+    const uint32_t RO[4] = {SL_R1 * vsb, SL_R3 * vsb, PAN_R1T * vsb, PAN_R3T * vsb};
+    constexpr bool LS[4] = {false, false, true, true};
+    constexpr bool RS[4] = {true, true, false, false};
+    uint32_t ao[4], bo[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        ao[v] = (W == 0 ? LO[v] : RO[v]) + ob;
+        bo[v] = (W == 0 ? RO[v] : LO[v]) + oc;
+    }
+    const int nks = g.nks;
+    double xa[4], xb[4], ya[4], yb[4];
+#define F2_LOAD(da, db, kstep)                                                                   \
+    _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                             \
+        const bool sa = (W == 0) ? LS[v] : RS[v], sb = (W == 0) ? RS[v] : LS[v];                  \
+        da[v] = buffer_load_f64(sa ? rs : rg, voffA, ao[v] + (uint32_t)(kstep) * 512);            \
+        db[v] = buffer_load_f64(sb ? rs : rg, voffB, bo[v] + (uint32_t)(kstep) * 512);            \
+    }
+#define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
+#define F2_MFMA(da, db)                                                                          \
+    {                                                                                            \
+        const double *a = da, *b = db;                                                           \
+        if (W == 0) {                                                                            \
+            MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);                                  \
+            MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);                                  \
+        } else {                                                                                 \
+            MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);                                  \
+            MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);                                  \
+        }                                                                                        \
+    }
+    F2_LOAD(xa, xb, 0)
+    int ks = 0;
+    for (; ks + 1 < nks; ks += 2) {
+        F2_LOAD(ya, yb, ks + 1)
+        F2_MFMA(xa, xb)
+        const int k2 = (ks + 2 < nks) ? ks + 2 : nks - 1;  // clamped prefetch, in bounds
+        F2_LOAD(xa, xb, k2)
+        F2_MFMA(ya, yb)
+    }
+    if (nks & 1) F2_MFMA(xa, xb)
+#undef F2_LOAD
+#undef F2_MFMA
+#undef MS
+}
+
+// Epilogue of a two-wave block-pair item.  On entry wave 0 holds the eight
+// products at (b,c) and wave 1 the eight mirrored ones, both in the layout
+// lane l, element r -> row 4*(l/16)+r, column l%16.  On a diagonal pair
+// (B = C) wave 1 computed nothing: the mirrored products are the direct ones
+// transposed, read back from one LDS tile.
+template <bool TENGY, bool WRITEF>
+__device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, const dbl4 (&facc)[8],
+                                              int bi, int ci, int w, int lane,
+                                              double *__restrict__ part4, double *__restrict__ fo)
 {
     __shared__ double xch[2][8][2][64];
     __shared__ double red[2][4];
     const Geom &g = jd.g;
-    const SplitGeom &sg = jd.sg;
-    const double *__restrict__ scales = jd.scales;
     const double *__restrict__ ev = jd.ev;
     const double *__restrict__ vec6 = jd.vec6;
     const int nvp = g.nvp;
-    const int ps = PAN_SLICE0 + 4 * slice;
-    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
-                                                  (uint32_t)(sg.npanels * S * sg.slice_bytes));
-    const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
     const int row0 = 4 * (lane >> 4);
     const int cx = ci * 16 + (lane & 15);
-
-    dbl4 facc[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
-    // On a diagonal pair (B = C) the mirrored products are the direct ones
-    // transposed: wave 1 skips its products and the epilogue reads both
-    // orientations from one LDS tile.
     const bool diag = (bi == ci);
-    if (w == 0)
-        split_wave<S, 0>(facc, rd, rt, scales, g, sg, ps, bi, ci, lane);
-    else if (!diag)
-        split_wave<S, 1>(facc, rd, rt, scales, g, sg, ps, bi, ci, lane);
-
     const int nv = g.nvir;
     if (WRITEF) {
         const int64_t vv = (int64_t)nv * nv;
@@ -681,18 +759,51 @@ __device__ __forceinline__ void split_item(const JobDev &jd, const Task &tk, int
     __syncthreads();  // the LDS buffers are reused by the workgroup's next item
 }
 
-// The split form's work item (one 16x16 block pair of one task) on two waves.
-template <int S, bool TENGY, bool WRITEF, bool PERSIST>
-__global__ __launch_bounds__(128) void k_fused_split(const JobDev *__restrict__ jobs,
-                                                     const Segment *__restrict__ segs,
-                                                     const WorkHdr *__restrict__ hdr, int64_t fstride)
+// One two-wave work item: the 16x16 block pair (bi, ci) of one task.  S = 0
+// takes the FP64 panels (k_fused2), S >= 3 the INT8 digit planes of the split
+// form (k_fused_split).
+template <int S, bool TENGY, bool WRITEF>
+__device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, int slice, int bi,
+                                          int ci, int w, int lane, double *__restrict__ part4,
+                                          double *__restrict__ fo)
 {
-    const WorkHdr h = *hdr;
-    const ItemRange ir = item_range(h.total, gridDim.x, blockIdx.x);
+    const int ps = PAN_SLICE0 + 4 * slice;
+    dbl4 facc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const bool diag = (bi == ci);
+    if constexpr (S == 0) {
+        if (w == 0)
+            fp64_wave<0>(facc, jd.pan, jd.g, ps, bi, ci, lane);
+        else if (!diag)
+            fp64_wave<1>(facc, jd.pan, jd.g, ps, bi, ci, lane);
+    } else {
+        const SplitGeom &sg = jd.sg;
+        const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
+                                                      (uint32_t)(sg.npanels * S * sg.slice_bytes));
+        const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
+        if (w == 0)
+            split_wave<S, 0>(facc, rd, rt, jd.scales, jd.g, sg, ps, bi, ci, lane);
+        else if (!diag)
+            split_wave<S, 1>(facc, rd, rt, jd.scales, jd.g, sg, ps, bi, ci, lane);
+    }
+    pair_epilogue<TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo);
+}
+
+// Item loop of the two-wave kernels (128 threads, one block pair per item).
+template <int S, bool TENGY, bool WRITEF, bool PERSIST>
+__device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
+                                           const Segment *__restrict__ segs,
+                                           WorkHdr *hdr, int64_t fstride)
+{
+    const int nsegs = hdr->nsegs;
+    const ItemRange ir = item_range(hdr->total, blockIdx.x);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    for (int item = uniform(ir.first); item < ir.end; item = PERSIST ? item + ir.stride : ir.end) {
-        const int si = uniform(find_segment(segs, h.nsegs, item));
+    // pair_item ends with a workgroup barrier, which next_item<2> relies on
+    for (int item = PERSIST ? next_item<2>(hdr, ir) : plain_item(ir); item >= 0;
+         item = PERSIST ? next_item<2>(hdr, ir) : -1) {
+        const int si = uniform(find_segment(segs, nsegs, item));
         const Segment sg = segs[si];
         const JobDev jd = jobs[uniform(sg.job)];
         const int L = uniform(item - sg.item0);
@@ -700,13 +811,51 @@ __global__ __launch_bounds__(128) void k_fused_split(const JobDev *__restrict__ 
         const int p = uniform(L - t * jd.g.npairs);
         const Task tk = sg.tasks[t];
         const short2 pr = jd.pairs[p];
-        split_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane, sg.partial + (int64_t)L * 4,
+        pair_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane, sg.partial + (int64_t)L * 4,
                                      WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }
 }
 
+// The split form's work item (one 16x16 block pair of one task) on two waves.
+template <int S, bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(128) void k_fused_split(const JobDev *__restrict__ jobs,
+                                                     const Segment *__restrict__ segs,
+                                                     WorkHdr *hdr, int64_t fstride)
+{
+    pair_items<S, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
+}
+
+// The FP64 form's work item on two waves (the alternative tile of k_fused).
+template <bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(128) void k_fused2(const JobDev *__restrict__ jobs,
+                                                const Segment *__restrict__ segs,
+                                                WorkHdr *hdr, int64_t fstride)
+{
+    pair_items<0, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
+}
+
+static hipError_t launch_fused2(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
+                                int grid, bool persist, int64_t fstride, bool tengy, bool writef,
+                                hipStream_t st)
+{
+    dim3 g3((unsigned)grid), block(128);
+    if (persist && tengy && !writef)
+        hipLaunchKernelGGL((k_fused2<true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (persist)
+        return hipErrorInvalidValue;
+    else if (tengy && writef)
+        hipLaunchKernelGGL((k_fused2<true, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (tengy)
+        hipLaunchKernelGGL((k_fused2<true, false, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (writef)
+        hipLaunchKernelGGL((k_fused2<false, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 template <int S>
-static hipError_t launch_split_s(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+static hipError_t launch_split_s(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
                                  int grid, bool persist, int64_t fstride, bool tengy, bool writef,
                                  hipStream_t st)
 {
@@ -726,7 +875,7 @@ static hipError_t launch_split_s(const JobDev *jobs, const Segment *segs, const 
     return hipGetLastError();
 }
 
-hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
+hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
                               int grid, bool persist, int slices, int64_t fstride, bool tengy,
                               bool writef, hipStream_t st)
 {
@@ -742,11 +891,13 @@ hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, const Wor
 }
 
 // Resident workgroups per CU of each fused kernel, from the occupancy API.
-int fused_blocks_per_cu(bool split, int slices)
+int fused_blocks_per_cu(bool split, int slices, int variant)
 {
     int n = 0;
     hipError_t e;
-    if (!split) {
+    if (!split && variant == 1) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused2<true, false, true>, 128, 0);
+    } else if (!split) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<true, false, true>, 64, 0);
     } else {
         switch (slices) {

@@ -65,3 +65,67 @@ def test_uneven_blocks_batched(cc, orc):
     assert np.max(np.abs(f["terms"] - ref["terms"]) / scale) < 1e-11
     E = ref["emp4"] + ref["emp5"]
     assert abs(s["emp4"] + s["emp5"] - E) / abs(E) < 1e-8
+
+
+def _rel_terms(got, ref):
+    scale = np.maximum(np.abs(ref["terms"]), ref["abs4"])
+    return np.max(np.abs(got - ref["terms"]) / scale)
+
+
+@pytest.mark.parametrize("nocc,nvir,kchunk", [(8, 40, 8), (5, 37, 2), (7, 70, 4)])
+def test_two_wave_tile_against_oracle(cc, orc, nocc, nvir, kchunk):
+    """The two-wave FP64 tile (plain, persistent at two grid sizes, graph) meets
+    the per-term bar against the oracle; shapes cover nvir % 16 != 0 and odd
+    K-step counts (nvir + nocc = 42, 42, 77 -> 11, 11, 20 steps of 4)."""
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 301)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    ref = orc.run_list(arrays, 0, nocc, nvir, trip, with_abs=True)
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        one = job.run(trip, terms=True)
+        runs = [job.run(trip, terms=True, variant=1),
+                job.run(trip, terms=True, variant=1, persistent=True),
+                job.run(trip, terms=True, variant=1, persistent=True, wg_per_cu=1),
+                job.run(trip, terms=True, variant=1, persistent=True, graph=True)]
+    for r in runs:
+        assert _rel_terms(r["terms"], ref) < 1e-11
+        assert np.max(np.abs(r["terms"] - one["terms"])) <= 1e-12 * np.max(np.abs(one["terms"]))
+    # the persistent grid size does not change any partial's summation order
+    assert np.array_equal(runs[1]["terms"], runs[2]["terms"])
+    assert np.array_equal(runs[1]["terms"], runs[3]["terms"])
+
+
+def test_two_wave_f_matrices_and_separate_pass(cc, orc):
+    """k_fused2's f-writing forms: single-triple f output against the oracle's
+    16 DGEMMs, and the separate-energy-pass mode on a list."""
+    nocc, nvir, kchunk = 6, 45, 3
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 17)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    ref = orc.run_list(arrays, 0, nocc, nvir, trip, with_abs=True)
+    k, klo = int(trip[5, 3]), int(trip[5, 4])
+    F = orc.fmats(arrays, nocc, nvir, k, klo)
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        r = job.run(trip[5:6], terms=True, fout=True, variant=1)
+        s = job.run(trip, mode="fp64_split", terms=True, variant=1)
+    assert np.max(np.abs(r["f"] - F)) <= 1e-13 * np.max(np.abs(F))
+    assert _rel_terms(s["terms"], ref) < 1e-11
+
+
+@pytest.mark.parametrize("mode,slices", [("fp64", 0), ("ozaki_int8", 4)])
+def test_autotune_picks_a_working_candidate(cc, mode, slices):
+    nocc, nvir, kchunk = 6, 45, 3
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 23)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        best, table = job.autotune(trip, mode=mode, slices=slices, reps=2)
+        ref = job.run(trip, mode=mode, slices=slices, terms=True)
+        got = job.run(trip, terms=True, **best)
+    variants = {row["variant"] for row in table}
+    assert variants == ({0, 1} if mode == "fp64" else {0})
+    assert any(row["persistent"] for row in table) and any(not row["persistent"] for row in table)
+    assert all(row["status"] == 0 and row["ms"] > 0 for row in table)
+    assert min(row["ms"] for row in table) == min(
+        row["ms"] for row in table if row["variant"] == best["variant"]
+        and row["persistent"] == best["persistent"] and row["wg_per_cu"] == best["wg_per_cu"])
+    assert best["graph"]
+    tol = 1e-12 if mode == "fp64" else 1e-9
+    assert abs(got["emp4"] + got["emp5"] - ref["emp4"] - ref["emp5"]) <= tol * abs(ref["emp4"] + ref["emp5"])
