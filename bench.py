@@ -151,6 +151,7 @@ def batched_report(cc, steps, warmup, slices):
                       "nocc": nocc, "nvir": nvir, "kchunk": kchunk, "ntriples": n},
            "forms": {}}
     with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        # slices 0: the autotuner also picks the split form's slice count
         for mode, S in (("fp64", 0), ("ozaki_int8", slices)):
             t0 = time.perf_counter()
             best, table = job.autotune(trip, mode=mode, slices=S, graph=True, reps=2)
@@ -174,7 +175,7 @@ def batched_report(cc, steps, warmup, slices):
     head = sp if dev <= ET_TOL else f
     out["value"] = head["value"]
     out["ms_per_step"] = head["ms_per_step"]
-    out["dtype"] = ("i8x%d+f64" % slices) if dev <= ET_TOL else "f64"
+    out["dtype"] = ("i8x%d+f64" % sp["chosen"]["slices"]) if dev <= ET_TOL else "f64"
     out["frac_fp64_peak_fp64_form"] = f["value"] / FP64_MFMA_PEAK
     return out
 
@@ -193,6 +194,8 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="only run the batched configuration (nocc=40, nvir=400, uneven k "
                          "blocks) with the autotuner")
+    ap.add_argument("--batched-slices", type=int, default=0,
+                    help="slices of the batched split form (0: chosen by the autotuner)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (reference point for the "
                          "FETCH_SIZE/WRITE_SIZE counters)")
@@ -212,7 +215,8 @@ def main():
                                     "nvir log-U[50,400], one GPU", "runs": reps}), flush=True)
         return
     if args.batched:
-        print(json.dumps(batched_report(cc, args.steps, args.warmup, args.slices)), flush=True)
+        print(json.dumps(batched_report(cc, args.steps, args.warmup, args.batched_slices)),
+              flush=True)
         return
     nocc, nvir, S = 30, 300, args.slices
     arrays, blocks, trip, kchunk = cc.config_case(1)

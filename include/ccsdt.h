@@ -230,7 +230,10 @@ int ccsdt_job_destroy(ccsdt_job job);
  * Each candidate runs once untimed, then `reps` times; its time is the median
  * of ms_total.  table (may be NULL) receives up to cap entries, *count the
  * number of candidates tried.  Candidates the device rejects are recorded with
- * their status and skipped. */
+ * their status and skipped.  With mode CCSDT_MODE_OZAKI_INT8 and slices 0 the
+ * slice count is chosen first: the smallest S in 3..7 whose E[T] = emp4 + emp5
+ * over these triples is within 1e-9 relative of the FP64 form's (ten times
+ * under the split form's 1e-8 bound); best->slices reports it. */
 typedef struct {
     ccsdt_run_opts opts;
     double ms;      /* median ms_total, or -1 if the candidate failed */

@@ -809,6 +809,10 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
 }
 
 // ------------------------------------------------------------------ autotuner
+// E[T] agreement with the FP64 form the slice-count choice asks for: ten times
+// under the split form's 1e-8 bound, since the tuning list is a sample.
+constexpr double kSliceTol = 1e-9;
+
 // Times every launch configuration of one triple list (tile variant x schedule)
 // and keeps the fastest.  The schedules are one workgroup per work item, and
 // persistent grids at the occupancy limit and at half of it (fewer resident
@@ -824,8 +828,29 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     ccsdt_run_opts b0 = base ? *base : ccsdt_run_opts{};
     const bool split = (b0.mode == CCSDT_MODE_OZAKI_INT8);
     const bool fused = (b0.mode != CCSDT_MODE_FP64_SPLIT);
-    const int slices = split ? (b0.slices ? b0.slices : 4) : 0;
     std::vector<ccsdt_run_opts> cand;
+    if (split && b0.slices == 0) {
+        // slice count: the smallest S whose E[T] on this list is within
+        // kSliceTol of the FP64 form's
+        ccsdt_run_opts f = b0;
+        f.mode = CCSDT_MODE_FP64;
+        f.flags = 0;
+        f.variant = 0;
+        ccsdt_run_result rf{};
+        if ((rc = job_run(j, trip, n, &f, &rf, nullptr, nullptr))) return rc;
+        const double ef = rf.emp4 + rf.emp5;
+        for (int S = 3; S <= 7 && b0.slices == 0; ++S) {
+            ccsdt_run_opts o = b0;
+            o.slices = S;
+            o.flags = 0;
+            ccsdt_run_result r{};
+            if ((rc = job_run(j, trip, n, &o, &r, nullptr, nullptr))) return rc;
+            const double dev = std::fabs(r.emp4 + r.emp5 - ef) / std::fabs(ef);
+            if (dev <= kSliceTol) b0.slices = S;
+        }
+        if (b0.slices == 0) b0.slices = 7;
+    }
+    const int slices = split ? (b0.slices ? b0.slices : 4) : 0;
     for (int v = 0; v < (split ? 1 : 2); ++v) {
         ccsdt_run_opts o = b0;
         o.variant = v;

@@ -811,7 +811,12 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
         const int p = uniform(L - t * jd.g.npairs);
         const Task tk = sg.tasks[t];
         const short2 pr = jd.pairs[p];
-        pair_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane, sg.partial + (int64_t)L * 4,
+        // an opaque copy of the lane index: keeps the compiler from hoisting
+        // lane-dependent offsets out of the item loop, where they would hold
+        // VGPRs through the whole item
+        int lane_i = lane;
+        if (PERSIST) asm volatile("" : "+v"(lane_i));
+        pair_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane_i, sg.partial + (int64_t)L * 4,
                                      WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }
 }
@@ -919,7 +924,9 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
 // Reads the eight f matrices of each task from HBM (ld = nvir, body order
 // f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t) in 32x32 tile pairs (BI >= CI).  The direct
 // tile is read coalesced along b; the mirrored tile f(c,b) is read coalesced
-// along c into LDS and transposed there.
+// along c and transposed through LDS.  All 64 loads of a thread are issued
+// before the first transpose, so they are in flight together; the transposes
+// alternate between two LDS tiles (one barrier each).
 __host__ __device__ inline int tengy_tiles(int nv) { return (nv + 31) / 32; }
 
 int tengy_npairs(const Geom &g)
@@ -934,7 +941,7 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
                                                const double *__restrict__ vec6,
                                                double *__restrict__ partial)
 {
-    __shared__ double tile[32][33];
+    __shared__ double tile[2][32][33];
     __shared__ double red[4][4];
     const int nv = g.nvir, nvp = g.nvp;
     const int nt = tengy_tiles(nv);
@@ -951,21 +958,48 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const int bx = BI * 32 + tx;
 
+    // byte offsets inside one matrix (0 for padding lanes, whose values are
+    // replaced by 0); the matrix is selected by the uniform offset m*vv*8
+    const __amdgpu_buffer_rsrc_t rf = buffer_rsrc(ft, (uint32_t)(8 * vv * 8));
+    uint32_t od[4], om[4];
+    bool okd[4], okm[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int cx = CI * 32 + ty + 8 * q;
+        okd[q] = bx < nv && cx < nv;
+        od[q] = okd[q] ? (uint32_t)(bx + cx * nv) * 8 : 0;
+        const int cc = CI * 32 + tx, bb = BI * 32 + ty + 8 * q;
+        okm[q] = cc < nv && bb < nv;
+        om[q] = okm[q] ? (uint32_t)(cc + bb * nv) * 8 : 0;
+    }
+    // the mirrored tiles first: their transposes can start while the direct
+    // tiles are still arriving
     double dv[8][4], mv[8][4];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        const double *fm = ft + m * vv;
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mv[m][q] = buffer_load_f64(rf, om[q], (uint32_t)(m * vv * 8));
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dv[m][q] = buffer_load_f64(rf, od[q], (uint32_t)(m * vv * 8));
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int cx = CI * 32 + ty + 8 * q;
-            dv[m][q] = (bx < nv && cx < nv) ? fm[bx + (int64_t)cx * nv] : 0.0;
-            const int cc = CI * 32 + tx, bb = BI * 32 + ty + 8 * q;
-            tile[ty + 8 * q][tx] = (cc < nv && bb < nv) ? fm[cc + (int64_t)bb * nv] : 0.0;
+            dv[m][q] = okd[q] ? dv[m][q] : 0.0;
+            mv[m][q] = okm[q] ? mv[m][q] : 0.0;
         }
+    // transpose: thread (tx, ty) loaded f(c = CI*32+tx, b = BI*32+ty+8q) and
+    // needs f(c = CI*32+ty+8q, b = BI*32+tx)
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        double(*T)[33] = tile[m & 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) T[ty + 8 * q][tx] = mv[m][q];
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q) mv[m][q] = tile[tx][ty + 8 * q];
-        __syncthreads();
+        for (int q = 0; q < 4; ++q) mv[m][q] = T[tx][ty + 8 * q];
     }
     Sum4 s = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1006,6 +1040,7 @@ hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const T
 {
     const int64_t total = (int64_t)ntasks * tengy_npairs(g);
     if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
+    if ((int64_t)g.nvir * g.nvir * 64 >= (int64_t(1) << 32)) return hipErrorInvalidValue;  // 32-bit offsets
     hipLaunchKernelGGL(k_tengy, dim3((unsigned)total), dim3(256), 0, st, f, fstride, g, tasks,
                        ntasks, ev, vec6, partial);
     return hipGetLastError();
