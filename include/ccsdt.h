@@ -192,13 +192,22 @@ enum {
     CCSDT_TILE_TWO_WAVE = 1   /* two waves share the pair, eight products each */
 };
 
+/* Order in which the fused kernel's work items (triple x 16x16 block pair) run.
+ * Results do not depend on it (each partial is stored by triple and pair). */
+enum {
+    CCSDT_ORDER_TASK = 0,  /* all block pairs of a triple together, triples in list order */
+    CCSDT_ORDER_PAIR = 1   /* one block pair across all triples (sorted by k slice) at a
+                              time: the workgroups in flight share operand panels */
+};
+
 typedef struct {
     int mode;       /* CCSDT_MODE_* */
     int slices;     /* Ozaki slice count (3..7), ignored otherwise; 0 = 4 */
     int flags;      /* CCSDT_RUN_* */
     int variant;    /* CCSDT_TILE_* (FP64 modes) */
     int wg_per_cu;  /* persistent grid: workgroups per CU (0 = as many as fit) */
-    int reserved[3];
+    int order;      /* CCSDT_ORDER_* (fused modes; the separate-pass mode runs task order) */
+    int reserved[2];
 } ccsdt_run_opts;
 
 typedef struct {
@@ -224,8 +233,9 @@ int ccsdt_job_destroy(ccsdt_job job);
 
 /* Autotuner.  No counterpart in the reference (one DGEMM call per product,
  * omp/ccsd_trpdrv_omp_cbody.c:71-116, with MKL's own blocking).  Times every
- * launch configuration of the job's triple list -- tile variant x schedule
- * (one workgroup per item, or persistent grids of 1..max workgroups per CU) --
+ * launch configuration of the job's triple list -- item order x tile variant x
+ * schedule (one workgroup per item, or persistent grids at the occupancy limit
+ * and at half of it) --
  * with base's mode, slices and graph flag, and writes the fastest to *best.
  * Each candidate runs once untimed, then `reps` times; its time is the median
  * of ms_total.  table (may be NULL) receives up to cap entries, *count the

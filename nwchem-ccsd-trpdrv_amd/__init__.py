@@ -33,6 +33,7 @@ JK_NAMES = ["jia", "jka", "kia", "kka", "jij", "jkj", "kij", "kkj"]
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3}
 MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
+ORDER_TASK, ORDER_PAIR = 0, 1
 RUN_PERSISTENT, RUN_GRAPH = 1, 2
 E_CODES = {0: "OK", -1: "E_ARG", -2: "E_DEVICE", -3: "E_NOMEM", -4: "E_BUDGET",
            -5: "E_STATE", -6: "E_INJECTED"}
@@ -51,14 +52,14 @@ class JobDesc(ctypes.Structure):
 
 class RunOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slices", ctypes.c_int), ("flags", ctypes.c_int),
-                ("variant", ctypes.c_int), ("wg_per_cu", ctypes.c_int),
-                ("reserved", ctypes.c_int * 3)]
+                ("variant", ctypes.c_int), ("wg_per_cu", ctypes.c_int), ("order", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
     def as_dict(self):
         return {"mode": MODE_NAMES[self.mode], "slices": self.slices,
                 "persistent": bool(self.flags & RUN_PERSISTENT),
                 "graph": bool(self.flags & RUN_GRAPH), "variant": self.variant,
-                "wg_per_cu": self.wg_per_cu}
+                "wg_per_cu": self.wg_per_cu, "order": self.order}
 
 
 class TuneEntry(ctypes.Structure):
@@ -323,14 +324,15 @@ class Job:
         self.handle = h
 
     def run(self, triples, mode="fp64", slices=0, terms=False, fout=False, persistent=False,
-            graph=False, variant=0, wg_per_cu=0):
+            graph=False, variant=0, wg_per_cu=0, order=0):
         """Runs the triples; persistent / graph select CCSDT_RUN_PERSISTENT and
         CCSDT_RUN_GRAPH, variant the FP64 tile (CCSDT_TILE_*), wg_per_cu the
-        persistent grid (0 = as many workgroups per CU as fit)."""
+        persistent grid (0 = as many workgroups per CU as fit), order the work
+        item order (ORDER_TASK or ORDER_PAIR)."""
         trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
         n = trip.shape[0]
         flags = (RUN_PERSISTENT if persistent else 0) | (RUN_GRAPH if graph else 0)
-        opts = RunOpts(MODES[mode], slices, flags, variant, wg_per_cu)
+        opts = RunOpts(MODES[mode], slices, flags, variant, wg_per_cu, order)
         res = RunResult()
         t = np.empty((n, 4)) if terms else None
         f = np.empty((8, self.nvir, self.nvir)) if fout else None

@@ -236,7 +236,7 @@ constexpr int64_t kFWorkspaceBytes = int64_t(4) << 30;
 
 // ------------------------------------------------------------------ job
 // entries of a captured graph's signature (job_run)
-constexpr int kGraphKey = 15;
+constexpr int kGraphKey = 18;
 
 struct ccsdt_job_s {
     Geom g{};
@@ -257,6 +257,8 @@ struct ccsdt_job_s {
     short2 *d_pairs = nullptr;
     Task *d_tasks = nullptr;
     Task *h_tasks = nullptr;
+    int *d_order = nullptr;  // tasks sorted by k slice (pair-major execution order)
+    int *h_order = nullptr;
     int64_t cap_tasks = 0;
     double *d_partial = nullptr;
     int64_t cap_partial = 0;
@@ -275,7 +277,7 @@ struct ccsdt_job_s {
     // captured run (hipGraph) and the signature it was captured for
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int64_t gkey[kGraphKey] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    int64_t gkey[kGraphKey];  // filled with -1 by job_create
     // Ozaki INT8 split form (allocated on the first split run)
     int split_slices = 0;
     int8_t *d_digits = nullptr;
@@ -309,6 +311,8 @@ void job_free(ccsdt_job j)
     if (j->d_pairs) dev_free(j->d_pairs);
     if (j->d_tasks) dev_free(j->d_tasks);
     if (j->h_tasks) (void)hipHostFree(j->h_tasks);
+    if (j->d_order) dev_free(j->d_order);
+    if (j->h_order) (void)hipHostFree(j->h_order);
     if (j->h_stage) (void)hipHostFree(j->h_stage);
     if (j->h_vec) (void)hipHostFree(j->h_vec);
     if (j->h_fstage) (void)hipHostFree(j->h_fstage);
@@ -391,12 +395,18 @@ int ensure_tasks(ccsdt_job j, int64_t n)
     if (n <= j->cap_tasks) return CCSDT_OK;
     if (j->d_tasks) dev_free(j->d_tasks);
     if (j->h_tasks) (void)hipHostFree(j->h_tasks);
+    if (j->d_order) dev_free(j->d_order);
+    if (j->h_order) (void)hipHostFree(j->h_order);
     j->d_tasks = nullptr;
     j->h_tasks = nullptr;
+    j->d_order = nullptr;
+    j->h_order = nullptr;
     j->cap_tasks = 0;
     int rc = dev_alloc((void **)&j->d_tasks, (size_t)n * sizeof(Task));
     if (rc) return rc;
+    if ((rc = dev_alloc((void **)&j->d_order, (size_t)n * sizeof(int)))) return rc;
     HIP_TRY(hipHostMalloc((void **)&j->h_tasks, (size_t)n * sizeof(Task), 0));
+    HIP_TRY(hipHostMalloc((void **)&j->h_order, (size_t)n * sizeof(int), 0));
     j->cap_tasks = n;
     return CCSDT_OK;
 }
@@ -409,6 +419,7 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged)
     rc = ensure_device(-1);
     if (rc) return rc;
     ccsdt_job j = new ccsdt_job_s();
+    for (int q = 0; q < kGraphKey; ++q) j->gkey[q] = -1;
     j->g = make_geom(desc->ncor, desc->nocc, desc->nvir);
     j->kchunk = desc->kchunk;
     const Geom &g = j->g;
@@ -513,6 +524,11 @@ int job_tasks(ccsdt_job j, const int *trip, int64_t n)
         // omp/ccsd_trpdrv_omp_cbody.c:126, same operation order
         tk.eaijk = e[a - 1] - (e[g.ncor + i - 1] + e[g.ncor + jj - 1] + e[g.ncor + k - 1]);
     }
+    // pair-major execution order: the tasks sorted by k slice (stable counting sort)
+    std::vector<int64_t> start(j->kchunk + 1, 0);
+    for (int64_t t = 0; t < n; ++t) ++start[j->h_tasks[t].slice + 1];
+    for (int s = 0; s < j->kchunk; ++s) start[s + 1] += start[s];
+    for (int64_t t = 0; t < n; ++t) j->h_order[start[j->h_tasks[t].slice]++] = (int)t;
     return CCSDT_OK;
 }
 
@@ -558,7 +574,7 @@ int persistent_grid(bool split, int slices, int variant, int wg_per_cu)
 }
 
 // Fills the one-job tables of a fused run (job 0, one segment of n tasks).
-void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f)
+void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f, bool pmajor)
 {
     JobDev &jd = *j->h_jobdev;
     jd.g = j->g;
@@ -578,7 +594,8 @@ void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f)
     sgm.job = 0;
     sgm.item0 = 0;
     sgm.ntasks = (int)n;
-    sgm.pad = 0;
+    sgm.order = pmajor ? j->d_order : nullptr;
+    sgm.pmajor = pmajor ? 1 : 0;
     WorkHdr &h = *j->h_hdr;
     memset(&h, 0, sizeof(h));  // also zeroes the persistent item counters
     h.total = (int)(n * j->g.npairs);
@@ -597,6 +614,8 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
     hipEvent_t *E = j->ev;
     const bool split = (mode == CCSDT_MODE_OZAKI_INT8);
     HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice, j->st));
+    if (j->h_seg->order)
+        HIP_TRY(hipMemcpyAsync(j->d_order, j->h_order, (size_t)n * sizeof(int), hipMemcpyHostToDevice, j->st));
     HIP_TRY(hipMemcpyAsync(j->d_jobdev, j->h_jobdev, sizeof(JobDev), hipMemcpyHostToDevice, j->st));
     HIP_TRY(hipMemcpyAsync(j->d_seg, j->h_seg, sizeof(Segment), hipMemcpyHostToDevice, j->st));
     HIP_TRY(hipMemcpyAsync(j->d_hdr, j->h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, j->st));
@@ -652,6 +671,9 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const int wg_per_cu = opts ? opts->wg_per_cu : 0;
     if (wg_per_cu < 0 || wg_per_cu > 64)
         return set_error(CCSDT_E_ARG, "workgroups per CU %d outside 0..64", wg_per_cu);
+    const int order = opts ? opts->order : CCSDT_ORDER_TASK;
+    if (order != CCSDT_ORDER_TASK && order != CCSDT_ORDER_PAIR)
+        return set_error(CCSDT_E_ARG, "unknown item order %d", order);
     int rc = ensure_device(-1);
     if (rc) return rc;
     const Geom &g = j->g;
@@ -692,7 +714,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const bool use_graph = (flags & CCSDT_RUN_GRAPH) != 0 && fused && !fouts;
     INJECT_POINT("launch");
     if (fused) {
-        fill_tables(j, sg, n, fouts != nullptr);
+        fill_tables(j, sg, n, fouts != nullptr, order == CCSDT_ORDER_PAIR);
         if (use_graph) {
             if (terms && n > j->cap_hterms) {
                 if (j->h_terms) (void)hipHostFree(j->h_terms);
@@ -704,7 +726,8 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             // the signature, and every buffer the captured nodes reference (a run
             // of another size may have reallocated them)
             const int64_t key[kGraphKey] = {mode, slices, n, persistent, terms != nullptr, variant,
-                                     persistent ? wg_per_cu : 0,
+                                     persistent ? wg_per_cu : 0, order,
+                                     (int64_t)(intptr_t)j->d_order, (int64_t)(intptr_t)j->h_order,
                                      (int64_t)(intptr_t)j->d_scales,
                                      (int64_t)(intptr_t)j->d_tasks, (int64_t)(intptr_t)j->h_tasks,
                                      (int64_t)(intptr_t)j->d_partial, (int64_t)(intptr_t)j->d_terms,
@@ -748,7 +771,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
         HIP_TRY(hipEventRecord(E[1], j->st));
         launches = 1;
-        fill_tables(j, sg, n, true);
+        fill_tables(j, sg, n, true, false);
         for (int64_t t0 = 0; t0 < n; t0 += chunk) {
             const int m = (int)std::min<int64_t>(chunk, n - t0);
             j->h_seg->tasks = j->d_tasks + t0;
@@ -813,10 +836,10 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
 // under the split form's 1e-8 bound, since the tuning list is a sample.
 constexpr double kSliceTol = 1e-9;
 
-// Times every launch configuration of one triple list (tile variant x schedule)
-// and keeps the fastest.  The schedules are one workgroup per work item, and
-// persistent grids at the occupancy limit and at half of it (fewer resident
-// workgroups trade latency hiding for L2 reuse).
+// Times every launch configuration of one triple list (item order x tile
+// variant x schedule) and keeps the fastest.  The schedules are one workgroup
+// per work item, and persistent grids at the occupancy limit and at half of it
+// (fewer resident workgroups trade latency hiding for L2 reuse).
 int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *base, int reps,
                  ccsdt_run_opts *best, ccsdt_tune_entry *table, int cap, int *count)
 {
@@ -851,19 +874,22 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
         if (b0.slices == 0) b0.slices = 7;
     }
     const int slices = split ? (b0.slices ? b0.slices : 4) : 0;
-    for (int v = 0; v < (split ? 1 : 2); ++v) {
-        ccsdt_run_opts o = b0;
-        o.variant = v;
-        o.flags = b0.flags & ~CCSDT_RUN_PERSISTENT;
-        o.wg_per_cu = 0;
-        cand.push_back(o);
-        if (!fused) continue;  // the f-writing kernel has no persistent form
-        const int fit = fused_blocks_per_cu(split, slices, v);
-        for (int w : {fit, fit / 2}) {
-            if (w < 1 || (w == fit / 2 && w == fit)) continue;
-            o.flags = b0.flags | CCSDT_RUN_PERSISTENT;
-            o.wg_per_cu = w;
+    for (int ord = 0; ord < (fused ? 2 : 1); ++ord) {
+        for (int v = 0; v < (split ? 1 : 2); ++v) {
+            ccsdt_run_opts o = b0;
+            o.order = ord;
+            o.variant = v;
+            o.flags = b0.flags & ~CCSDT_RUN_PERSISTENT;
+            o.wg_per_cu = 0;
             cand.push_back(o);
+            if (!fused) continue;  // the f-writing kernel has no persistent form
+            const int fit = fused_blocks_per_cu(split, slices, v);
+            for (int w : {fit, fit / 2}) {
+                if (w < 1 || (w == fit / 2 && w == fit)) continue;
+                o.flags = b0.flags | CCSDT_RUN_PERSISTENT;
+                o.wg_per_cu = w;
+                cand.push_back(o);
+            }
         }
     }
     int nbest = -1;
@@ -1256,7 +1282,8 @@ int server_launch_tick(ccsdt_server s, bool *launched)
         sg.job = sj->slot;
         sg.item0 = (int)items;
         sg.ntasks = (int)m;
-        sg.pad = 0;
+        sg.order = nullptr;
+        sg.pmajor = 0;
         items += m * np;
         tasks += m;
         sj->next += m;

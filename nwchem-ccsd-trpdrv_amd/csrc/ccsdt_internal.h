@@ -71,16 +71,22 @@ struct JobDev {
 };
 
 // A run of consecutive tasks of one job inside a launch.  Work items are
-// (task, block pair); the segment's items start at item0.
+// (task, block pair); the segment's items start at item0.  Task-major order
+// (pmajor = 0): item L of the segment is pair L % npairs of task L / npairs.
+// Pair-major order (pmajor = 1): item L is pair L / ntasks of task
+// order[L % ntasks], so the workgroups in flight share one block pair (and,
+// with order sorting the tasks by k slice, the same operand panels).  Partials
+// are stored by (task, pair) either way, so the order changes no result.
 struct Segment {
     const Task *tasks;   // the segment's first task
     double *partial;     // 4 doubles per (task, pair), task-major, from the first task
     double *fout;        // f matrices of the first task (fstride apart) or null
     double *terms;       // 4 doubles per task (emp4i, emp5i, emp4k, emp5k)
+    const int *order;    // pair-major: execution order of the tasks (null = as stored)
     int job;             // index into the JobDev table
     int item0;           // first work item of the segment
     int ntasks;
-    int pad;
+    int pmajor;
 };
 
 // Size of the current work list, read by the kernels (so a persistent launch

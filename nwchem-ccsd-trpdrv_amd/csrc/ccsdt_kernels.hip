@@ -140,6 +140,20 @@ __device__ __forceinline__ int next_item(WorkHdr *hdr, const ItemRange &ir)
 }
 
 
+// Task and block pair of work item `item` of segment sg (wave-uniform).
+__device__ __forceinline__ void decode_item(const Segment &sg, int npairs, int item, int &t, int &p)
+{
+    const int L = uniform(item - sg.item0);
+    if (sg.pmajor) {
+        p = uniform(L / sg.ntasks);
+        const int tt = L - p * sg.ntasks;
+        t = uniform(sg.order ? sg.order[tt] : tt);
+    } else {
+        t = uniform(L / npairs);
+        p = uniform(L - t * npairs);
+    }
+}
+
 // Segment holding work item `item` (segments sorted by item0; wave-uniform).
 __device__ __forceinline__ int find_segment(const Segment *__restrict__ segs, int nsegs, int item)
 {
@@ -357,12 +371,12 @@ __global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
         const int si = uniform(find_segment(segs, nsegs, item));
         const Segment sg = segs[si];
         const JobDev jd = jobs[uniform(sg.job)];
-        const int L = uniform(item - sg.item0);
-        const int t = uniform(L / jd.g.npairs);
-        const int p = uniform(L - t * jd.g.npairs);
+        int t, p;
+        decode_item(sg, jd.g.npairs, item, t, p);
         const Task tk = sg.tasks[t];
         const short2 pr = jd.pairs[p];
-        fused_item<TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), threadIdx.x, sg.partial + (int64_t)L * 4,
+        fused_item<TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), threadIdx.x,
+                                  sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
                                   WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }
 }
@@ -806,9 +820,8 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
         const int si = uniform(find_segment(segs, nsegs, item));
         const Segment sg = segs[si];
         const JobDev jd = jobs[uniform(sg.job)];
-        const int L = uniform(item - sg.item0);
-        const int t = uniform(L / jd.g.npairs);
-        const int p = uniform(L - t * jd.g.npairs);
+        int t, p;
+        decode_item(sg, jd.g.npairs, item, t, p);
         const Task tk = sg.tasks[t];
         const short2 pr = jd.pairs[p];
         // an opaque copy of the lane index: keeps the compiler from hoisting
@@ -816,7 +829,8 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
         // VGPRs through the whole item
         int lane_i = lane;
         if (PERSIST) asm volatile("" : "+v"(lane_i));
-        pair_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane_i, sg.partial + (int64_t)L * 4,
+        pair_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane_i,
+                                    sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
                                      WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }
 }

@@ -125,7 +125,9 @@ def test_autotune_picks_a_working_candidate(cc, mode, slices):
     assert all(row["status"] == 0 and row["ms"] > 0 for row in table)
     assert min(row["ms"] for row in table) == min(
         row["ms"] for row in table if row["variant"] == best["variant"]
-        and row["persistent"] == best["persistent"] and row["wg_per_cu"] == best["wg_per_cu"])
+        and row["persistent"] == best["persistent"] and row["wg_per_cu"] == best["wg_per_cu"]
+        and row["order"] == best["order"])
+    assert {row["order"] for row in table} == {0, 1}
     assert best["graph"]
     tol = 1e-12 if mode == "fp64" else 1e-9
     assert abs(got["emp4"] + got["emp5"] - ref["emp4"] - ref["emp5"]) <= tol * abs(ref["emp4"] + ref["emp5"])
@@ -153,3 +155,20 @@ def test_autotune_chooses_slice_count(cc, illcond):
     assert abs(s["emp4"] + s["emp5"] - E) <= 1e-9 * abs(E)
     # no smaller slice count met the 1e-9 selection bound
     assert all(devs[S] > 1e-9 * abs(E) for S in range(3, best["slices"]))
+
+
+@pytest.mark.parametrize("mode,slices,variant", [("fp64", 0, 0), ("fp64", 0, 1), ("ozaki_int8", 4, 0)])
+def test_pair_major_order_is_bitwise_task_order(cc, mode, slices, variant):
+    """Pair-major execution (tasks sorted by k slice) changes only the schedule:
+    every term is bitwise the task-major one, plain, persistent and in a graph."""
+    nocc, nvir = 10, 50
+    blocks = cc.blocks_uneven(nocc, 9, lo=2, hi=5)
+    kchunk = max(b - a + 1 for a, b in blocks)
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 71)
+    trip = cc.triple_list(nocc, blocks)
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        ref = job.run(trip, mode=mode, slices=slices, variant=variant, terms=True)
+        for kw in (dict(), dict(persistent=True), dict(persistent=True, graph=True)):
+            got = job.run(trip, mode=mode, slices=slices, variant=variant, terms=True, order=1, **kw)
+            assert np.array_equal(got["terms"], ref["terms"]), kw
+            assert got["emp4"] == ref["emp4"] and got["emp5"] == ref["emp5"]
