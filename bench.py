@@ -10,6 +10,12 @@ job's arrays already resident in HBM.  Both forms are timed in the same run:
 * the split form (the K=nvir contractions as Ozaki INT8 split products with S
   slices, FP64 recombination, FP64 K=nocc part and energy epilogue).
 
+Each form runs the configuration the library's autotuner (ccsdt_job_autotune)
+picks on the full list -- work-item order, tile variant, plain or persistent
+grid, and for the split form the slice count (the smallest whose E[T] is within
+1e-9 of the FP64 form's) -- captured in a hipGraph and replayed once per step.
+--fp64-opts / --split-opts pin a configuration instead (profiling runs).
+
 `value` is the split form's FP64-equivalent TFLOP/s (16*nvir^2*(nvir+nocc) flops
 per triple, 2 per multiply-add) when its E[T] = emp4+emp5 agrees with the FP64
 form's to 1e-8 relative, else the FP64 form's.  The JSON line also carries the
@@ -46,14 +52,28 @@ def load_pkg():
     return mod
 
 
-def traffic_from_profile(mode):
-    """HBM-side bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s_nocc30_nvir300.json" % mode)
+def traffic_from_profile(form, opts):
+    """HBM-side bytes per launch of the form's fused kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated),
+    or None when that profile was taken with other run options."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_nocc30_nvir300.json" % form)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        d = json.load(fh)
+    keys = ("order", "variant", "persistent", "slices")
+    prof = d.get("opts", {})
+    if any(prof.get(k) != opts.get(k) for k in keys):
+        return None
+    return d.get("bytes_per_launch")
+
+
+def kernel_name(opts):
+    """Name of the fused kernel a configuration launches."""
+    p = "true" if opts.get("persistent") else "false"
+    if opts["mode"] == "ozaki_int8":
+        return "k_fused_split<%d,true,false,%s>" % (opts["slices"], p)
+    return ("k_fused" if opts.get("variant", 0) == 0 else "k_fused2") + "<true,false,%s>" % p
 
 
 def time_form(job, trip, steps, warmup, **kw):
@@ -185,7 +205,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--slices", type=int, default=4, help="Ozaki slices of the split form")
+    ap.add_argument("--slices", type=int, default=0,
+                    help="Ozaki slices of the split form (0: chosen by the autotuner)")
+    ap.add_argument("--fp64-opts", default="", metavar="JSON",
+                    help="run the FP64 form with these run options instead of autotuning, "
+                         "e.g. '{\"order\": 1, \"variant\": 0, \"persistent\": true, "
+                         "\"wg_per_cu\": 0}'")
+    ap.add_argument("--split-opts", default="", metavar="JSON",
+                    help="the same for the split form (may hold \"slices\")")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", default="", metavar="RATES",
@@ -218,25 +245,35 @@ def main():
         print(json.dumps(batched_report(cc, args.steps, args.warmup, args.batched_slices)),
               flush=True)
         return
-    nocc, nvir, S = 30, 300, args.slices
+    nocc, nvir = 30, 300
     arrays, blocks, trip, kchunk = cc.config_case(1)
     n = len(trip)
     flops = cc.gemm_flops_per_triple(nocc, nvir)
     total_flops = n * flops
     job = cc.Job(arrays, nocc, nvir, kchunk)
 
-    # the FP64 form on both tiles; the faster one is the FP64 form
-    fps = {}
-    for v, vname in ((0, "one_wave"), (1, "two_wave")):
-        fps[vname] = time_form(job, trip, args.steps, args.warmup, mode="fp64", variant=v)
-        print("bench: fp64 form, %s tile: %.2f ms/step" % (vname, fps[vname]["ms_per_step"]),
-              file=sys.stderr, flush=True)
-    fp_tile = min(fps, key=lambda k: fps[k]["kernel_ms"])
-    fp = fps[fp_tile]
-    fp_kernel = "k_fused<true,false>" if fp_tile == "one_wave" else "k_fused2<true,false>"
-    sp = time_form(job, trip, args.steps, args.warmup, mode="ozaki_int8", slices=S)
-    print("bench: split form (s=%d) %.2f ms/step" % (S, sp["ms_per_step"]), file=sys.stderr,
-          flush=True)
+    # Each form: the autotuner picks item order, tile and schedule (and, for the
+    # split form with --slices 0, the slice count) on the full list; the pick
+    # then runs as a captured graph per step.  The fused kernel's own time comes
+    # from eager runs of the same configuration (HIP events around the launch).
+    forms = {}
+    for mode, slices, pinned in (("fp64", 0, args.fp64_opts), ("ozaki_int8", args.slices, args.split_opts)):
+        if pinned:
+            best = dict(json.loads(pinned), mode=mode, graph=True)
+            best.setdefault("slices", slices or (4 if mode == "ozaki_int8" else 0))
+            table = []
+        else:
+            best, table = job.autotune(trip, mode=mode, slices=slices, graph=True, reps=2)
+        r = time_form(job, trip, args.steps, args.warmup, **best)
+        eager = dict(best, graph=False)
+        kern = [job.run(trip, **eager)["ms_main"] for _ in range(3)]
+        r["kernel_ms"] = float(np.median(kern))
+        r["opts"], r["table"] = best, table
+        forms[mode] = r
+        print("bench: %s form %.2f ms/step, kernel %.2f ms (%s)" % (mode, r["ms_per_step"],
+              r["kernel_ms"], best), file=sys.stderr, flush=True)
+    fp, sp = forms["fp64"], forms["ozaki_int8"]
+    S = sp["opts"]["slices"]
     sep = job.run(trip, mode="fp64_split")  # GEMM writes f to HBM, separate energy pass
     job.close()
 
@@ -249,16 +286,18 @@ def main():
     def tf(ms):
         return total_flops / (ms * 1e-3) / 1e12
 
+    fp_kernel = kernel_name(fp["opts"])
+    sp_kernel = kernel_name(sp["opts"])
     # roofline of the dominant kernel of the headline form
     if split_ok:
         int8_ops = n * 2.0 * 8 * nvir ** 3 * S * (S + 1) / 2  # Ozaki products over K=nvir
         fp64_part = n * 2.0 * 8 * nvir ** 2 * nocc           # K=nocc products in FP64
         t = sp["kernel_ms"] * 1e-3
         ideal = int8_ops / (INT8_MFMA_PEAK * 1e12) + fp64_part / (FP64_MFMA_PEAK * 1e12)
-        roof = {"bound": "mfma", "kernel": "k_fused_split<%d,true,false>" % S,
+        roof = {"bound": "mfma", "kernel": sp_kernel,
                 "achieved": int8_ops / t / 1e12, "peak": INT8_MFMA_PEAK, "unit": "TOP/s",
                 "frac": int8_ops / t / 1e12 / INT8_MFMA_PEAK,
-                "traffic": traffic_from_profile("split_s%d" % S),
+                "traffic": traffic_from_profile("split", sp["opts"]),
                 "kernel_ms": sp["kernel_ms"], "int8_ops_per_launch": int8_ops,
                 "fp64_flops_per_launch": fp64_part,
                 "ideal_time_frac": ideal / t,
@@ -267,8 +306,12 @@ def main():
         roof = {"bound": "mfma", "kernel": fp_kernel,
                 "achieved": tf(fp["kernel_ms"]), "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
                 "frac": tf(fp["kernel_ms"]) / FP64_MFMA_PEAK,
-                "traffic": traffic_from_profile("fp64"), "kernel_ms": fp["kernel_ms"],
+                "traffic": traffic_from_profile("fp64", fp["opts"]), "kernel_ms": fp["kernel_ms"],
                 "flops_per_launch": total_flops}
+
+    def cand(table):
+        return [{k: v for k, v in row.items() if k not in ("mode", "graph", "status")}
+                for row in table]
 
     tengy_gbs = n * cc.tengy_bytes_per_triple(nvir) / (sep["ms_tengy"] * 1e-3) / 1e9
     line = {
@@ -286,14 +329,14 @@ def main():
         "job_p50_ms": head["p50_ms"], "job_p99_ms": head["p99_ms"],
         "fp64_form": {"value": tf(fp["ms_per_step"]), "ms_per_step": fp["ms_per_step"],
                       "kernel": fp_kernel, "kernel_ms": fp["kernel_ms"],
-                      "tiles": {k: {"kernel_ms": x["kernel_ms"], "kernel_tflops": tf(x["kernel_ms"])}
-                                for k, x in fps.items()},
                       "kernel_tflops": tf(fp["kernel_ms"]),
                       "frac_fp64_peak": tf(fp["kernel_ms"]) / FP64_MFMA_PEAK,
-                      "traffic": traffic_from_profile("fp64"),
+                      "traffic": traffic_from_profile("fp64", fp["opts"]),
                       "p50_ms": fp["p50_ms"], "p99_ms": fp["p99_ms"],
-                      "emp4": fp["res"]["emp4"], "emp5": fp["res"]["emp5"]},
-        "split_form": {"slices": S, "value": tf(sp["ms_per_step"]),
+                      "emp4": fp["res"]["emp4"], "emp5": fp["res"]["emp5"],
+                      "autotuned": fp["opts"], "candidates": cand(fp["table"])},
+        "split_form": {"slices": S, "kernel": sp_kernel, "autotuned": sp["opts"],
+                       "candidates": cand(sp["table"]), "value": tf(sp["ms_per_step"]),
                        "ms_per_step": sp["ms_per_step"], "kernel_ms": sp["kernel_ms"],
                        "p50_ms": sp["p50_ms"], "p99_ms": sp["p99_ms"],
                        "emp4": sp["res"]["emp4"], "emp5": sp["res"]["emp5"],
