@@ -200,6 +200,29 @@ def batched_report(cc, steps, warmup, slices):
     return out
 
 
+def dropin_report(cc, arrays, trip, nocc, nvir, flops, ncalls=24):
+    """The synchronous drop-in body (ccsd_trpdrv_omp_cbody_) on host buffers:
+    every call copies the slices it reads over PCIe and, with f write-back on
+    (the reference leaves f1n..f4t filled), copies 8 nvir^2 doubles back."""
+    out = {}
+    f = [np.zeros(nvir * nvir) for _ in range(8)]
+    for wb in (1, 0):
+        cc.lib().ccsdt_set_f_writeback(wb)
+        e4 = e5 = 0.0
+        for a, i, j, k, klo in trip[:2]:
+            e4, e5 = cc.cbody(f, arrays["eorb"], 0, nocc, nvir, e4, e5, a, i, j, k, klo, arrays)
+        t0 = time.perf_counter()
+        for a, i, j, k, klo in trip[:ncalls]:
+            e4, e5 = cc.cbody(f, arrays["eorb"], 0, nocc, nvir, e4, e5, a, i, j, k, klo, arrays)
+        dt = (time.perf_counter() - t0) / ncalls
+        out["f_writeback" if wb else "no_f_writeback"] = {
+            "ms_per_call": dt * 1e3, "triples_per_s": 1.0 / dt,
+            "fp64_eq_tflops": flops / dt / 1e12}
+    cc.lib().ccsdt_set_f_writeback(1)
+    out["calls"] = ncalls
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -276,6 +299,7 @@ def main():
     S = sp["opts"]["slices"]
     sep = job.run(trip, mode="fp64_split")  # GEMM writes f to HBM, separate energy pass
     job.close()
+    dropin = dropin_report(cc, arrays, trip, nocc, nvir, flops)
 
     e_fp = fp["res"]["emp4"] + fp["res"]["emp5"]
     e_sp = sp["res"]["emp4"] + sp["res"]["emp5"]
@@ -342,6 +366,7 @@ def main():
                        "emp4": sp["res"]["emp4"], "emp5": sp["res"]["emp5"],
                        "rel_dE_vs_fp64": dev, "within_tol": split_ok,
                        "speedup_vs_fp64": fp["ms_per_step"] / sp["ms_per_step"]},
+        "dropin_pcie_inclusive": dropin,
         "tengy_pass": {"kernel": "k_tengy", "ms": sep["ms_tengy"], "GB_per_s": tengy_gbs,
                        "hbm_frac": tengy_gbs / HBM_PEAK,
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),

@@ -1283,7 +1283,7 @@ int server_launch_tick(ccsdt_server s, bool *launched)
         sg.item0 = (int)items;
         sg.ntasks = (int)m;
         sg.order = nullptr;
-        sg.pmajor = 0;
+        sg.pmajor = 1;  // block-pair-major over the share: workgroups in flight share panels
         items += m * np;
         tasks += m;
         sj->next += m;

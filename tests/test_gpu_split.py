@@ -84,14 +84,17 @@ def test_production_shape_golden(cc, orc):
 
 def test_full_list_vs_fp64_form(cc):
     """All 4960 triples at 30/300: the split form's E[T] within 1e-8 of the FP64
-    form's, and bitwise repeatable."""
+    form's at 4 slices and at 3 (the count the autotuner picks at this shape),
+    and bitwise repeatable."""
     arrays, blocks, trip, kchunk = cc.config_case(1)
     with cc.Job(arrays, 30, 300, kchunk) as job:
         a = job.run(trip, mode="fp64")
         b = job.run(trip, mode="ozaki_int8", slices=4, terms=True)
         c = job.run(trip, mode="ozaki_int8", slices=4, terms=True)
+        d = job.run(trip, mode="ozaki_int8", slices=3, order=1)
     E = et(a["emp4"], a["emp5"])
     assert abs(et(b["emp4"], b["emp5"]) - E) / abs(E) < ET_TOL
+    assert abs(et(d["emp4"], d["emp5"]) - E) / abs(E) < ET_TOL
     assert np.array_equal(b["terms"], c["terms"])
 
 
