@@ -257,6 +257,29 @@ int ccsdt_job_autotune(ccsdt_job job, const int *triples, int64_t n, const ccsdt
 /* Device bytes the job holds (arrays + packed operands + workspace for n triples). */
 int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples);
 
+/* ---------------------------------------------------------------- full-tensor jobs
+ * No counterpart in the reference: its harness holds one slice per operand
+ * (omp/test.c:112-146) and its NWChem caller fetches the slices of every
+ * (i,j,k) from global arrays before each body call.  A full-tensor job holds the
+ * slices of every occupied index, so any triple runs without a host gather.
+ * With 0-based i, j, k, vv = nvir*nvir and ov = nocc*nvir, the body's operands
+ * of triple (a,i,j,k) are (each slice in the body's layout):
+ *   Tij = t2 + (j*nocc + i)*vv      Tkj = t2 + (j*nocc + k)*vv
+ *   Jia = jv + i*vv                 Jka = jv + k*vv        (Kia, Kka from kv alike)
+ *   Tia = t1o + i*ov                Tka = t1o + k*ov       (Xia, Xka from xo alike)
+ *   Jij = jo + (j*nocc + i)*ov      Jkj = jo + (j*nocc + k)*ov  (Kij, Kkj from ko alike)
+ * eorb and the six nvir vectors are as in ccsdt_job_desc.  Triples are run
+ * with ccsdt_job_run; their klo is checked (1 <= klo <= k) and otherwise unused. */
+typedef struct {
+    int ncor, nocc, nvir, pad;
+    const double *eorb;
+    const double *t2, *jv, *kv, *t1o, *xo, *jo, *ko;
+    const double *dintc1, *dintx1, *t1v1, *dintc2, *dintx2, *t1v2;
+} ccsdt_tensor_desc;
+
+int ccsdt_job_create_tensors(const ccsdt_tensor_desc *desc, ccsdt_job *out);
+int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriples);
+
 /* ---------------------------------------------------------------- serving */
 /* No counterpart in the reference (whose harness runs one job, omp/test.c).  A
  * server keeps admitted jobs device-resident and interleaves their triples in
@@ -289,6 +312,9 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out);
  * the budget. */
 int ccsdt_server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples,
                         int64_t n, int64_t *id);
+/* The same for a full-tensor job (footprint from ccsdt_tensor_job_footprint). */
+int ccsdt_server_submit_tensors(ccsdt_server s, const ccsdt_tensor_desc *desc, const int *triples,
+                                int64_t n, int64_t *id);
 /* Retires finished ticks, finishes jobs, admits waiting jobs and launches ticks.
  * block = 1 waits for the oldest tick in flight first.  Returns the number of
  * jobs finished by this call, or a negative error. */

@@ -30,6 +30,15 @@ STREAM = {"eorb": 9, "tij": 20, "tkj": 21, "tia": 22, "tka": 23, "xia": 24, "xka
 ILLCOND_STREAM_BASE = 100   # exponent draws of the ill-conditioned variant
 JK_NAMES = ["jia", "jka", "kia", "kka", "jij", "jkj", "kij", "kkj"]
 
+# Full-tensor jobs (ccsdt_job_create_tensors): every occupied-indexed slice of
+# the body's operands.  Each family draws from the stream of the body argument
+# it generalises.
+TENSOR_NAMES = ["t2", "jv", "kv", "t1o", "xo", "jo", "ko"]
+TENSOR_STREAM = {"t2": STREAM["tij"], "jv": STREAM["jia"], "kv": STREAM["kia"],
+                 "t1o": STREAM["tia"], "xo": STREAM["xia"], "jo": STREAM["jij"],
+                 "ko": STREAM["kij"]}
+VECTOR_NAMES = ["dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2"]
+
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3}
 MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
@@ -48,6 +57,24 @@ class CcsdtError(RuntimeError):
 class JobDesc(ctypes.Structure):
     _fields_ = [("ncor", ctypes.c_int), ("nocc", ctypes.c_int), ("nvir", ctypes.c_int),
                 ("kchunk", ctypes.c_int), ("arrays", ctypes.c_void_p * len(ARRAY_NAMES))]
+
+
+class TensorDesc(ctypes.Structure):
+    _fields_ = ([("ncor", ctypes.c_int), ("nocc", ctypes.c_int), ("nvir", ctypes.c_int),
+                 ("pad", ctypes.c_int), ("eorb", ctypes.c_void_p)]
+                + [(n, ctypes.c_void_p) for n in TENSOR_NAMES]
+                + [(n, ctypes.c_void_p) for n in VECTOR_NAMES])
+
+
+def _tensor_desc(t, nocc, nvir, ncor):
+    """TensorDesc over contiguous float64 copies of t (returned to keep alive)."""
+    d = TensorDesc(ncor, nocc, nvir, 0)
+    keep = []
+    for name in ["eorb"] + TENSOR_NAMES + VECTOR_NAMES:
+        a = np.ascontiguousarray(t[name], dtype=np.float64)
+        keep.append(a)
+        setattr(d, name, a.ctypes.data)
+    return d, keep
 
 
 class RunOpts(ctypes.Structure):
@@ -126,6 +153,10 @@ def lib():
                                                   ctypes.POINTER(RunOpts), ctypes.POINTER(TuneEntry),
                                                   ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
             "ccsdt_job_footprint": (i64, [ctypes.c_int] * 4 + [i64]),
+            "ccsdt_job_create_tensors": (ctypes.c_int, [ctypes.POINTER(TensorDesc), ctypes.POINTER(vp)]),
+            "ccsdt_tensor_job_footprint": (i64, [ctypes.c_int] * 3 + [i64]),
+            "ccsdt_server_submit_tensors": (ctypes.c_int, [vp, ctypes.POINTER(TensorDesc), vp, i64,
+                                                           ctypes.POINTER(i64)]),
             "ccsdt_selftest_mfma": (ctypes.c_int, [vp, vp, vp]),
             "ccsdt_selftest_mfma_i8": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
             "ccsdt_debug_copy": (ctypes.c_double, [i64, ctypes.c_int]),
@@ -229,6 +260,57 @@ def make_inputs(nocc, nvir, kchunk, seed, ncor=0, illcond=False):
     return arrays
 
 
+def make_tensor_inputs(nocc, nvir, seed, ncor=0, illcond=False):
+    """Seeded full-tensor job inputs (see ccsdt_tensor_desc): eorb as in
+    make_inputs; t2 (nocc*nocc*nvir^2), jv, kv (nocc*nvir^2), t1o, xo
+    (nocc*nocc*nvir), jo, ko (nocc*nocc*nocc*nvir) and the six nvir vectors, all
+    U(-0.05,0.05).  ``illcond`` scales jv, kv, jo, ko by 10**u, u ~ U(-6,0)."""
+    vv, ov = nvir * nvir, nocc * nvir
+    base = make_inputs(nocc, nvir, 1, seed, ncor=ncor)
+    sizes = {"t2": nocc * nocc * vv, "jv": nocc * vv, "kv": nocc * vv, "t1o": nocc * ov,
+             "xo": nocc * ov, "jo": nocc * nocc * ov, "ko": nocc * nocc * ov}
+    t = {"eorb": base["eorb"]}
+    for name, n in sizes.items():
+        t[name] = pcg_uniform(seed, TENSOR_STREAM[name], -0.05, 0.05, n)
+        if illcond and name in ("jv", "kv", "jo", "ko"):
+            e = pcg_uniform(seed, ILLCOND_STREAM_BASE + TENSOR_STREAM[name], -6.0, 0.0, n)
+            t[name] = t[name] * np.power(10.0, e)
+    for name in VECTOR_NAMES:
+        t[name] = base[name][:nvir].copy()
+    return t
+
+
+def gather_triple(t, nocc, nvir, i, j, k):
+    """Harness-shaped arrays (kchunk 1, klo = k) holding the slices triple
+    (i,j,k) (1-based) reads from a full-tensor job -- the host gather the
+    tensor job makes unnecessary, used to check it against the oracle."""
+    vv, ov = nvir * nvir, nocc * nvir
+    i0, j0, k0 = i - 1, j - 1, k - 1
+
+    def sl(name, idx, n):
+        return t[name][idx * n:(idx + 1) * n]
+    a = {"eorb": t["eorb"],
+         "tij": sl("t2", j0 * nocc + i0, vv), "tkj": sl("t2", j0 * nocc + k0, vv),
+         "jia": sl("jv", i0, vv), "jka": sl("jv", k0, vv),
+         "kia": sl("kv", i0, vv), "kka": sl("kv", k0, vv),
+         "tia": sl("t1o", i0, ov), "tka": sl("t1o", k0, ov),
+         "xia": sl("xo", i0, ov), "xka": sl("xo", k0, ov),
+         "jij": sl("jo", j0 * nocc + i0, ov), "jkj": sl("jo", j0 * nocc + k0, ov),
+         "kij": sl("ko", j0 * nocc + i0, ov), "kkj": sl("ko", j0 * nocc + k0, ov)}
+    # at the harness allocation sizes (omp/test.c:125-146), of which the body reads
+    # the first nocc*nvir (or nvir) elements
+    for name in ("tia", "xia", "jij", "kij"):
+        a[name] = np.concatenate([a[name], np.zeros(ov * (nocc - 1))])
+    for name in VECTOR_NAMES:
+        a[name] = np.concatenate([t[name], np.zeros(vv - nvir)])
+    return a
+
+
+def tensor_footprint(nocc, nvir, ntriples, ncor=0):
+    """Device bytes of a full-tensor job (ccsdt_tensor_job_footprint)."""
+    return lib().ccsdt_tensor_job_footprint(ncor, nocc, nvir, ntriples)
+
+
 def blocks_uniform(nocc, length):
     """k blocks klo..khi of a fixed length (kchunk, omp/test.c:97,160-161)."""
     length = max(1, min(int(length), nocc))
@@ -323,6 +405,20 @@ class Job:
         self._keep = None
         self.handle = h
 
+    @classmethod
+    def from_tensors(cls, tensors, nocc, nvir, ncor=0):
+        """A full-tensor job (ccsdt_job_create_tensors); tensors as made by
+        make_tensor_inputs.  Any triple (a,i,j,k,klo) with 1 <= klo <= k runs."""
+        self = cls.__new__(cls)
+        self.ncor, self.nocc, self.nvir, self.kchunk = ncor, nocc, nvir, nocc
+        d, keep = _tensor_desc(tensors, nocc, nvir, ncor)
+        h = ctypes.c_void_p()
+        _check(lib().ccsdt_job_create_tensors(ctypes.byref(d), ctypes.byref(h)))
+        del keep
+        self._keep = None
+        self.handle = h
+        return self
+
     def run(self, triples, mode="fp64", slices=0, terms=False, fout=False, persistent=False,
             graph=False, variant=0, wg_per_cu=0, order=0):
         """Runs the triples; persistent / graph select CCSDT_RUN_PERSISTENT and
@@ -410,6 +506,15 @@ class Server:
         jid = ctypes.c_int64()
         _check(lib().ccsdt_server_submit(self.handle, ctypes.byref(d), _ptr(trip), trip.shape[0],
                                          ctypes.byref(jid)))
+        return jid.value
+
+    def submit_tensors(self, tensors, nocc, nvir, triples, ncor=0):
+        d, keep = _tensor_desc(tensors, nocc, nvir, ncor)
+        trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
+        jid = ctypes.c_int64()
+        _check(lib().ccsdt_server_submit_tensors(self.handle, ctypes.byref(d), _ptr(trip),
+                                                 trip.shape[0], ctypes.byref(jid)))
+        del keep
         return jid.value
 
     def progress(self, block=False):

@@ -209,9 +209,24 @@ SplitGeom make_split_geom(const Geom &g, int slices, int npanels)
 
 // Element counts of the job arrays (omp/test.c:112-146 shapes; tia, xia, jij,
 // kij and the dint/t1v vectors at the lengths the body reads).
-void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n)
+void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n, bool tensors = false)
 {
     const int64_t vv = (int64_t)nvir * nvir, ov = (int64_t)nocc * nvir;
+    if (tensors) {
+        // full-tensor job: the i-indexed slots hold every occupied index, the
+        // k-indexed slots are unused (ccsdt_tensor_desc)
+        for (int q = 0; q < CCSDT_NARRAYS; ++q) n[q] = 0;
+        n[CCSDT_EORB] = ncor + nocc + nvir;
+        n[CCSDT_TIJ] = (int64_t)nocc * nocc * vv;
+        n[CCSDT_JIA] = nocc * vv;
+        n[CCSDT_KIA] = nocc * vv;
+        n[CCSDT_TIA] = nocc * ov;
+        n[CCSDT_XIA] = nocc * ov;
+        n[CCSDT_JIJ] = (int64_t)nocc * nocc * ov;
+        n[CCSDT_KIJ] = (int64_t)nocc * nocc * ov;
+        for (int q = CCSDT_DINTC1; q <= CCSDT_T1V2; ++q) n[q] = nvir;
+        return;
+    }
     n[CCSDT_EORB] = ncor + nocc + nvir;
     n[CCSDT_TIJ] = vv;
     n[CCSDT_TKJ] = kchunk * vv;
@@ -234,7 +249,28 @@ void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n)
 constexpr int64_t kFWorkspaceBytes = int64_t(4) << 30;
 }  // namespace
 
+// Panels of a job: 4 job-wide + 4 per k slice (harness shape), or the L(p)
+// and R(p, j) pair families of a full-tensor job.
+int64_t job_npanels(int nocc, int kchunk, bool tensors)
+{
+    return tensors ? 2 * ((int64_t)nocc + (int64_t)nocc * nocc) : PAN_SLICE0 + 4 * (int64_t)kchunk;
+}
+
 // ------------------------------------------------------------------ job
+// Device bytes of a job: arrays, packed panels, vectors, and per triple the
+// partials, terms, task record and order entry.
+int64_t job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples, bool tensors)
+{
+    const Geom g = make_geom(ncor, nocc, nvir);
+    int64_t cnt[CCSDT_NARRAYS];
+    array_sizes(ncor, nocc, nvir, kchunk, cnt, tensors);
+    int64_t raw = 0;
+    for (int q = 0; q < CCSDT_NARRAYS; ++q) raw += round_up(cnt[q], 32);
+    const int64_t pan = job_npanels(nocc, kchunk, tensors) * g.vecsz;
+    const int64_t per_task = (int64_t)g.npairs * 4 + 4 + (int64_t)(sizeof(Task) + sizeof(int)) / 8 + 1;
+    return (raw + pan + 7 * g.nvp + ntriples * per_task) * (int64_t)sizeof(double);
+}
+
 // entries of a captured graph's signature (job_run)
 constexpr int kGraphKey = 18;
 
@@ -255,6 +291,7 @@ struct ccsdt_job_s {
     PanelDesc *d_desc = nullptr;
     int npanels = 0;
     short2 *d_pairs = nullptr;
+    bool tensors = false;  // full-tensor job (ccsdt_job_create_tensors)
     Task *d_tasks = nullptr;
     Task *h_tasks = nullptr;
     int *d_order = nullptr;  // tasks sorted by k slice (pair-major execution order)
@@ -335,7 +372,7 @@ int grow(double **p, int64_t *cap, int64_t need)
     return CCSDT_OK;
 }
 
-int check_desc(const ccsdt_job_desc *d)
+int check_desc(const ccsdt_job_desc *d, bool tensors = false)
 {
     if (!d) return set_error(CCSDT_E_ARG, "null job descriptor");
     if (d->ncor < 0 || d->nocc < 1 || d->nvir < 1 || d->kchunk < 1 || d->kchunk > d->nocc)
@@ -343,8 +380,14 @@ int check_desc(const ccsdt_job_desc *d)
                          d->nocc, d->nvir, d->kchunk);
     if (d->nvir > 8192 || d->nocc > 4096)
         return set_error(CCSDT_E_ARG, "nvir=%d / nocc=%d beyond supported range", d->nvir, d->nocc);
+    // the pack kernels put one panel per grid row (at most 65535)
+    if (job_npanels(d->nocc, d->kchunk, tensors) > 65535)
+        return set_error(CCSDT_E_ARG, "nocc=%d gives %lld operand panels (at most 65535)", d->nocc,
+                         (long long)job_npanels(d->nocc, d->kchunk, tensors));
+    int64_t cnt[CCSDT_NARRAYS];
+    array_sizes(d->ncor, d->nocc, d->nvir, d->kchunk, cnt, tensors);
     for (int q = 0; q < CCSDT_NARRAYS; ++q)
-        if (!d->arrays[q]) return set_error(CCSDT_E_ARG, "array %d of the job is null", q);
+        if (cnt[q] > 0 && !d->arrays[q]) return set_error(CCSDT_E_ARG, "array %d of the job is null", q);
     return CCSDT_OK;
 }
 
@@ -380,8 +423,9 @@ int job_upload(ccsdt_job j, const double *const *arrays, const int64_t *slice_of
     } else {
         // one-time job uploads: straight from the caller's memory
         for (int q = 0; q < CCSDT_NARRAYS; ++q)
-            HIP_TRY(hipMemcpyAsync(j->d_raw + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
-                                   (size_t)j->cnt[q] * sizeof(double), hipMemcpyHostToDevice, j->st));
+            if (j->cnt[q] > 0)
+                HIP_TRY(hipMemcpyAsync(j->d_raw + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
+                                       (size_t)j->cnt[q] * sizeof(double), hipMemcpyHostToDevice, j->st));
     }
     // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2: the order the kernels read
     const double *v6[6] = {arrays[CCSDT_T1V1], arrays[CCSDT_T1V2], arrays[CCSDT_DINTX1],
@@ -411,10 +455,10 @@ int ensure_tasks(ccsdt_job j, int64_t n)
     return CCSDT_OK;
 }
 
-int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged)
+int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, bool tensors = false)
 {
     *out = nullptr;
-    int rc = check_desc(desc);
+    int rc = check_desc(desc, tensors);
     if (rc) return rc;
     rc = ensure_device(-1);
     if (rc) return rc;
@@ -422,8 +466,9 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged)
     for (int q = 0; q < kGraphKey; ++q) j->gkey[q] = -1;
     j->g = make_geom(desc->ncor, desc->nocc, desc->nvir);
     j->kchunk = desc->kchunk;
+    j->tensors = tensors;
     const Geom &g = j->g;
-    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, j->cnt);
+    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, j->cnt, tensors);
     int64_t o = 0;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) {
         j->off[q] = o;
@@ -461,7 +506,7 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged)
     JOB_HIP(hipHostMalloc((void **)&j->h_jobdev, sizeof(JobDev), 0));
     JOB_HIP(hipHostMalloc((void **)&j->h_seg, sizeof(Segment), 0));
     JOB_HIP(hipHostMalloc((void **)&j->h_hdr, sizeof(WorkHdr), 0));
-    j->npanels = PAN_SLICE0 + 4 * j->kchunk;
+    j->npanels = (int)job_npanels(g.nocc, j->kchunk, tensors);
     JOB_TRY(dev_alloc((void **)&j->d_pan, (size_t)j->npanels * g.vecsz * sizeof(double)));
 
     // panel descriptors: which raw arrays feed each packed operand
@@ -471,11 +516,28 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged)
     auto set = [&](int p, const double *P, int ft, const double *Q, int st2, double sgn) {
         ds[p] = PanelDesc{P, Q, j->d_pan + (int64_t)p * g.vecsz, ft, st2, sgn};
     };
-    set(PAN_LJ, raw(CCSDT_JIA), 0, raw(CCSDT_TIA), 0, 1.0);
-    set(PAN_LK, raw(CCSDT_KIA), 0, raw(CCSDT_XIA), 0, 1.0);
-    set(PAN_R1T, raw(CCSDT_TIJ), 0, raw(CCSDT_KIJ), 1, -1.0);
-    set(PAN_R3T, raw(CCSDT_TIJ), 1, raw(CCSDT_JIJ), 1, -1.0);
-    for (int s = 0; s < j->kchunk; ++s) {
+    if (!tensors) {
+        set(PAN_LJ, raw(CCSDT_JIA), 0, raw(CCSDT_TIA), 0, 1.0);
+        set(PAN_LK, raw(CCSDT_KIA), 0, raw(CCSDT_XIA), 0, 1.0);
+        set(PAN_R1T, raw(CCSDT_TIJ), 0, raw(CCSDT_KIJ), 1, -1.0);
+        set(PAN_R3T, raw(CCSDT_TIJ), 1, raw(CCSDT_JIJ), 1, -1.0);
+    } else {
+        // L(p) = pair p, R(p, jj) = pair nocc + jj*nocc + p: the operand roles of
+        // the harness panels, with the slice picked by occupied index
+        const int no = g.nocc;
+        for (int p = 0; p < no; ++p) {
+            set(2 * p, raw(CCSDT_JIA) + p * vv, 0, raw(CCSDT_TIA) + p * ov, 0, 1.0);
+            set(2 * p + 1, raw(CCSDT_KIA) + p * vv, 0, raw(CCSDT_XIA) + p * ov, 0, 1.0);
+        }
+        for (int jj = 0; jj < no; ++jj)
+            for (int p = 0; p < no; ++p) {
+                const int q = no + jj * no + p;
+                const int64_t sl = (int64_t)jj * no + p;
+                set(2 * q, raw(CCSDT_TIJ) + sl * vv, 0, raw(CCSDT_KIJ) + sl * ov, 1, -1.0);
+                set(2 * q + 1, raw(CCSDT_TIJ) + sl * vv, 1, raw(CCSDT_JIJ) + sl * ov, 1, -1.0);
+            }
+    }
+    for (int s = 0; s < j->kchunk && !tensors; ++s) {
         const int b = PAN_SLICE0 + 4 * s;
         set(b + SL_R1, raw(CCSDT_TKJ) + s * vv, 0, raw(CCSDT_KKJ) + s * ov, 1, -1.0);
         set(b + SL_R3, raw(CCSDT_TKJ) + s * vv, 1, raw(CCSDT_JKJ) + s * ov, 1, -1.0);
@@ -515,20 +577,31 @@ int job_tasks(ccsdt_job j, const int *trip, int64_t n)
                              "triple %lld (a=%d i=%d j=%d k=%d klo=%d) outside nbf=%d nocc=%d kchunk=%d",
                              (long long)t, a, i, jj, k, klo, nbf, g.nocc, j->kchunk);
         Task &tk = j->h_tasks[t];
-        tk.a = a - 1;
-        tk.i = i - 1;
-        tk.j = jj - 1;
-        tk.k = k - 1;
-        tk.slice = k - klo;
+        if (j->tensors) {
+            // L(p) is pair p, R(p, j) is pair nocc + j*nocc + p (0-based p, j)
+            tk.li = i - 1;
+            tk.lk = k - 1;
+            tk.rij = g.nocc + (jj - 1) * g.nocc + (i - 1);
+            tk.rkj = g.nocc + (jj - 1) * g.nocc + (k - 1);
+        } else {
+            // omp/ccsd_trpdrv_omp_cbody.c:64-70: the k operands at slice k - klo
+            tk.li = 0;
+            tk.rij = 1;
+            tk.rkj = 2 + 2 * (k - klo);
+            tk.lk = 3 + 2 * (k - klo);
+        }
         tk.ik_differ = (i != k);
+        tk.pad = 0;
         // omp/ccsd_trpdrv_omp_cbody.c:126, same operation order
         tk.eaijk = e[a - 1] - (e[g.ncor + i - 1] + e[g.ncor + jj - 1] + e[g.ncor + k - 1]);
     }
-    // pair-major execution order: the tasks sorted by k slice (stable counting sort)
-    std::vector<int64_t> start(j->kchunk + 1, 0);
-    for (int64_t t = 0; t < n; ++t) ++start[j->h_tasks[t].slice + 1];
-    for (int s = 0; s < j->kchunk; ++s) start[s + 1] += start[s];
-    for (int64_t t = 0; t < n; ++t) j->h_order[start[j->h_tasks[t].slice]++] = (int)t;
+    // pair-major execution order: the tasks sorted by their R(k,j) panel pair
+    // (the k slice of a harness-shaped job), stable counting sort
+    const int nq = j->npanels / 2;
+    std::vector<int64_t> start(nq + 1, 0);
+    for (int64_t t = 0; t < n; ++t) ++start[j->h_tasks[t].rkj + 1];
+    for (int q = 0; q < nq; ++q) start[q + 1] += start[q];
+    for (int64_t t = 0; t < n; ++t) j->h_order[start[j->h_tasks[t].rkj]++] = (int)t;
     return CCSDT_OK;
 }
 
@@ -1079,7 +1152,7 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
                     memcpy(j->h_fstage + m * vv, fs[m], (size_t)vv * sizeof(double));
                 HIP_TRY(hipMemcpyAsync(j->d_f, j->h_fstage, (size_t)8 * vv * sizeof(double),
                                        hipMemcpyHostToDevice, j->st));
-                j->h_tasks[0] = Task{0, 0, 0, 0, 0, 1, eaijk};
+                j->h_tasks[0] = Task{0, 0, 0, 0, 1, 0, eaijk};
                 HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, sizeof(Task), hipMemcpyHostToDevice,
                                        j->st));
                 HIP_TRY(launch_tengy(j->d_f, 8 * vv, j->g, j->d_tasks, 1, j->d_ev, j->d_vec6,
@@ -1124,6 +1197,7 @@ struct SJob {
     int64_t n = 0, next = 0, done = 0;
     int slot = -1;
     int64_t footprint = 0;
+    bool tensors = false;  // full-tensor job
     double t_submit = 0.0, t_admit = 0.0, t_finish = 0.0;
     int state = 0;  // 0 waiting, 1 running, 2 reducing, 3 finished, -1 failed
     int status = CCSDT_OK;
@@ -1164,12 +1238,12 @@ double server_now(ccsdt_server s)
     return std::chrono::duration<double>(Clock::now() - s->t0).count();
 }
 
-int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n)
+int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n, bool tensors)
 {
-    int64_t b = ccsdt_job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n);
+    int64_t b = job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n, tensors);
     if (o.mode == CCSDT_MODE_OZAKI_INT8) {
         const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
-        const int npanels = PAN_SLICE0 + 4 * d->kchunk;
+        const int npanels = (int)job_npanels(d->nocc, d->kchunk, tensors);
         const SplitGeom sg = make_split_geom(g, o.slices, npanels);
         b += npanels * (o.slices * sg.slice_bytes + sg.tail_sz * 8 + (int64_t)g.nvp * 8);
     }
@@ -1181,7 +1255,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
 {
     ccsdt_job_desc d = sj->desc;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q];
-    int rc = job_create(&d, &sj->job, false);
+    int rc = job_create(&d, &sj->job, false, sj->tensors);
     if (rc) return rc;
     ccsdt_job j = sj->job;
     const Geom &g = j->g;
@@ -1373,6 +1447,89 @@ int server_finish(ccsdt_server s)
 }
 }  // namespace
 
+// A full-tensor descriptor in the job descriptor's slots: the i-indexed slots
+// carry the tensors, the k-indexed slots stay empty, kchunk = nocc.
+static ccsdt_job_desc tensor_job_desc(const ccsdt_tensor_desc *t)
+{
+    ccsdt_job_desc d{};
+    d.ncor = t->ncor;
+    d.nocc = t->nocc;
+    d.nvir = t->nvir;
+    d.kchunk = t->nocc;
+    d.arrays[CCSDT_EORB] = t->eorb;
+    d.arrays[CCSDT_TIJ] = t->t2;
+    d.arrays[CCSDT_JIA] = t->jv;
+    d.arrays[CCSDT_KIA] = t->kv;
+    d.arrays[CCSDT_TIA] = t->t1o;
+    d.arrays[CCSDT_XIA] = t->xo;
+    d.arrays[CCSDT_JIJ] = t->jo;
+    d.arrays[CCSDT_KIJ] = t->ko;
+    d.arrays[CCSDT_DINTC1] = t->dintc1;
+    d.arrays[CCSDT_DINTX1] = t->dintx1;
+    d.arrays[CCSDT_T1V1] = t->t1v1;
+    d.arrays[CCSDT_DINTC2] = t->dintc2;
+    d.arrays[CCSDT_DINTX2] = t->dintx2;
+    d.arrays[CCSDT_T1V2] = t->t1v2;
+    return d;
+}
+
+// Validates and queues a job (admitting it at once when it fits and nothing
+// waits before it).
+static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples, int64_t n,
+                         int64_t *id, bool tensors)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    int rc = check_desc(desc, tensors);
+    if (rc) return rc;
+    if (n < 1 || !triples) return set_error(CCSDT_E_ARG, "empty triple list");
+    const Geom g = make_geom(desc->ncor, desc->nocc, desc->nvir);
+    if (n * g.npairs > 0x7fffffff) return set_error(CCSDT_E_ARG, "triple list too long");
+    const int nbf = desc->ncor + desc->nocc + desc->nvir;
+    for (int64_t t = 0; t < n; ++t) {
+        const int *q = triples + 5 * t;
+        if (q[0] < 1 || q[0] > nbf || q[1] < 1 || q[1] > desc->nocc || q[2] < 1 || q[2] > desc->nocc ||
+            q[3] < 1 || q[3] > desc->nocc || q[4] < 1 || q[4] > q[3] || q[3] - q[4] >= desc->kchunk)
+            return set_error(CCSDT_E_ARG, "triple %lld outside the job's shape", (long long)t);
+    }
+    if (s->o.mode == CCSDT_MODE_OZAKI_INT8) {
+        const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, tensors));
+        if ((int64_t)sg.npanels * s->o.slices * sg.slice_bytes >= (int64_t(1) << 32) ||
+            (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
+            return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    SJob *sj = new SJob();
+    sj->id = s->next_id++;
+    sj->desc = *desc;
+    sj->tensors = tensors;
+    sj->n = n;
+    sj->trip.assign(triples, triples + 5 * n);
+    sj->footprint = server_footprint(s->o, desc, n, tensors);
+    sj->t_submit = server_now(s);
+    if (sj->footprint > s->o.budget_bytes) {
+        const int64_t sj_fp = sj->footprint;
+        delete sj;
+        return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
+                         (long long)sj_fp, (long long)s->o.budget_bytes);
+    }
+    s->jobs[sj->id] = sj;
+    if (id) *id = sj->id;
+    if (s->waiting.empty() && !s->free_slots.empty() && s->used + sj->footprint <= s->o.budget_bytes) {
+        s->used += sj->footprint;
+        rc = server_admit(s, sj, desc->arrays);  // straight from the caller's arrays
+        if (rc) server_fail(s, sj, rc);
+    } else {
+        int64_t cnt[CCSDT_NARRAYS];
+        array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt, tensors);
+        sj->host.resize(CCSDT_NARRAYS);
+        for (int q = 0; q < CCSDT_NARRAYS; ++q)
+            if (cnt[q] > 0) sj->host[q].assign(desc->arrays[q], desc->arrays[q] + cnt[q]);
+        s->waiting.push_back(sj->id);
+    }
+    t_err = CCSDT_OK;
+    return CCSDT_OK;
+}
+
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
@@ -1425,6 +1582,16 @@ void ccsdt_pcg_uniform(uint64_t seed, uint64_t stream, double lo, double hi, dou
     }
 }
 
+int ccsdt_job_create_tensors(const ccsdt_tensor_desc *desc, ccsdt_job *out)
+{
+    if (!out) return set_error(CCSDT_E_ARG, "null output handle");
+    if (!desc) return set_error(CCSDT_E_ARG, "null tensor descriptor");
+    const ccsdt_job_desc d = tensor_job_desc(desc);
+    int rc = job_create(&d, out, false, true);
+    if (!rc) t_err = CCSDT_OK;
+    return rc;
+}
+
 int ccsdt_job_create(const ccsdt_job_desc *desc, ccsdt_job *out)
 {
     if (!out) return set_error(CCSDT_E_ARG, "null output handle");
@@ -1464,14 +1631,13 @@ int ccsdt_job_destroy(ccsdt_job job)
 int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples)
 {
     if (nocc < 1 || nvir < 1 || kchunk < 1 || ncor < 0) return -1;
-    const Geom g = make_geom(ncor, nocc, nvir);
-    int64_t cnt[CCSDT_NARRAYS];
-    array_sizes(ncor, nocc, nvir, kchunk, cnt);
-    int64_t raw = 0;
-    for (int q = 0; q < CCSDT_NARRAYS; ++q) raw += round_up(cnt[q], 32);
-    const int64_t pan = (int64_t)(PAN_SLICE0 + 4 * kchunk) * g.vecsz;
-    const int64_t per_task = (int64_t)g.npairs * 4 + 4 + (int64_t)sizeof(Task) / 8;
-    return (raw + pan + 7 * g.nvp + ntriples * per_task) * (int64_t)sizeof(double);
+    return job_footprint(ncor, nocc, nvir, kchunk, ntriples, false);
+}
+
+int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriples)
+{
+    if (nocc < 1 || nvir < 1 || ncor < 0) return -1;
+    return job_footprint(ncor, nocc, nvir, nocc, ntriples, true);
 }
 
 
@@ -1522,47 +1688,15 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
 int ccsdt_server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples, int64_t n,
                         int64_t *id)
 {
-    if (!s) return set_error(CCSDT_E_STATE, "null server");
-    int rc = check_desc(desc);
-    if (rc) return rc;
-    if (n < 1 || !triples) return set_error(CCSDT_E_ARG, "empty triple list");
-    const Geom g = make_geom(desc->ncor, desc->nocc, desc->nvir);
-    if (n * g.npairs > 0x7fffffff) return set_error(CCSDT_E_ARG, "triple list too long");
-    const int nbf = desc->ncor + desc->nocc + desc->nvir;
-    for (int64_t t = 0; t < n; ++t) {
-        const int *q = triples + 5 * t;
-        if (q[0] < 1 || q[0] > nbf || q[1] < 1 || q[1] > desc->nocc || q[2] < 1 || q[2] > desc->nocc ||
-            q[3] < 1 || q[3] > desc->nocc || q[4] < 1 || q[4] > q[3] || q[3] - q[4] >= desc->kchunk)
-            return set_error(CCSDT_E_ARG, "triple %lld outside the job's shape", (long long)t);
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
-    SJob *sj = new SJob();
-    sj->id = s->next_id++;
-    sj->desc = *desc;
-    sj->n = n;
-    sj->trip.assign(triples, triples + 5 * n);
-    sj->footprint = server_footprint(s->o, desc, n);
-    sj->t_submit = server_now(s);
-    if (sj->footprint > s->o.budget_bytes) {
-        delete sj;
-        return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
-                         (long long)server_footprint(s->o, desc, n), (long long)s->o.budget_bytes);
-    }
-    s->jobs[sj->id] = sj;
-    if (id) *id = sj->id;
-    if (s->waiting.empty() && !s->free_slots.empty() && s->used + sj->footprint <= s->o.budget_bytes) {
-        s->used += sj->footprint;
-        rc = server_admit(s, sj, desc->arrays);  // straight from the caller's arrays
-        if (rc) server_fail(s, sj, rc);
-    } else {
-        int64_t cnt[CCSDT_NARRAYS];
-        array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt);
-        sj->host.resize(CCSDT_NARRAYS);
-        for (int q = 0; q < CCSDT_NARRAYS; ++q) sj->host[q].assign(desc->arrays[q], desc->arrays[q] + cnt[q]);
-        s->waiting.push_back(sj->id);
-    }
-    t_err = CCSDT_OK;
-    return CCSDT_OK;
+    return server_submit(s, desc, triples, n, id, false);
+}
+
+int ccsdt_server_submit_tensors(ccsdt_server s, const ccsdt_tensor_desc *desc, const int *triples,
+                                int64_t n, int64_t *id)
+{
+    if (!desc) return set_error(CCSDT_E_ARG, "null tensor descriptor");
+    const ccsdt_job_desc d = tensor_job_desc(desc);
+    return server_submit(s, &d, triples, n, id, true);
 }
 
 int ccsdt_server_progress(ccsdt_server s, int block)

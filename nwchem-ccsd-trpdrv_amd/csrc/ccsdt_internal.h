@@ -6,12 +6,20 @@
 
 namespace ccsdt {
 
-// One (a,i,j,k) triple on the device.  Indices are 0-based; `slice` selects the
-// k-indexed operand slice (k-klo in the reference, omp/ccsd_trpdrv_omp_cbody.c:72).
+// One (a,i,j,k) triple on the device: the four operand panel pairs it reads
+// and its energy denominator shift.  Panel pair q is panels 2q and 2q+1 of the
+// job's packed buffer:
+//   li  = L(i)   = {LJ, LK}     (Jia/Tia, Kia/Xia)
+//   lk  = L(k)   = {LJt, LKt}   (Jka/Tka, Kka/Xka)
+//   rij = R(i,j) = {R1t, R3t}   (Tij with Kij, Jij)
+//   rkj = R(k,j) = {R1, R3}     (Tkj with Kkj, Jkj)
+// A harness-shaped job (omp/test.c arrays) has li = 0, rij = 1 and, for k
+// slice s = k-klo (omp/ccsd_trpdrv_omp_cbody.c:64-70), rkj = 2+2s, lk = 3+2s;
+// a full-tensor job indexes its L and R families by occupied indices.
 struct Task {
-    int a, i, j, k;
-    int slice;
+    int li, lk, rij, rkj;
     int ik_differ;  // 1 when i != k: the k terms enter the totals
+    int pad;
     double eaijk;   // eorb[a] - (eorb[ncor+i] + eorb[ncor+j] + eorb[ncor+k])
 };
 
@@ -39,9 +47,9 @@ struct PanelDesc {
     double sgn;
 };
 
-// Panel order inside a job's packed buffer.
+// Panel order inside a harness-shaped job's packed buffer (pairs 0 and 1, then
+// per k slice s the pairs 2+2s = {R1, R3} and 3+2s = {LJt, LKt}).
 enum { PAN_LJ = 0, PAN_LK = 1, PAN_R1T = 2, PAN_R3T = 3, PAN_SLICE0 = 4 };
-// Per k-slice s the four panels start at PAN_SLICE0 + 4*s.
 enum { SL_R1 = 0, SL_R3 = 1, SL_LJT = 2, SL_LKT = 3 };
 
 // Geometry of the Ozaki INT8 split form: the K = nvir half of every panel as

@@ -206,21 +206,46 @@ hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipS
 // K loop of k_fused.  Ping-pong operand buffers X and Y, two K steps per trip:
 // each buffer's loads are in flight while the other buffer's MFMAs run; an odd
 // last step runs from X after the loop.  DIAG skips the eight mirrored products.
+// Resources of a task's four panel pairs, indexed by these constants.
+enum { RP_LI = 0, RP_LK = 1, RP_RIJ = 2, RP_RKJ = 3 };
+
+// Panel pairs of one task, wave-uniform (Task fields after readfirstlane).
+struct PanelPairs {
+    int li, lk, rij, rkj;
+};
+
+__device__ __forceinline__ PanelPairs panel_pairs(const Task &tk)
+{
+    return PanelPairs{uniform(tk.li), uniform(tk.lk), uniform(tk.rij), uniform(tk.rkj)};
+}
+
+// One raw-buffer resource per panel pair (two panels each), so byte offsets fit
+// 32 bits whatever the job's panel count.
+__device__ __forceinline__ void pair_resources(__amdgpu_buffer_rsrc_t (&rp)[4], const double *pan,
+                                               int64_t vecsz, const PanelPairs &pp)
+{
+    const uint32_t bytes = (uint32_t)(2 * vecsz * 8);
+    rp[RP_LI] = buffer_rsrc(pan + 2 * (int64_t)pp.li * vecsz, bytes);
+    rp[RP_LK] = buffer_rsrc(pan + 2 * (int64_t)pp.lk * vecsz, bytes);
+    rp[RP_RIJ] = buffer_rsrc(pan + 2 * (int64_t)pp.rij * vecsz, bytes);
+    rp[RP_RKJ] = buffer_rsrc(pan + 2 * (int64_t)pp.rkj * vecsz, bytes);
+}
+
 template <bool DIAG>
-__device__ __forceinline__ void fused_mainloop(dbl4 (&acc)[16], const __amdgpu_buffer_rsrc_t rg,
-                                               const __amdgpu_buffer_rsrc_t rs,
+__device__ __forceinline__ void fused_mainloop(dbl4 (&acc)[16], const __amdgpu_buffer_rsrc_t (&rp)[4],
                                                const uint32_t (&AO)[8], const uint32_t (&BO)[8],
                                                uint32_t voff, int nks)
 {
-    // which resource each operand lives in (true = the task's k-slice panels)
-    constexpr bool AS[8] = {false, false, true, true, true, true, false, false};
-    constexpr bool BS[8] = {true, true, false, false, false, false, true, true};
+    // the panel pair each operand lives in; A side (index b): LJ, LK, LJt, LKt,
+    // R1, R3, R1t, R3t; B side (index c): R1, R3, R1t, R3t, LJ, LK, LJt, LKt
+    constexpr int AR[8] = {RP_LI, RP_LI, RP_LK, RP_LK, RP_RKJ, RP_RKJ, RP_RIJ, RP_RIJ};
+    constexpr int BR[8] = {RP_RKJ, RP_RKJ, RP_RIJ, RP_RIJ, RP_LI, RP_LI, RP_LK, RP_LK};
     constexpr int NV = DIAG ? 4 : 8;
     double xa[8], xb[8], ya[8], yb[8];
 #define STEP_LOAD(da, db, kstep)                                                              \
     _Pragma("unroll") for (int v = 0; v < NV; ++v) {                                        \
-        da[v] = buffer_load_f64(AS[v] ? rs : rg, voff, AO[v] + (uint32_t)(kstep) * 512);     \
-        db[v] = buffer_load_f64(BS[v] ? rs : rg, voff, BO[v] + (uint32_t)(kstep) * 512);     \
+        da[v] = buffer_load_f64(rp[AR[v]], voff, AO[v] + (uint32_t)(kstep) * 512);           \
+        db[v] = buffer_load_f64(rp[BR[v]], voff, BO[v] + (uint32_t)(kstep) * 512);           \
     }
 #define STEP_MFMA(da, db)                                                                     \
     {                                                                                         \
@@ -248,29 +273,26 @@ __device__ __forceinline__ void fused_mainloop(dbl4 (&acc)[16], const __amdgpu_b
 
 // One work item of k_fused: the 16x16 block pair (bi, ci) of one task.
 template <bool TENGY, bool WRITEF>
-__device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, int slice, int bi,
-                                           int ci, int lane, double *__restrict__ part4,
+__device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
+                                           int bi, int ci, int lane, double *__restrict__ part4,
                                            double *__restrict__ fo)
 {
     const Geom &g = jd.g;
-    const double *__restrict__ pan = jd.pan;
     const double *__restrict__ ev = jd.ev;
     const double *__restrict__ vec6 = jd.vec6;
-    // Two buffer resources: the job-wide panels (LJ, LK, R1t, R3t) and the four
-    // panels of the task's k slice (R1, R3, LJt, LKt).  Byte offsets fit 32 bits
-    // (checked on the host: 4 panels < 4 GiB).
+    __amdgpu_buffer_rsrc_t rp[4];
+    pair_resources(rp, jd.pan, g.vecsz, pp);
     const uint32_t vsb = (uint32_t)(g.vecsz * 8);
-    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(pan, 4 * vsb);
-    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pan + (int64_t)(PAN_SLICE0 + 4 * slice) * g.vecsz, 4 * vsb);
     const uint32_t blkb = (uint32_t)g.nks * 512;  // bytes of one 16-row block of a panel
     const uint32_t ob = bi * blkb, oc = ci * blkb;
     const uint32_t voff = lane * 8;
-    // A side, index b: LJ, LK, LJt, LKt, R1, R3, R1t, R3t
-    const uint32_t AO[8] = {PAN_LJ * vsb + ob, PAN_LK * vsb + ob, SL_LJT * vsb + ob, SL_LKT * vsb + ob,
-                            SL_R1 * vsb + ob, SL_R3 * vsb + ob, (PAN_R1T) * vsb + ob, (PAN_R3T) * vsb + ob};
-    // B side, index c: R1, R3, R1t, R3t, LJ, LK, LJt, LKt
-    const uint32_t BO[8] = {SL_R1 * vsb + oc, SL_R3 * vsb + oc, PAN_R1T * vsb + oc, PAN_R3T * vsb + oc,
-                            PAN_LJ * vsb + oc, PAN_LK * vsb + oc, SL_LJT * vsb + oc, SL_LKT * vsb + oc};
+    // operand v is the first (v even) or second panel of its pair
+    uint32_t AO[8], BO[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        AO[v] = (v & 1) * vsb + ob;
+        BO[v] = (v & 1) * vsb + oc;
+    }
     dbl4 acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -279,9 +301,9 @@ __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, int
     // mirrored values are the same products at (c,b), transposed through LDS below.
     const bool diag = (bi == ci);
     if (diag)
-        fused_mainloop<true>(acc, rg, rs, AO, BO, voff, g.nks);
+        fused_mainloop<true>(acc, rp, AO, BO, voff, g.nks);
     else
-        fused_mainloop<false>(acc, rg, rs, AO, BO, voff, g.nks);
+        fused_mainloop<false>(acc, rp, AO, BO, voff, g.nks);
     if (diag) {
         // f(c,b) at (b,c) = product q at (c,b): element (row, col) sits in lane
         // col + 16*(row%4), register row/4 (result layout below)
@@ -375,7 +397,7 @@ __global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
         decode_item(sg, jd.g.npairs, item, t, p);
         const Task tk = sg.tasks[t];
         const short2 pr = jd.pairs[p];
-        fused_item<TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), threadIdx.x,
+        fused_item<TENGY, WRITEF>(jd, tk, panel_pairs(tk), uniform(pr.x), uniform(pr.y), threadIdx.x,
                                   sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
                                   WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }
@@ -501,8 +523,12 @@ template <int S, int W>
 __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
                                            const __amdgpu_buffer_rsrc_t rt,
                                            const double *__restrict__ scales, const Geom &g,
-                                           const SplitGeom &sg, int ps, int bi, int ci, int lane)
+                                           const SplitGeom &sg, const PanelPairs &pp, int bi,
+                                           int ci, int lane)
 {
+    // panel ids: pair q holds panels 2q, 2q+1
+    const int LJ = 2 * pp.li, LK = LJ + 1, LJT = 2 * pp.lk, LKT = LJT + 1;
+    const int R1 = 2 * pp.rkj, R3 = R1 + 1, R1T = 2 * pp.rij, R3T = R1T + 1;
     const int nvp = g.nvp;
     const uint32_t plane = (uint32_t)sg.slice_bytes;
     const uint32_t blk8 = (uint32_t)sg.nkv * 1024;  // bytes of a 16-row block of one plane
@@ -513,11 +539,11 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
         // panels {A0, A1, B0, B1}: W=0 groups n, t; W=1 groups n', t'
         int A0, A1, B0, B1;
         if (W == 0) {
-            A0 = gi == 0 ? PAN_LJ : ps + SL_LJT;  A1 = gi == 0 ? PAN_LK : ps + SL_LKT;
-            B0 = gi == 0 ? ps + SL_R1 : PAN_R1T;  B1 = gi == 0 ? ps + SL_R3 : PAN_R3T;
+            A0 = gi == 0 ? LJ : LJT;  A1 = gi == 0 ? LK : LKT;
+            B0 = gi == 0 ? R1 : R1T;  B1 = gi == 0 ? R3 : R3T;
         } else {
-            A0 = gi == 0 ? ps + SL_R1 : PAN_R1T;  A1 = gi == 0 ? ps + SL_R3 : PAN_R3T;
-            B0 = gi == 0 ? PAN_LJ : ps + SL_LJT;  B1 = gi == 0 ? PAN_LK : ps + SL_LKT;
+            A0 = gi == 0 ? R1 : R1T;  A1 = gi == 0 ? R3 : R3T;
+            B0 = gi == 0 ? LJ : LJT;  B1 = gi == 0 ? LK : LKT;
         }
         int4v iacc[4][S];
 #pragma unroll
@@ -576,8 +602,8 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
     const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
     const uint32_t voffB = lane * 8;
     // W=0: A {LJ, LK, LJt, LKt} at b, B {R1, R3, R1t, R3t} at c; W=1 roles swapped
-    const int PL[4] = {PAN_LJ, PAN_LK, ps + SL_LJT, ps + SL_LKT};
-    const int PR[4] = {ps + SL_R1, ps + SL_R3, PAN_R1T, PAN_R3T};
+    const int PL[4] = {LJ, LK, LJT, LKT};
+    const int PR[4] = {R1, R3, R1T, R3T};
     uint32_t ao[4], bo[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -610,35 +636,34 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
 // 4*(i%4) + i/4 so the result layout is the one pair_epilogue expects.
 template <int W>
 __device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restrict__ pan,
-                                          const Geom &g, int ps, int bi, int ci, int lane)
+                                          const Geom &g, const PanelPairs &pp, int bi, int ci,
+                                          int lane)
 {
+    __amdgpu_buffer_rsrc_t rp[4];
+    pair_resources(rp, pan, g.vecsz, pp);
     const uint32_t vsb = (uint32_t)(g.vecsz * 8);
-    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(pan, 4 * vsb);
-    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pan + (int64_t)ps * g.vecsz, 4 * vsb);
     const uint32_t blkb = (uint32_t)g.nks * 512;
     const uint32_t ob = bi * blkb, oc = ci * blkb;
     const int li = lane & 15;
     const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
     const uint32_t voffB = lane * 8;
-    // L panels {LJ, LK, LJt, LKt} and R panels {R1, R3, R1t, R3t}: offsets inside
-    // their resource, and whether that resource is the task's k slice.
-    const uint32_t LO[4] = {PAN_LJ * vsb, PAN_LK * vsb, SL_LJT * vsb, SL_LKT * vsb};
-    const uint32_t RO[4] = {SL_R1 * vsb, SL_R3 * vsb, PAN_R1T * vsb, PAN_R3T * vsb};
-    constexpr bool LS[4] = {false, false, true, true};
-    constexpr bool RS[4] = {true, true, false, false};
+    // L panels {LJ, LK, LJt, LKt} and R panels {R1, R3, R1t, R3t}: their pair,
+    // and first (v even) or second panel of it
+    constexpr int LR[4] = {RP_LI, RP_LI, RP_LK, RP_LK};
+    constexpr int RR[4] = {RP_RKJ, RP_RKJ, RP_RIJ, RP_RIJ};
     uint32_t ao[4], bo[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        ao[v] = (W == 0 ? LO[v] : RO[v]) + ob;
-        bo[v] = (W == 0 ? RO[v] : LO[v]) + oc;
+        ao[v] = (v & 1) * vsb + ob;
+        bo[v] = (v & 1) * vsb + oc;
     }
     const int nks = g.nks;
     double xa[4], xb[4], ya[4], yb[4];
 #define F2_LOAD(da, db, kstep)                                                                   \
     _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                             \
-        const bool sa = (W == 0) ? LS[v] : RS[v], sb = (W == 0) ? RS[v] : LS[v];                  \
-        da[v] = buffer_load_f64(sa ? rs : rg, voffA, ao[v] + (uint32_t)(kstep) * 512);            \
-        db[v] = buffer_load_f64(sb ? rs : rg, voffB, bo[v] + (uint32_t)(kstep) * 512);            \
+        const int ra = (W == 0) ? LR[v] : RR[v], rb = (W == 0) ? RR[v] : LR[v];                   \
+        da[v] = buffer_load_f64(rp[ra], voffA, ao[v] + (uint32_t)(kstep) * 512);                  \
+        db[v] = buffer_load_f64(rp[rb], voffB, bo[v] + (uint32_t)(kstep) * 512);                  \
     }
 #define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
 #define F2_MFMA(da, db)                                                                          \
@@ -777,29 +802,28 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, 
 // takes the FP64 panels (k_fused2), S >= 3 the INT8 digit planes of the split
 // form (k_fused_split).
 template <int S, bool TENGY, bool WRITEF>
-__device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, int slice, int bi,
-                                          int ci, int w, int lane, double *__restrict__ part4,
-                                          double *__restrict__ fo)
+__device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
+                                          int bi, int ci, int w, int lane,
+                                          double *__restrict__ part4, double *__restrict__ fo)
 {
-    const int ps = PAN_SLICE0 + 4 * slice;
     dbl4 facc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
     const bool diag = (bi == ci);
     if constexpr (S == 0) {
         if (w == 0)
-            fp64_wave<0>(facc, jd.pan, jd.g, ps, bi, ci, lane);
+            fp64_wave<0>(facc, jd.pan, jd.g, pp, bi, ci, lane);
         else if (!diag)
-            fp64_wave<1>(facc, jd.pan, jd.g, ps, bi, ci, lane);
+            fp64_wave<1>(facc, jd.pan, jd.g, pp, bi, ci, lane);
     } else {
         const SplitGeom &sg = jd.sg;
         const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
                                                       (uint32_t)(sg.npanels * S * sg.slice_bytes));
         const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
         if (w == 0)
-            split_wave<S, 0>(facc, rd, rt, jd.scales, jd.g, sg, ps, bi, ci, lane);
+            split_wave<S, 0>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
         else if (!diag)
-            split_wave<S, 1>(facc, rd, rt, jd.scales, jd.g, sg, ps, bi, ci, lane);
+            split_wave<S, 1>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
     }
     pair_epilogue<TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo);
 }
@@ -829,7 +853,7 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
         // VGPRs through the whole item
         int lane_i = lane;
         if (PERSIST) asm volatile("" : "+v"(lane_i));
-        pair_item<S, TENGY, WRITEF>(jd, tk, uniform(tk.slice), uniform(pr.x), uniform(pr.y), w, lane_i,
+        pair_item<S, TENGY, WRITEF>(jd, tk, panel_pairs(tk), uniform(pr.x), uniform(pr.y), w, lane_i,
                                     sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
                                      WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
     }

@@ -83,6 +83,40 @@ def test_footprint(cc):
     big = cc.footprint(30, 300, 16, 4960)
     assert 0 < small < big
     assert cc.footprint(0, 40, 1, 1) == -1
+    # a full-tensor job holds nocc^2 T2 slices and 2(nocc + nocc^2) panels: at
+    # nocc=40, nvir=400 that is several GB, so a 200 GB budget admits tens of jobs
+    t = cc.tensor_footprint(40, 400, 11480)
+    assert 5e9 < t < 12e9 and t > cc.footprint(40, 400, 40, 11480)
+    assert cc.tensor_footprint(0, 400, 1) == -1
+
+
+def test_gather_triple_reads_the_tensor_slices(cc):
+    """The host gather used to check tensor jobs picks slice (j*nocc + i) of t2
+    for Tij, (j*nocc + k) for Tkj, slice i / k of jv for Jia / Jka, and so on."""
+    nocc, nvir = 3, 5
+    vv, ov = nvir * nvir, nocc * nvir
+    t = {"eorb": np.zeros(nocc + nvir)}
+    for name, n, per in (("t2", nocc * nocc, vv), ("jv", nocc, vv), ("kv", nocc, vv),
+                         ("t1o", nocc, ov), ("xo", nocc, ov), ("jo", nocc * nocc, ov),
+                         ("ko", nocc * nocc, ov)):
+        t[name] = np.repeat(np.arange(n, dtype=float), per)   # slice index as value
+    for name in cc.VECTOR_NAMES:
+        t[name] = np.ones(nvir)
+    a = cc.gather_triple(t, nocc, nvir, 3, 2, 1)   # i=3, j=2, k=1 (1-based)
+    i0, j0, k0 = 2, 1, 0
+    assert a["tij"][0] == j0 * nocc + i0 and a["tkj"][0] == j0 * nocc + k0
+    assert a["jia"][0] == i0 and a["jka"][0] == k0 and a["kka"][0] == k0
+    assert a["tia"][0] == i0 and a["xka"][0] == k0
+    assert a["jij"][0] == j0 * nocc + i0 and a["kkj"][0] == j0 * nocc + k0
+    assert a["tia"].size == ov * nocc and a["t1v1"].size == vv
+
+
+def test_tensor_job_without_device(cc):
+    if _has_gpu(cc):
+        pytest.skip("a GPU is present")
+    t = cc.make_tensor_inputs(3, 8, 1)
+    with pytest.raises(cc.CcsdtError):
+        cc.Job.from_tensors(t, 3, 8)
 
 
 def test_triple_lists(cc):
