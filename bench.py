@@ -92,18 +92,29 @@ def time_form(job, trip, steps, warmup, **kw):
             "p99_ms": float(np.percentile(step_ms, 99)), "res": res}
 
 
-def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536):
+def submit_job(srv, jb):
+    if "tensors" in jb:
+        return srv.submit_tensors(jb["tensors"], jb["nocc"], jb["nvir"], jb["triples"])
+    return srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+
+
+def job_alone(cc, jb):
+    if "tensors" in jb:
+        return cc.Job.from_tensors(jb["tensors"], jb["nocc"], jb["nvir"])
+    return cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"])
+
+
+def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536, budget=0):
     """Feeds the jobs to a server at their arrival times (seconds after start)
     and returns per-job latency and results."""
-    srv = cc.Server(mode=mode, slices=slices, items_per_tick=items_per_tick)
+    srv = cc.Server(budget_bytes=budget, mode=mode, slices=slices, items_per_tick=items_per_tick)
     ids = [None] * len(jobs)
     nxt = 0
     start = srv.clock()
     while True:
         now = srv.clock() - start
         while nxt < len(jobs) and arrivals[nxt] <= now:
-            jb = jobs[nxt]
-            ids[nxt] = srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+            ids[nxt] = submit_job(srv, jobs[nxt])
             nxt += 1
         srv.progress()
         c = srv.counts()
@@ -118,24 +129,48 @@ def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536):
     return res, lat, span
 
 
-def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True):
-    jobs = cc.serving_jobs(njobs, seed)
+def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, tensors=False,
+                   budget=0, oracle_gflop=20.0, jobs=None):
+    """One Poisson serving run.  check: every job's E[T] against the same job
+    run alone in the FP64 form on the GPU, and the jobs of at most
+    oracle_gflop GEMM work against the CPU oracle (on gathered arrays for
+    tensor jobs)."""
+    if jobs is None:
+        jobs = cc.serving_jobs(njobs, seed, tensors=tensors)
     arrivals = cc.poisson_arrivals(njobs, rate, seed)
-    res, lat, span = run_serving(cc, jobs, arrivals, mode, slices)
+    res, lat, span = run_serving(cc, jobs, arrivals, mode, slices, budget=budget)
     flops = sum(len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) for jb in jobs)
     out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": mode if mode == "fp64" else "%s s=%d" % (mode, slices),
+           "job_format": "tensors" if tensors else "harness",
            "jobs_per_s": njobs / span, "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
-           "fp64_eq_tflops_over_span": flops / span / 1e12}
+           "fp64_eq_tflops_over_span": flops / span / 1e12,
+           "footprint_gb_max": max(r["footprint"] for r in res) / 1e9,
+           "budget_gb": (budget or 200e9) / 1e9}
     if check:
-        # every job's E[T] against the same job run alone in the FP64 form
-        worst = 0.0
+        import oracle
+        worst, worst_o, nor = 0.0, 0.0, 0
         for jb, r in zip(jobs, res):
-            with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            with job_alone(cc, jb) as job:
                 a = job.run(jb["triples"], mode="fp64")
             E = a["emp4"] + a["emp5"]
             worst = max(worst, abs(r["emp4"] + r["emp5"] - E) / abs(E))
+            if len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) <= oracle_gflop * 1e9:
+                no, nv = jb["nocc"], jb["nvir"]
+                if "tensors" in jb:
+                    Eo = 0.0
+                    for q in jb["triples"]:
+                        arr = cc.gather_triple(jb["tensors"], no, nv, int(q[1]), int(q[2]), int(q[3]))
+                        t = oracle.run_list(arr, 0, no, nv, np.array([[q[0], q[1], q[2], q[3], q[3]]], np.int32))
+                        Eo += t["emp4"] + t["emp5"]
+                else:
+                    t = oracle.run_list(jb["arrays"], 0, no, nv, jb["triples"])
+                    Eo = t["emp4"] + t["emp5"]
+                worst_o = max(worst_o, abs(r["emp4"] + r["emp5"] - Eo) / abs(Eo))
+                nor += 1
         out["max_rel_dE_vs_fp64_alone"] = worst
+        out["oracle_checked_jobs"] = nor
+        out["max_rel_dE_vs_oracle"] = worst_o
     return out
 
 
@@ -241,6 +276,10 @@ def main():
     ap.add_argument("--serving", default="", metavar="RATES",
                     help="only run the serving loop at these Poisson rates (jobs/s, comma list)")
     ap.add_argument("--serving-jobs", type=int, default=24)
+    ap.add_argument("--tensors", action="store_true",
+                    help="serving jobs as full-tensor jobs (every occupied slice resident)")
+    ap.add_argument("--budget-gb", type=float, default=0.0,
+                    help="serving HBM budget in GB (0: the server's 200 GB default)")
     ap.add_argument("--batched", action="store_true",
                     help="only run the batched configuration (nocc=40, nvir=400, uneven k "
                          "blocks) with the autotuner")
@@ -259,10 +298,19 @@ def main():
     cc.init(0)
     if args.serving:
         rates = [float(x) for x in args.serving.split(",")]
-        reps = [serving_report(cc, r, args.serving_jobs, "ozaki_int8", args.slices) for r in rates]
-        reps += [serving_report(cc, rates[-1], args.serving_jobs, "fp64", 0)]
+        S = args.slices or 4
+        jobs = cc.serving_jobs(args.serving_jobs, 0xC0DE0004, tensors=args.tensors)
+        kw = dict(tensors=args.tensors, budget=int(args.budget_gb * 1e9), jobs=jobs)
+        reps = []
+        for rate, mode, sl in [(r, "ozaki_int8", S) for r in rates] + [(rates[-1], "fp64", 0)]:
+            r = serving_report(cc, rate, args.serving_jobs, mode, sl, **kw)
+            reps.append(r)
+            print("bench: serving %s %s rate %.0f: %.1f jobs/s p50 %.1f ms p99 %.1f ms" % (
+                r["job_format"], r["mode"], r["rate_jobs_per_s"], r["jobs_per_s"], r["p50_ms"],
+                r["p99_ms"]), file=sys.stderr, flush=True)
         print(json.dumps({"metric": "serving: Poisson stream of triples jobs, nocc log-U[5,40], "
-                                    "nvir log-U[50,400], one GPU", "runs": reps}), flush=True)
+                                    "nvir log-U[50,400], one GPU", "unit": "jobs/s", "runs": reps}),
+              flush=True)
         return
     if args.batched:
         print(json.dumps(batched_report(cc, args.steps, args.warmup, args.batched_slices)),

@@ -559,10 +559,10 @@ class Server:
         self.close()
 
 
-def serving_jobs(njobs, seed, nocc_range=(5, 40), nvir_range=(50, 400)):
+def serving_jobs(njobs, seed, nocc_range=(5, 40), nvir_range=(50, 400), tensors=False):
     """Independent jobs of the serving workload: nocc and nvir log-uniform in
     their ranges, inputs seeded per job as in the 8/40 case (one k block, all
-    i >= j >= k triples)."""
+    i >= j >= k triples).  tensors=True makes full-tensor jobs (key "tensors")."""
     u = pcg_uniform(seed, 90, 0.0, 1.0, 2 * njobs)
     jobs = []
     for q in range(njobs):
@@ -570,9 +570,13 @@ def serving_jobs(njobs, seed, nocc_range=(5, 40), nvir_range=(50, 400)):
                                 (np.log(nocc_range[1]) - np.log(nocc_range[0])))))
         nvir = int(round(np.exp(np.log(nvir_range[0]) + u[2 * q + 1] *
                                 (np.log(nvir_range[1]) - np.log(nvir_range[0])))))
-        arrays = make_inputs(nocc, nvir, nocc, seed + 1000 + q)
         trip = triple_list(nocc, blocks_uniform(nocc, nocc))
-        jobs.append({"nocc": nocc, "nvir": nvir, "kchunk": nocc, "arrays": arrays, "triples": trip})
+        jb = {"nocc": nocc, "nvir": nvir, "kchunk": nocc, "triples": trip}
+        if tensors:
+            jb["tensors"] = make_tensor_inputs(nocc, nvir, seed + 1000 + q)
+        else:
+            jb["arrays"] = make_inputs(nocc, nvir, nocc, seed + 1000 + q)
+        jobs.append(jb)
     return jobs
 
 
