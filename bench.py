@@ -258,6 +258,42 @@ def dropin_report(cc, arrays, trip, nocc, nvir, flops, ncalls=24):
     return out
 
 
+def split_sweep_report(cc, steps, warmup):
+    """The split-precision workload at nocc=30, nvir=300: INT8 slice counts 3..7
+    and the BF16x3 form, on the well-conditioned inputs and on the ill-conditioned
+    variant (J/K entries scaled by 10^u, u ~ U(-6,0)); per case the FP64-eq
+    TFLOP/s and E[T] against the FP64 form on the same inputs (itself checked
+    against the oracle in the tests)."""
+    nocc, nvir = 30, 300
+    out = {"metric": "split form sweep nocc=30 nvir=300: FP64-eq TFLOP/s and rel dE in E[T]",
+           "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "cases": []}
+    for ill in (False, True):
+        arrays, blocks, trip, kchunk = cc.config_case(2, illcond=ill)
+        flops = len(trip) * cc.gemm_flops_per_triple(nocc, nvir)
+        with cc.Job(arrays, nocc, nvir, kchunk) as job:
+            f = time_form(job, trip, steps, warmup, mode="fp64", order=1, persistent=True, graph=True)
+            E = f["res"]["emp4"] + f["res"]["emp5"]
+            case = {"inputs": "ill-conditioned" if ill else "well-conditioned",
+                    "fp64": {"value": flops / (f["ms_per_step"] * 1e-3) / 1e12, "E": E}, "split": []}
+            for S in range(3, 8):
+                r = time_form(job, trip, steps, warmup, mode="ozaki_int8", slices=S, order=1, graph=True)
+                e = r["res"]["emp4"] + r["res"]["emp5"]
+                case["split"].append({"slices": S, "value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
+                                      "ms_per_step": r["ms_per_step"], "rel_dE": abs(e - E) / abs(E)})
+                print("bench: split sweep %s S=%d %.1f TFLOP/s rel dE %.2e" % (
+                    case["inputs"], S, case["split"][-1]["value"], case["split"][-1]["rel_dE"]),
+                    file=sys.stderr, flush=True)
+            r = time_form(job, trip, steps, warmup, mode="bf16x3", order=1, graph=True)
+            e = r["res"]["emp4"] + r["res"]["emp5"]
+            case["bf16x3"] = {"value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
+                              "ms_per_step": r["ms_per_step"], "rel_dE": abs(e - E) / abs(E)}
+            print("bench: split sweep %s bf16x3 %.1f TFLOP/s rel dE %.2e" % (
+                case["inputs"], case["bf16x3"]["value"], case["bf16x3"]["rel_dE"]),
+                file=sys.stderr, flush=True)
+        out["cases"].append(case)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -283,6 +319,9 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="only run the batched configuration (nocc=40, nvir=400, uneven k "
                          "blocks) with the autotuner")
+    ap.add_argument("--split-sweep", action="store_true",
+                    help="only run the split form at 3..7 slices on well- and ill-conditioned "
+                         "inputs (nocc=30, nvir=300)")
     ap.add_argument("--batched-slices", type=int, default=0,
                     help="slices of the batched split form (0: chosen by the autotuner)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
@@ -311,6 +350,9 @@ def main():
         print(json.dumps({"metric": "serving: Poisson stream of triples jobs, nocc log-U[5,40], "
                                     "nvir log-U[50,400], one GPU", "unit": "jobs/s", "runs": reps}),
               flush=True)
+        return
+    if args.split_sweep:
+        print(json.dumps(split_sweep_report(cc, args.steps, args.warmup)), flush=True)
         return
     if args.batched:
         print(json.dumps(batched_report(cc, args.steps, args.warmup, args.batched_slices)),

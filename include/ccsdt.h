@@ -175,7 +175,7 @@ enum {
     CCSDT_MODE_FP64 = 0,        /* fused FP64 MFMA GEMM + energy epilogue */
     CCSDT_MODE_FP64_SPLIT = 1,  /* FP64 GEMM writes f to HBM, separate tengy pass */
     CCSDT_MODE_OZAKI_INT8 = 2,  /* Ozaki INT8 slices, INT32 accumulate, FP64 recombine */
-    CCSDT_MODE_BF16X3 = 3       /* three-term BF16 split, FP32 accumulate */
+    CCSDT_MODE_BF16X3 = 3       /* three BF16 planes, FP32 accumulate per level, FP64 recombine */
 };
 
 /* Run flags. */
@@ -293,7 +293,7 @@ typedef struct ccsdt_server_s *ccsdt_server;
 typedef struct {
     int64_t budget_bytes;    /* HBM budget for admitted jobs (0 = 200 GB) */
     int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = 65536 */
-    int mode;                /* CCSDT_MODE_FP64 or CCSDT_MODE_OZAKI_INT8 */
+    int mode;                /* CCSDT_MODE_FP64, CCSDT_MODE_OZAKI_INT8 or CCSDT_MODE_BF16X3 */
     int slices;              /* Ozaki slices (0 = 4) */
     int variant;             /* CCSDT_TILE_* of the FP64 kernel */
 } ccsdt_server_opts;
@@ -334,10 +334,12 @@ int ccsdt_server_destroy(ccsdt_server s);
  * (D[lane/16 + 4r][lane%16] on gfx950).  No
  * counterpart in the reference: checks the lane layout the kernels assume. */
 int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256);
-/* One v_mfma_i32_16x16x64_i8 (shape 0; d receives 4 ints per lane) or
- * v_mfma_i32_32x32x32_i8 (shape 1; 16 ints per lane) on raw operand registers:
- * a256 and b256 hold each lane's 16 operand bytes as four int32 words.  No
- * counterpart in the reference: checks the INT8 lane layout. */
+/* One v_mfma_i32_16x16x64_i8 (shape 0; d receives 4 ints per lane),
+ * v_mfma_i32_32x32x32_i8 (shape 1; 16 ints per lane) or
+ * v_mfma_f32_16x16x32_bf16 (shape 2; the bits of 4 floats per lane) on raw
+ * operand registers: a256 and b256 hold each lane's 16 operand bytes as four
+ * int32 words.  No counterpart in the reference: checks the INT8 and BF16 lane
+ * layouts. */
 int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d, int shape);
 /* Streams `bytes` through a plain device copy kernel `reps` times and returns the
  * average milliseconds per copy (or -1): the reference point for HBM counters.

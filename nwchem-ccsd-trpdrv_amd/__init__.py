@@ -636,7 +636,7 @@ def selftest_mfma_i8(a_bytes, b_bytes, shape):
     """a_bytes, b_bytes: (64, 16) int8 per-lane operand bytes."""
     a = np.ascontiguousarray(a_bytes, dtype=np.int8).view(np.int32).ravel()
     b = np.ascontiguousarray(b_bytes, dtype=np.int8).view(np.int32).ravel()
-    d = np.empty(64 * (16 if shape else 4), dtype=np.int32)
+    d = np.empty(64 * (16 if shape == 1 else 4), dtype=np.int32)
     _check(lib().ccsdt_selftest_mfma_i8(_ptr(a), _ptr(b), _ptr(d), shape))
     return d.reshape(64, -1)
 

@@ -193,15 +193,17 @@ Geom make_geom(int ncor, int nocc, int nvir)
     return g;
 }
 
-SplitGeom make_split_geom(const Geom &g, int slices, int npanels)
+SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = false)
 {
     SplitGeom sg;
-    sg.slices = slices;
-    sg.kv = (int)round_up(g.nvir, 64);
-    sg.nkv = sg.kv / 64;
+    sg.slices = bf16 ? 3 : slices;
+    sg.bf16 = bf16 ? 1 : 0;
+    const int kstep = bf16 ? 32 : 64;  // K of one MFMA
+    sg.kv = (int)round_up(g.nvir, kstep);
+    sg.nkv = sg.kv / kstep;
     sg.kt = (int)round_up(g.nocc, 8);
     sg.nkt = sg.kt / 4;
-    sg.slice_bytes = (int64_t)g.nvp * sg.kv;
+    sg.slice_bytes = (int64_t)g.nvp * sg.kv * (bf16 ? 2 : 1);
     sg.tail_sz = (int64_t)g.nvp * sg.kt;
     sg.npanels = npanels;
     return sg;
@@ -618,7 +620,8 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
 // Digit planes, scales and FP64 tail panels of the split form for sg.slices.
 int ensure_split(ccsdt_job j, const SplitGeom &sg)
 {
-    if (j->split_slices == sg.slices) return CCSDT_OK;
+    const int key = sg.slices + (sg.bf16 ? kSlicesBf16x3 : 0);  // INT8 S or BF16x3
+    if (j->split_slices == key) return CCSDT_OK;
     if (j->d_digits) dev_free(j->d_digits);
     if (j->d_scales) dev_free(j->d_scales);
     if (j->d_tail) dev_free(j->d_tail);
@@ -630,9 +633,15 @@ int ensure_split(ccsdt_job j, const SplitGeom &sg)
     if ((rc = dev_alloc((void **)&j->d_digits, (size_t)j->npanels * sg.slices * sg.slice_bytes))) return rc;
     if ((rc = dev_alloc((void **)&j->d_scales, (size_t)j->npanels * j->g.nvp * sizeof(double)))) return rc;
     if ((rc = dev_alloc((void **)&j->d_tail, (size_t)j->npanels * sg.tail_sz * sizeof(double)))) return rc;
-    j->split_slices = sg.slices;
+    j->split_slices = key;
     return CCSDT_OK;
 }
+
+// The two split forms: Ozaki INT8 digits and BF16x3 planes.
+bool is_split_mode(int mode) { return mode == CCSDT_MODE_OZAKI_INT8 || mode == CCSDT_MODE_BF16X3; }
+
+// `slices` argument of the split launchers: the INT8 slice count, or the BF16x3 code.
+int slice_code(const SplitGeom &sg) { return sg.bf16 ? kSlicesBf16x3 : sg.slices; }
 
 // Persistent grid of a fused kernel: wg_per_cu workgroups per CU (0 or more
 // than fit: every workgroup slot of the device), in a multiple of 8 (one share
@@ -685,7 +694,7 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
 {
     const Geom &g = j->g;
     hipEvent_t *E = j->ev;
-    const bool split = (mode == CCSDT_MODE_OZAKI_INT8);
+    const bool split = is_split_mode(mode);
     HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice, j->st));
     if (j->h_seg->order)
         HIP_TRY(hipMemcpyAsync(j->d_order, j->h_order, (size_t)n * sizeof(int), hipMemcpyHostToDevice, j->st));
@@ -699,12 +708,12 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
     if (timed) HIP_TRY(hipEventRecord(E[1], j->st));
     const int64_t items = n * g.npairs;
-    const int grid = persistent ? persistent_grid(split, sg.slices, variant, wg_per_cu) : (int)items;
+    const int grid = persistent ? persistent_grid(split, slice_code(sg), variant, wg_per_cu) : (int)items;
     const int64_t vv = (int64_t)g.nvir * g.nvir;
     const bool persist = persistent && !with_f;
     if (split)
         HIP_TRY(launch_fused_split(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
-                                   sg.slices, 8 * vv, true, with_f, j->st));
+                                   slice_code(sg), 8 * vv, true, with_f, j->st));
     else
         HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
                              variant, 8 * vv, true, with_f, j->st));
@@ -731,14 +740,14 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     if (fouts && n != 1) return set_error(CCSDT_E_ARG, "f output needs a single triple");
     const int mode = opts ? opts->mode : CCSDT_MODE_FP64;
     const int flags = opts ? opts->flags : 0;
-    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT && mode != CCSDT_MODE_OZAKI_INT8)
+    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT && !is_split_mode(mode))
         return set_error(CCSDT_E_ARG, "mode %d not available in this build", mode);
     if (flags & ~(CCSDT_RUN_PERSISTENT | CCSDT_RUN_GRAPH))
         return set_error(CCSDT_E_ARG, "unknown run flags 0x%x", flags);
     const int slices = (mode == CCSDT_MODE_OZAKI_INT8) ? (opts && opts->slices ? opts->slices : 4) : 0;
     if (mode == CCSDT_MODE_OZAKI_INT8 && (slices < 3 || slices > 7))
         return set_error(CCSDT_E_ARG, "Ozaki slice count %d outside 3..7", slices);
-    const int variant = (mode == CCSDT_MODE_OZAKI_INT8 || !opts) ? 0 : opts->variant;
+    const int variant = (is_split_mode(mode) || !opts) ? 0 : opts->variant;
     if (variant != CCSDT_TILE_ONE_WAVE && variant != CCSDT_TILE_TWO_WAVE)
         return set_error(CCSDT_E_ARG, "unknown tile variant %d", variant);
     const int wg_per_cu = opts ? opts->wg_per_cu : 0;
@@ -758,9 +767,9 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const bool fused = (mode != CCSDT_MODE_FP64_SPLIT);
     const int64_t npart = (int64_t)n * (fused ? g.npairs : tnp) * 4;
     SplitGeom sg{};
-    if (mode == CCSDT_MODE_OZAKI_INT8) {
-        sg = make_split_geom(g, slices, j->npanels);
-        const int64_t dig = (int64_t)j->npanels * slices * sg.slice_bytes;
+    if (is_split_mode(mode)) {
+        sg = make_split_geom(g, slices, j->npanels, mode == CCSDT_MODE_BF16X3);
+        const int64_t dig = (int64_t)j->npanels * sg.slices * sg.slice_bytes;
         const int64_t tb = (int64_t)j->npanels * sg.tail_sz * 8;
         if (dig >= (int64_t(1) << 32) || tb >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands exceed 4 GiB (nvir=%d kchunk=%d slices=%d)",
@@ -922,10 +931,10 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     int rc = ensure_device(-1);
     if (rc) return rc;
     ccsdt_run_opts b0 = base ? *base : ccsdt_run_opts{};
-    const bool split = (b0.mode == CCSDT_MODE_OZAKI_INT8);
+    const bool split = is_split_mode(b0.mode);
     const bool fused = (b0.mode != CCSDT_MODE_FP64_SPLIT);
     std::vector<ccsdt_run_opts> cand;
-    if (split && b0.slices == 0) {
+    if (b0.mode == CCSDT_MODE_OZAKI_INT8 && b0.slices == 0) {
         // slice count: the smallest S whose E[T] on this list is within
         // kSliceTol of the FP64 form's
         ccsdt_run_opts f = b0;
@@ -946,7 +955,8 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
         }
         if (b0.slices == 0) b0.slices = 7;
     }
-    const int slices = split ? (b0.slices ? b0.slices : 4) : 0;
+    const int slices = b0.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
+                       : split ? (b0.slices ? b0.slices : 4) : 0;
     for (int ord = 0; ord < (fused ? 2 : 1); ++ord) {
         for (int v = 0; v < (split ? 1 : 2); ++v) {
             ccsdt_run_opts o = b0;
@@ -1241,11 +1251,11 @@ double server_now(ccsdt_server s)
 int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n, bool tensors)
 {
     int64_t b = job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n, tensors);
-    if (o.mode == CCSDT_MODE_OZAKI_INT8) {
+    if (is_split_mode(o.mode)) {
         const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
         const int npanels = (int)job_npanels(d->nocc, d->kchunk, tensors);
-        const SplitGeom sg = make_split_geom(g, o.slices, npanels);
-        b += npanels * (o.slices * sg.slice_bytes + sg.tail_sz * 8 + (int64_t)g.nvp * 8);
+        const SplitGeom sg = make_split_geom(g, o.slices, npanels, o.mode == CCSDT_MODE_BF16X3);
+        b += npanels * (sg.slices * sg.slice_bytes + sg.tail_sz * 8 + (int64_t)g.nvp * 8);
     }
     return b;
 }
@@ -1263,12 +1273,12 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
     if ((rc = grow(&j->d_partial, &j->cap_partial, sj->n * g.npairs * 4))) return rc;
     if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * sj->n))) return rc;
     SplitGeom sg{};
-    if (s->o.mode == CCSDT_MODE_OZAKI_INT8) {
-        sg = make_split_geom(g, s->o.slices, j->npanels);
+    if (is_split_mode(s->o.mode)) {
+        sg = make_split_geom(g, s->o.slices, j->npanels, s->o.mode == CCSDT_MODE_BF16X3);
         if ((rc = ensure_split(j, sg))) return rc;
     }
     HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)sj->n * sizeof(Task), hipMemcpyHostToDevice, j->st));
-    if (s->o.mode == CCSDT_MODE_OZAKI_INT8)
+    if (is_split_mode(s->o.mode))
         HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
     else
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
@@ -1372,8 +1382,9 @@ int server_launch_tick(ccsdt_server s, bool *launched)
     T.h_hdr->pad = 0;
     HIP_TRY(hipMemcpyAsync(T.d_seg, T.h_seg, (size_t)nseg * sizeof(Segment), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipMemcpyAsync(T.d_hdr, T.h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, s->st));
-    if (s->o.mode == CCSDT_MODE_OZAKI_INT8)
-        HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.slices, 0,
+    if (is_split_mode(s->o.mode))
+        HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false,
+                                   s->o.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3 : s->o.slices, 0,
                                    true, false, s->st));
     else
         HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.variant, 0, true,
@@ -1491,9 +1502,10 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
             q[3] < 1 || q[3] > desc->nocc || q[4] < 1 || q[4] > q[3] || q[3] - q[4] >= desc->kchunk)
             return set_error(CCSDT_E_ARG, "triple %lld outside the job's shape", (long long)t);
     }
-    if (s->o.mode == CCSDT_MODE_OZAKI_INT8) {
-        const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, tensors));
-        if ((int64_t)sg.npanels * s->o.slices * sg.slice_bytes >= (int64_t(1) << 32) ||
+    if (is_split_mode(s->o.mode)) {
+        const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, tensors),
+                                             s->o.mode == CCSDT_MODE_BF16X3);
+        if ((int64_t)sg.npanels * sg.slices * sg.slice_bytes >= (int64_t(1) << 32) ||
             (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
     }
@@ -1648,11 +1660,15 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
     ccsdt_server_opts o = opts ? *opts : ccsdt_server_opts{};
     if (o.budget_bytes <= 0) o.budget_bytes = int64_t(200) * 1000 * 1000 * 1000;
     if (o.items_per_tick <= 0) o.items_per_tick = 65536;
-    if (o.mode != CCSDT_MODE_FP64 && o.mode != CCSDT_MODE_OZAKI_INT8)
-        return set_error(CCSDT_E_ARG, "server mode %d must be FP64 or OZAKI_INT8", o.mode);
+    if (o.mode != CCSDT_MODE_FP64 && !is_split_mode(o.mode))
+        return set_error(CCSDT_E_ARG, "server mode %d must be FP64, OZAKI_INT8 or BF16X3", o.mode);
     if (o.mode == CCSDT_MODE_OZAKI_INT8) {
         if (o.slices == 0) o.slices = 4;
         if (o.slices < 3 || o.slices > 7) return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
+        o.variant = 0;
+    }
+    if (o.mode == CCSDT_MODE_BF16X3) {
+        o.slices = 3;
         o.variant = 0;
     }
     if (o.variant != CCSDT_TILE_ONE_WAVE && o.variant != CCSDT_TILE_TWO_WAVE)
@@ -1799,13 +1815,13 @@ int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d,
 {
     int rc = ensure_device(-1);
     if (rc) return rc;
-    if (shape != 0 && shape != 1) return set_error(CCSDT_E_ARG, "shape must be 0 or 1");
+    if (shape < 0 || shape > 2) return set_error(CCSDT_E_ARG, "shape must be 0, 1 or 2");
     int *buf = nullptr;
     if ((rc = dev_alloc((void **)&buf, (256 + 256 + 1024) * sizeof(int)))) return rc;
     HIP_TRY(hipMemcpy(buf, a256, 256 * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(buf + 256, b256, 256 * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(launch_mfma_i8_probe(buf, buf + 256, buf + 512, shape, nullptr));
-    HIP_TRY(hipMemcpy(d, buf + 512, (shape ? 1024 : 256) * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(d, buf + 512, (shape == 1 ? 1024 : 256) * sizeof(int), hipMemcpyDeviceToHost));
     dev_free(buf);
     return CCSDT_OK;
 }

@@ -58,9 +58,10 @@ enum { SL_R1 = 0, SL_R3 = 1, SL_LJT = 2, SL_LKT = 3 };
 // panel (kt = nocc rounded up to 8).
 struct SplitGeom {
     int slices;
-    int kv, nkv;          // kv = round_up(nvir, 64), nkv = kv / 64
+    int bf16;             // 1: three BF16 planes (v_mfma_f32_16x16x32_bf16), not INT8 digits
+    int kv, nkv;          // kv = round_up(nvir, 64), nkv = kv / 64 (BF16: 32)
     int kt, nkt;          // kt = round_up(nocc, 8), nkt = kt / 4 (even)
-    int64_t slice_bytes;  // bytes of one digit plane of one panel: nvp * kv
+    int64_t slice_bytes;  // bytes of one digit plane of one panel: nvp * kv (BF16: 2 nvp kv)
     int64_t tail_sz;      // doubles of one FP64 tail panel: nvp * kt
     int npanels;          // panels of the job (bounds of the buffer resources)
 };
@@ -132,6 +133,8 @@ hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *
 // Per-task sums of the pair partials over all segments (grid may be persistent).
 hipError_t launch_reduce_segments(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
                                   int grid, hipStream_t st);
+// `slices` code of launch_fused_split / fused_blocks_per_cu for the BF16x3 form.
+constexpr int kSlicesBf16x3 = 100;
 // Resident workgroups per CU of the fused kernels (for persistent grids).
 int fused_blocks_per_cu(bool split, int slices, int variant);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,

@@ -486,11 +486,71 @@ __global__ __launch_bounds__(64) void k_split_pack(const PanelDesc *__restrict__
     }
 }
 
+// BF16x3 form: the row is scaled by 2^-e(x) so its largest entry is below 1,
+// then written as three BF16 planes h1 + h2 + h3 (h1 = RN(v), h2 = RN(v - h1),
+// h3 = RN(v - h1 - h2); the residual is below 2^-24 |v|).  Plane layout per
+// 16-row block xb and 32-wide K step ks: lane l holds 8 BF16 values (16 bytes)
+// for x = 16*xb + l%16, k = 32*ks + 8*(l/16) + j, the operand layout of
+// v_mfma_f32_16x16x32_bf16.  The FP64 tail is as in k_split_pack.
+__device__ __forceinline__ double bf16_rn(double v, uint16_t &bits)
+{
+    // to FP32 by round-to-nearest, then to BF16 by round-to-nearest-even (the
+    // residual of the double rounding lands in the next plane)
+    const uint32_t u = __builtin_bit_cast(uint32_t, (float)v);
+    const uint32_t r = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    bits = (uint16_t)(r >> 16);
+    return (double)__builtin_bit_cast(float, r);
+}
+
+__global__ __launch_bounds__(64) void k_split_pack_bf16(const PanelDesc *__restrict__ descs, Geom g,
+                                                       SplitGeom sg, int8_t *__restrict__ planes,
+                                                       double *__restrict__ scales,
+                                                       double *__restrict__ tail)
+{
+    const int pnl = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
+    const PanelDesc d = descs[pnl];
+    const int nv = g.nvir, no = g.nocc;
+    auto val = [&](int k) -> double {
+        if (x >= nv || k >= nv) return 0.0;
+        return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
+    };
+    double m = 0.0;
+    for (int k = lane; k < nv; k += 64) m = fmax(m, fabs(val(k)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    int e = 0;
+    if (m > 0.0) (void)frexp(m, &e);  // m * 2^-e in [0.5, 1)
+    const double inv = ldexp(1.0, -e);
+    if (lane == 0) scales[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
+    const int xb = x >> 4, xi = x & 15;
+    uint16_t *pl = reinterpret_cast<uint16_t *>(planes);
+    for (int k = lane; k < sg.kv; k += 64) {
+        double v = val(k) * inv;
+        const int64_t o = ((int64_t)(xb * sg.nkv + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + xi) * 8 + (k & 7);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            uint16_t b;
+            v -= bf16_rn(v, b);
+            pl[((int64_t)pnl * 3 + q) * (sg.slice_bytes / 2) + o] = b;
+        }
+    }
+    for (int kk = lane; kk < sg.kt; kk += 64) {
+        double t = 0.0;
+        if (x < nv && kk < no)
+            t = d.sgn * (d.second_t ? d.Q[kk + (int64_t)x * no] : d.Q[x + (int64_t)kk * nv]);
+        tail[(int64_t)pnl * sg.tail_sz + ((int64_t)(xb * sg.nkt + (kk >> 2)) * 64 + xi + 16 * (kk & 3))] = t;
+    }
+}
+
 hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
                              const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
                              hipStream_t st)
 {
     dim3 grid(g.nvp, npanels), block(64);
+    if (sg.bf16) {
+        hipLaunchKernelGGL(k_split_pack_bf16, grid, block, 0, st, d_desc, g, sg, digits, scales, tail);
+        return hipGetLastError();
+    }
     switch (sg.slices) {
     case 3: hipLaunchKernelGGL(k_split_pack<3>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
     case 4: hipLaunchKernelGGL(k_split_pack<4>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
@@ -519,7 +579,39 @@ __device__ __forceinline__ int4v buffer_load_i8x16(__amdgpu_buffer_rsrc_t r, uin
 // through LDS and each wave finishes two of the four rows per lane.
 // The eight products of one wave role (W = 0: (b,c); W = 1: mirrored (c,b)),
 // panel ids resolved at compile time so every operand offset stays scalar.
-template <int S, int W>
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Digit-plane arithmetic of the two split forms: INT8 digits on
+// v_mfma_i32_16x16x64_i8 with INT32 accumulators, or BF16 planes on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulators.  Both take 16 bytes per lane
+// per operand and leave a 16x16 tile in the layout row 4*(l/16)+r, column l%16.
+template <bool BF>
+struct PlaneMath;
+
+template <>
+struct PlaneMath<false> {
+    typedef int4v acc_t;
+    static __device__ __forceinline__ acc_t mfma(int4v a, int4v b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+    // weight of accumulation level L (= plane p + plane q, 0-based): 2^(-7(L+2))
+    static __device__ __forceinline__ double weight(int L) { return __builtin_ldexp(1.0, -7 * (L + 2)); }
+};
+
+template <>
+struct PlaneMath<true> {
+    typedef flt4 acc_t;
+    static __device__ __forceinline__ acc_t mfma(int4v a, int4v b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ double weight(int) { return 1.0; }
+};
+
+template <int S, int W, bool BF = false>
 __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
                                            const __amdgpu_buffer_rsrc_t rt,
                                            const double *__restrict__ scales, const Geom &g,
@@ -545,11 +637,12 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             A0 = gi == 0 ? R1 : R1T;  A1 = gi == 0 ? R3 : R3T;
             B0 = gi == 0 ? LJ : LJT;  B1 = gi == 0 ? LK : LKT;
         }
-        int4v iacc[4][S];
+        typedef typename PlaneMath<BF>::acc_t acc_t;
+        acc_t iacc[4][S];
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int q = 0; q < S; ++q) iacc[c][q] = int4v{0, 0, 0, 0};
+            for (int q = 0; q < S; ++q) iacc[c][q] = acc_t{0, 0, 0, 0};
         const uint32_t oa0 = (uint32_t)A0 * S * plane + bi * blk8;
         const uint32_t oa1 = (uint32_t)A1 * S * plane + bi * blk8;
         const uint32_t ob0 = (uint32_t)B0 * S * plane + ci * blk8;
@@ -569,15 +662,16 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
 #pragma unroll
                 for (int sb = 0; sb < S - sa; ++sb) {
                     const int lv = sa + sb;
-                    iacc[0][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b0[sb], iacc[0][lv], 0, 0, 0);
-                    iacc[1][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[sa], b0[sb], iacc[1][lv], 0, 0, 0);
-                    iacc[2][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b1[sb], iacc[2][lv], 0, 0, 0);
-                    iacc[3][lv] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[sa], b1[sb], iacc[3][lv], 0, 0, 0);
+                    iacc[0][lv] = PlaneMath<BF>::mfma(a0[sa], b0[sb], iacc[0][lv]);
+                    iacc[1][lv] = PlaneMath<BF>::mfma(a1[sa], b0[sb], iacc[1][lv]);
+                    iacc[2][lv] = PlaneMath<BF>::mfma(a0[sa], b1[sb], iacc[2][lv]);
+                    iacc[3][lv] = PlaneMath<BF>::mfma(a1[sa], b1[sb], iacc[3][lv]);
                 }
             }
         }
-        // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * 2^(-7(L+2)); the
-        // combination (a, b) is product 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1)
+        // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * w(L) (INT8 w(L) =
+        // 2^(-7(L+2)), BF16 w = 1); the combination (a, b) is product
+        // 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int ca = c & 1, cb = c >> 1;
@@ -588,7 +682,7 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             for (int r = 0; r < 4; ++r) {
                 double v = 0.0;
 #pragma unroll
-                for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * __builtin_ldexp(1.0, -7 * (q + 2));
+                for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * PlaneMath<BF>::weight(q);
                 const double sA = scales[(int64_t)pa * nvp + bi * 16 + row0 + r];
                 facc[lp][r] += v * sA * sB;
             }
@@ -816,14 +910,17 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
         else if (!diag)
             fp64_wave<1>(facc, jd.pan, jd.g, pp, bi, ci, lane);
     } else {
+        // INT8 digits (S planes) or the BF16x3 planes
+        constexpr bool BF = (S == kSlicesBf16x3);
+        constexpr int NP = BF ? 3 : S;
         const SplitGeom &sg = jd.sg;
         const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
-                                                      (uint32_t)(sg.npanels * S * sg.slice_bytes));
+                                                      (uint32_t)(sg.npanels * NP * sg.slice_bytes));
         const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
         if (w == 0)
-            split_wave<S, 0>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+            split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
         else if (!diag)
-            split_wave<S, 1>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+            split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
     }
     pair_epilogue<TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo);
 }
@@ -866,6 +963,15 @@ __global__ __launch_bounds__(128) void k_fused_split(const JobDev *__restrict__ 
                                                      WorkHdr *hdr, int64_t fstride)
 {
     pair_items<S, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
+}
+
+// The BF16x3 split form (three BF16 planes, FP32 accumulation, FP64 recombination).
+template <bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(128) void k_fused_bf16x3(const JobDev *__restrict__ jobs,
+                                                      const Segment *__restrict__ segs,
+                                                      WorkHdr *hdr, int64_t fstride)
+{
+    pair_items<kSlicesBf16x3, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
 }
 
 // The FP64 form's work item on two waves (the alternative tile of k_fused).
@@ -923,6 +1029,22 @@ hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *
                               bool writef, hipStream_t st)
 {
     if (grid <= 0) return hipErrorInvalidValue;
+    if (slices == kSlicesBf16x3) {
+        dim3 g3((unsigned)grid), block(128);
+        if (persist && tengy && !writef)
+            hipLaunchKernelGGL((k_fused_bf16x3<true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else if (persist)
+            return hipErrorInvalidValue;
+        else if (tengy && writef)
+            hipLaunchKernelGGL((k_fused_bf16x3<true, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else if (tengy)
+            hipLaunchKernelGGL((k_fused_bf16x3<true, false, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else if (writef)
+            hipLaunchKernelGGL((k_fused_bf16x3<false, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     switch (slices) {
     case 3: return launch_split_s<3>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
     case 4: return launch_split_s<4>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
@@ -944,6 +1066,7 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<true, false, true>, 64, 0);
     } else {
         switch (slices) {
+        case kSlicesBf16x3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_bf16x3<true, false, true>, 128, 0); break;
         case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<3, true, false, true>, 128, 0); break;
         case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<4, true, false, true>, 128, 0); break;
         case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<5, true, false, true>, 128, 0); break;
@@ -1240,6 +1363,12 @@ __global__ __launch_bounds__(64) void k_mfma_i8_probe(const int *a, const int *b
         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
+    } else if (shape == 2) {
+        // the 16 bytes per lane as 8 BF16 values; FP32 results returned as bits
+        const flt4 acc = PlaneMath<true>::mfma(av, bv, flt4{0.f, 0.f, 0.f, 0.f});
+        float *df = reinterpret_cast<float *>(d);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) df[l * 4 + r] = acc[r];
     } else {
         int16v acc = {};
         acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc, 0, 0, 0);

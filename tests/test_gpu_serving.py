@@ -18,7 +18,7 @@ def small_jobs(cc, n=5):
     return cc.serving_jobs(n, 4242, nocc_range=(3, 9), nvir_range=(20, 70))
 
 
-@pytest.mark.parametrize("mode,slices", [("fp64", 0), ("ozaki_int8", 4)])
+@pytest.mark.parametrize("mode,slices", [("fp64", 0), ("ozaki_int8", 4), ("bf16x3", 0)])
 def test_alone_equals_mixed(cc, orc, mode, slices):
     jobs = small_jobs(cc)
     alone = []

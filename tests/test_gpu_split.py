@@ -1,8 +1,10 @@
-"""Ozaki INT8 split form against the FP64 oracle (needs a GPU).
+"""Split forms against the FP64 oracle (needs a GPU): Ozaki INT8 digits and
+the BF16x3 planes.
 
 Tolerance: E[T] = emp4 + emp5 over the triple list within 1e-8 relative of the
-oracle for slices >= 4 (the form's stated bound); every per-triple term within
-1e-6 of its absolute-summand scale; more slices must converge toward FP64.
+oracle for INT8 slices >= 4 and for BF16x3 (the forms' stated bound); every
+per-triple term within 1e-6 of its absolute-summand scale; more INT8 slices
+must converge toward FP64.
 """
 import os
 
@@ -26,6 +28,11 @@ def et(e4, e5):
 def run(cc, arrays, nocc, nvir, kchunk, trip, slices, **kw):
     with cc.Job(arrays, nocc, nvir, kchunk) as job:
         return job.run(trip, mode="ozaki_int8", slices=slices, **kw)
+
+
+def run_mode(cc, arrays, nocc, nvir, kchunk, trip, mode, **kw):
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        return job.run(trip, mode=mode, **kw)
 
 
 def test_config0_slices_converge(cc, orc):
@@ -103,3 +110,68 @@ def test_bad_slice_count(cc):
     with cc.Job(arrays, 8, 40, kchunk) as job:
         with pytest.raises(cc.CcsdtError):
             job.run(trip, mode="ozaki_int8", slices=9)
+
+
+def test_int8_and_bf16_mfma_layouts(cc):
+    """The operand and result lane layouts the split kernels assume: lane l holds
+    16 bytes of row l%16 (A) / column l%16 (B) at K offset (bytes per lane)*(l/16);
+    result element r of lane l is D[4*(l/16) + r][l%16]."""
+    rng = np.random.default_rng(3)
+    # INT8 16x16x64
+    A = rng.integers(-100, 101, (16, 64)).astype(np.int8)
+    B = rng.integers(-100, 101, (64, 16)).astype(np.int8)
+    a = np.array([A[l % 16, 16 * (l // 16):16 * (l // 16) + 16] for l in range(64)])
+    b = np.array([B[16 * (l // 16):16 * (l // 16) + 16, l % 16] for l in range(64)])
+    d = cc.selftest_mfma_i8(a, b, 0)
+    ref = A.astype(np.int64) @ B.astype(np.int64)
+    got = np.array([[d[16 * (r // 4) + c, r % 4] for c in range(16)] for r in range(16)])
+    assert np.array_equal(got, ref)
+    # BF16 16x16x32 (small integers are exact in BF16)
+    A = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    B = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+
+    def bits(x):
+        return (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    a = np.array([bits(A[l % 16, 8 * (l // 16):8 * (l // 16) + 8]) for l in range(64)])
+    b = np.array([bits(B[8 * (l // 16):8 * (l // 16) + 8, l % 16]) for l in range(64)])
+    d = cc.selftest_mfma_i8(a.view(np.int8).reshape(64, 16), b.view(np.int8).reshape(64, 16), 2)
+    D = d.view(np.float32)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    got = np.array([[D[16 * (r // 4) + c, r % 4] for c in range(16)] for r in range(16)])
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("illcond", [False, True])
+def test_bf16x3_config0(cc, orc, illcond):
+    arrays, blocks, trip, kchunk = cc.config_case(0, illcond=illcond)
+    ref = orc.run_list(arrays, 0, 8, 40, trip, with_abs=True)
+    got = run_mode(cc, arrays, 8, 40, kchunk, trip, "bf16x3", terms=True)
+    E = et(ref["emp4"], ref["emp5"])
+    assert abs(et(got["emp4"], got["emp5"]) - E) / abs(E) < ET_TOL
+    assert np.max(np.abs(got["terms"] - ref["terms"]) / ref["abs4"]) < 1e-6
+
+
+@pytest.mark.parametrize("nocc,nvir,kchunk", [(1, 1, 1), (3, 17, 2), (5, 70, 5), (4, 33, 2)])
+def test_bf16x3_ragged_shapes(cc, orc, nocc, nvir, kchunk):
+    """nvir not a multiple of the BF16 K step (32) nor of 16."""
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 400 + nvir)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    ref = orc.run_list(arrays, 0, nocc, nvir, trip, with_abs=True)
+    got = run_mode(cc, arrays, nocc, nvir, kchunk, trip, "bf16x3", terms=True)
+    scale = np.maximum(np.maximum(np.abs(ref["terms"]), ref["abs4"]), 1e-300)
+    assert np.max(np.abs(got["terms"] - ref["terms"]) / scale) < 1e-6
+    E = et(ref["emp4"], ref["emp5"])
+    assert abs(et(got["emp4"], got["emp5"]) - E) / abs(E) < ET_TOL
+
+
+def test_bf16x3_full_list_and_schedules(cc):
+    """All 4960 triples at 30/300: BF16x3 E[T] within 1e-8 of the FP64 form's;
+    pair-major / persistent / graph runs bitwise equal to the plain run."""
+    arrays, blocks, trip, kchunk = cc.config_case(1)
+    with cc.Job(arrays, 30, 300, kchunk) as job:
+        a = job.run(trip, mode="fp64")
+        b = job.run(trip, mode="bf16x3", terms=True)
+        c = job.run(trip, mode="bf16x3", terms=True, order=1, persistent=True, graph=True)
+    E = et(a["emp4"], a["emp5"])
+    assert abs(et(b["emp4"], b["emp5"]) - E) / abs(E) < ET_TOL
+    assert np.array_equal(b["terms"], c["terms"])
