@@ -259,8 +259,9 @@ def dropin_report(cc, arrays, trip, nocc, nvir, flops, ncalls=24):
 
 
 def split_sweep_report(cc, steps, warmup):
-    """The split-precision workload at nocc=30, nvir=300: INT8 slice counts 3..7
-    and the BF16x3 form, on the well-conditioned inputs and on the ill-conditioned
+    """The split-precision workload at nocc=30, nvir=300: INT8 slice counts 3..7,
+    the BF16x3 form and, as a precision comparison point, the FP32 form, on the
+    well-conditioned inputs and on the ill-conditioned
     variant (J/K entries scaled by 10^u, u ~ U(-6,0)); per case the FP64-eq
     TFLOP/s and E[T] against the FP64 form on the same inputs (itself checked
     against the oracle in the tests)."""
@@ -283,13 +284,14 @@ def split_sweep_report(cc, steps, warmup):
                 print("bench: split sweep %s S=%d %.1f TFLOP/s rel dE %.2e" % (
                     case["inputs"], S, case["split"][-1]["value"], case["split"][-1]["rel_dE"]),
                     file=sys.stderr, flush=True)
-            r = time_form(job, trip, steps, warmup, mode="bf16x3", order=1, graph=True)
-            e = r["res"]["emp4"] + r["res"]["emp5"]
-            case["bf16x3"] = {"value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
+            for mode in ("bf16x3", "fp32"):
+                r = time_form(job, trip, steps, warmup, mode=mode, order=1, graph=True)
+                e = r["res"]["emp4"] + r["res"]["emp5"]
+                case[mode] = {"value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
                               "ms_per_step": r["ms_per_step"], "rel_dE": abs(e - E) / abs(E)}
-            print("bench: split sweep %s bf16x3 %.1f TFLOP/s rel dE %.2e" % (
-                case["inputs"], case["bf16x3"]["value"], case["bf16x3"]["rel_dE"]),
-                file=sys.stderr, flush=True)
+                print("bench: split sweep %s %s %.1f TFLOP/s rel dE %.2e" % (
+                    case["inputs"], mode, case[mode]["value"], case[mode]["rel_dE"]),
+                    file=sys.stderr, flush=True)
         out["cases"].append(case)
     return out
 

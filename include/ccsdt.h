@@ -175,7 +175,11 @@ enum {
     CCSDT_MODE_FP64 = 0,        /* fused FP64 MFMA GEMM + energy epilogue */
     CCSDT_MODE_FP64_SPLIT = 1,  /* FP64 GEMM writes f to HBM, separate tengy pass */
     CCSDT_MODE_OZAKI_INT8 = 2,  /* Ozaki INT8 slices, INT32 accumulate, FP64 recombine */
-    CCSDT_MODE_BF16X3 = 3       /* three BF16 planes, FP32 accumulate per level, FP64 recombine */
+    CCSDT_MODE_BF16X3 = 3,      /* three BF16 planes, FP32 accumulate per level, FP64 recombine */
+    CCSDT_MODE_FP32 = 4         /* FP32 throughout, as the FP32 body gpu-omp/ccsd_trpdrv_omp_cbody.c
+                                   (operands rounded to FP32, FP32 MFMA GEMMs, FP32 energy
+                                   summand); per-pair partials summed in FP64.  A precision
+                                   comparison point */
 };
 
 /* Run flags. */
@@ -336,10 +340,11 @@ int ccsdt_server_destroy(ccsdt_server s);
 int ccsdt_selftest_mfma(const double *a64, const double *b64, double *d256);
 /* One v_mfma_i32_16x16x64_i8 (shape 0; d receives 4 ints per lane),
  * v_mfma_i32_32x32x32_i8 (shape 1; 16 ints per lane) or
- * v_mfma_f32_16x16x32_bf16 (shape 2; the bits of 4 floats per lane) on raw
- * operand registers: a256 and b256 hold each lane's 16 operand bytes as four
- * int32 words.  No counterpart in the reference: checks the INT8 and BF16 lane
- * layouts. */
+ * v_mfma_f32_16x16x32_bf16 (shape 2; the bits of 4 floats per lane) or
+ * v_mfma_f32_16x16x4_f32 (shape 3; the first word of each lane's operand is the
+ * float, 4 float results per lane) on raw operand registers: a256 and b256 hold
+ * each lane's 16 operand bytes as four int32 words.  No counterpart in the
+ * reference: checks the INT8, BF16 and FP32 lane layouts. */
 int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d, int shape);
 /* Streams `bytes` through a plain device copy kernel `reps` times and returns the
  * average milliseconds per copy (or -1): the reference point for HBM counters.

@@ -39,7 +39,7 @@ TENSOR_STREAM = {"t2": STREAM["tij"], "jv": STREAM["jia"], "kv": STREAM["kia"],
                  "ko": STREAM["kij"]}
 VECTOR_NAMES = ["dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2"]
 
-MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3}
+MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3, "fp32": 4}
 MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
 ORDER_TASK, ORDER_PAIR = 0, 1

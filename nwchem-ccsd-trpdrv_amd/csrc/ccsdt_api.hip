@@ -593,7 +593,11 @@ int job_tasks(ccsdt_job j, const int *trip, int64_t n)
             tk.lk = 3 + 2 * (k - klo);
         }
         tk.ik_differ = (i != k);
-        tk.pad = 0;
+        {   // gpu-omp/ccsd_trpdrv_omp_cbody.c:62: FP32 eorb, FP32 arithmetic
+            const float ea = (float)e[a - 1], ei = (float)e[g.ncor + i - 1];
+            const float ej = (float)e[g.ncor + jj - 1], ek = (float)e[g.ncor + k - 1];
+            tk.eaijk32 = ea - (ei + ej + ek);
+        }
         // omp/ccsd_trpdrv_omp_cbody.c:126, same operation order
         tk.eaijk = e[a - 1] - (e[g.ncor + i - 1] + e[g.ncor + jj - 1] + e[g.ncor + k - 1]);
     }
@@ -642,6 +646,9 @@ bool is_split_mode(int mode) { return mode == CCSDT_MODE_OZAKI_INT8 || mode == C
 
 // `slices` argument of the split launchers: the INT8 slice count, or the BF16x3 code.
 int slice_code(const SplitGeom &sg) { return sg.bf16 ? kSlicesBf16x3 : sg.slices; }
+
+// Modes that run a two-wave block-pair kernel through launch_fused_split.
+bool is_pair_mode(int mode) { return is_split_mode(mode) || mode == CCSDT_MODE_FP32; }
 
 // Persistent grid of a fused kernel: wg_per_cu workgroups per CU (0 or more
 // than fit: every workgroup slot of the device), in a multiple of 8 (one share
@@ -695,6 +702,8 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
     const Geom &g = j->g;
     hipEvent_t *E = j->ev;
     const bool split = is_split_mode(mode);
+    const bool f32 = (mode == CCSDT_MODE_FP32);
+    const int code = f32 ? kSlicesFp32 : slice_code(sg);
     HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice, j->st));
     if (j->h_seg->order)
         HIP_TRY(hipMemcpyAsync(j->d_order, j->h_order, (size_t)n * sizeof(int), hipMemcpyHostToDevice, j->st));
@@ -704,16 +713,18 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
     if (timed) HIP_TRY(hipEventRecord(E[0], j->st));
     if (split)
         HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
+    else if (f32)
+        HIP_TRY(launch_pack32(j->d_desc, j->npanels, g, reinterpret_cast<float *>(j->d_pan), j->st));
     else
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
     if (timed) HIP_TRY(hipEventRecord(E[1], j->st));
     const int64_t items = n * g.npairs;
-    const int grid = persistent ? persistent_grid(split, slice_code(sg), variant, wg_per_cu) : (int)items;
+    const int grid = persistent ? persistent_grid(split || f32, code, variant, wg_per_cu) : (int)items;
     const int64_t vv = (int64_t)g.nvir * g.nvir;
     const bool persist = persistent && !with_f;
-    if (split)
+    if (split || f32)
         HIP_TRY(launch_fused_split(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
-                                   slice_code(sg), 8 * vv, true, with_f, j->st));
+                                   code, 8 * vv, true, with_f, j->st));
     else
         HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
                              variant, 8 * vv, true, with_f, j->st));
@@ -740,14 +751,14 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     if (fouts && n != 1) return set_error(CCSDT_E_ARG, "f output needs a single triple");
     const int mode = opts ? opts->mode : CCSDT_MODE_FP64;
     const int flags = opts ? opts->flags : 0;
-    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT && !is_split_mode(mode))
+    if (mode != CCSDT_MODE_FP64 && mode != CCSDT_MODE_FP64_SPLIT && !is_pair_mode(mode))
         return set_error(CCSDT_E_ARG, "mode %d not available in this build", mode);
     if (flags & ~(CCSDT_RUN_PERSISTENT | CCSDT_RUN_GRAPH))
         return set_error(CCSDT_E_ARG, "unknown run flags 0x%x", flags);
     const int slices = (mode == CCSDT_MODE_OZAKI_INT8) ? (opts && opts->slices ? opts->slices : 4) : 0;
     if (mode == CCSDT_MODE_OZAKI_INT8 && (slices < 3 || slices > 7))
         return set_error(CCSDT_E_ARG, "Ozaki slice count %d outside 3..7", slices);
-    const int variant = (is_split_mode(mode) || !opts) ? 0 : opts->variant;
+    const int variant = (is_pair_mode(mode) || !opts) ? 0 : opts->variant;
     if (variant != CCSDT_TILE_ONE_WAVE && variant != CCSDT_TILE_TWO_WAVE)
         return set_error(CCSDT_E_ARG, "unknown tile variant %d", variant);
     const int wg_per_cu = opts ? opts->wg_per_cu : 0;
@@ -931,7 +942,7 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     int rc = ensure_device(-1);
     if (rc) return rc;
     ccsdt_run_opts b0 = base ? *base : ccsdt_run_opts{};
-    const bool split = is_split_mode(b0.mode);
+    const bool split = is_pair_mode(b0.mode);  // one tile: the two-wave kernel of the mode
     const bool fused = (b0.mode != CCSDT_MODE_FP64_SPLIT);
     std::vector<ccsdt_run_opts> cand;
     if (b0.mode == CCSDT_MODE_OZAKI_INT8 && b0.slices == 0) {
@@ -956,6 +967,7 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
         if (b0.slices == 0) b0.slices = 7;
     }
     const int slices = b0.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
+                       : b0.mode == CCSDT_MODE_FP32 ? kSlicesFp32
                        : split ? (b0.slices ? b0.slices : 4) : 0;
     for (int ord = 0; ord < (fused ? 2 : 1); ++ord) {
         for (int v = 0; v < (split ? 1 : 2); ++v) {
@@ -1162,7 +1174,7 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
                     memcpy(j->h_fstage + m * vv, fs[m], (size_t)vv * sizeof(double));
                 HIP_TRY(hipMemcpyAsync(j->d_f, j->h_fstage, (size_t)8 * vv * sizeof(double),
                                        hipMemcpyHostToDevice, j->st));
-                j->h_tasks[0] = Task{0, 0, 0, 0, 1, 0, eaijk};
+                j->h_tasks[0] = Task{0, 0, 0, 0, 1, (float)eaijk, eaijk};
                 HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, sizeof(Task), hipMemcpyHostToDevice,
                                        j->st));
                 HIP_TRY(launch_tengy(j->d_f, 8 * vv, j->g, j->d_tasks, 1, j->d_ev, j->d_vec6,
@@ -1815,7 +1827,7 @@ int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d,
 {
     int rc = ensure_device(-1);
     if (rc) return rc;
-    if (shape < 0 || shape > 2) return set_error(CCSDT_E_ARG, "shape must be 0, 1 or 2");
+    if (shape < 0 || shape > 3) return set_error(CCSDT_E_ARG, "shape must be 0..3");
     int *buf = nullptr;
     if ((rc = dev_alloc((void **)&buf, (256 + 256 + 1024) * sizeof(int)))) return rc;
     HIP_TRY(hipMemcpy(buf, a256, 256 * sizeof(int), hipMemcpyHostToDevice));

@@ -19,7 +19,7 @@ namespace ccsdt {
 struct Task {
     int li, lk, rij, rkj;
     int ik_differ;  // 1 when i != k: the k terms enter the totals
-    int pad;
+    float eaijk32;  // the same in FP32 arithmetic on FP32 eorb (the FP32 form)
     double eaijk;   // eorb[a] - (eorb[ncor+i] + eorb[ncor+j] + eorb[ncor+k])
 };
 
@@ -133,8 +133,14 @@ hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *
 // Per-task sums of the pair partials over all segments (grid may be persistent).
 hipError_t launch_reduce_segments(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
                                   int grid, hipStream_t st);
-// `slices` code of launch_fused_split / fused_blocks_per_cu for the BF16x3 form.
+// `slices` codes of launch_fused_split / fused_blocks_per_cu for the BF16x3
+// form and the FP32 form (both two-wave block-pair kernels).
 constexpr int kSlicesBf16x3 = 100;
+constexpr int kSlicesFp32 = 200;
+// FP32 panels (same element order as k_pack's, FP32 values; panel p at
+// pan32 + p * vecsz).
+hipError_t launch_pack32(const PanelDesc *d_desc, int npanels, const Geom &g, float *pan32,
+                         hipStream_t st);
 // Resident workgroups per CU of the fused kernels (for persistent grids).
 int fused_blocks_per_cu(bool split, int slices, int variant);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,

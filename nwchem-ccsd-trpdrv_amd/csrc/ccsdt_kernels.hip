@@ -20,6 +20,8 @@
 //   value(x = 16*xb + l%16, kk = 4*ks + l/16),
 // the A (or B) operand of one MFMA, so a wave's operand load is one contiguous
 // 512-byte global_load_dwordx2.
+#include <type_traits>
+
 #include "ccsdt_internal.h"
 
 namespace ccsdt {
@@ -27,20 +29,23 @@ namespace ccsdt {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-struct Sum4 {
-    double e4i, e5i, e4k, e5k;
+template <typename T>
+struct SumT {
+    T e4i, e5i, e4k, e5k;
 };
+typedef SumT<double> Sum4;
 
 // Energy summand of one (b,c) point.  n*, t* are f?n, f?t at (b,c); nr*, tr*
 // at (c,b); d = -1/(e_b + e_c + eaijk); u = t1v(b); x = dintx(c); y = dintc(c).
 // Contraction is off so the arithmetic follows omp/ccsd_tengy_c99.c:91-111
 // operation for operation.
-__device__ __forceinline__ void tengy_point(double n1, double n2, double n3, double n4,
-                                            double t1, double t2, double t3, double t4,
-                                            double nr1, double nr2, double nr3, double nr4,
-                                            double tr1, double tr2, double tr3, double tr4,
-                                            double d, double u1, double u2, double x1,
-                                            double x2, double y1, double y2, Sum4 &s)
+// T = float evaluates the same operations in FP32, as the FP32 body of
+// gpu-omp/ccsd_trpdrv_omp_cbody.c:204-221 does.
+template <typename T>
+__device__ __forceinline__ void tengy_point(T n1, T n2, T n3, T n4, T t1, T t2, T t3, T t4,
+                                            T nr1, T nr2, T nr3, T nr4, T tr1, T tr2, T tr3,
+                                            T tr4, T d, T u1, T u2, T x1, T x2, T y1, T y2,
+                                            SumT<T> &s)
 {
 #pragma clang fp contract(off)
     s.e4i += d * (t1 + nr1 + tr2 + n3 + nr4) * (t1 - t2 * 2 - t3 * 2 + t4)
@@ -57,7 +62,8 @@ __device__ __forceinline__ void tengy_point(double n1, double n2, double n3, dou
            + d * u2 * y2 * (t1 + t4 + nr1 - (t2 + t3 + nr2) * 2);
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -197,6 +203,45 @@ hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipS
     int64_t blocks = (g.vecsz + 255) / 256;
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(k_pack, dim3((unsigned)blocks, npanels), dim3(256), 0, st, d_desc, g);
+    return hipGetLastError();
+}
+
+// FP32 panels for the FP32 form: k_pack's element order with the values
+// rounded to FP32 (operand lane layout of v_mfma_f32_16x16x4_f32, which
+// matches the FP64 one: lane l holds row l%16, K index l/16 of a 4-wide step).
+__global__ __launch_bounds__(256) void k_pack32(const PanelDesc *__restrict__ descs, Geom g,
+                                                float *__restrict__ pan32)
+{
+    const PanelDesc d = descs[blockIdx.y];
+    float *out = pan32 + (int64_t)blockIdx.y * g.vecsz;
+    const int nv = g.nvir, no = g.nocc;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < g.vecsz;
+         idx += (int64_t)gridDim.x * 256) {
+        const int l = (int)(idx & 63);
+        const int64_t rest = idx >> 6;
+        const int ks = (int)(rest % g.nks);
+        const int xb = (int)(rest / g.nks);
+        const int x = xb * 16 + (l & 15);
+        const int kk = ks * 4 + (l >> 4);
+        float v = 0.f;
+        if (x < nv) {
+            if (kk < nv) {
+                v = (float)(d.first_t ? d.P[kk + (int64_t)x * nv] : d.P[x + (int64_t)kk * nv]);
+            } else if (kk < nv + no) {
+                const int l2 = kk - nv;
+                v = (float)d.sgn * (float)(d.second_t ? d.Q[l2 + (int64_t)x * no] : d.Q[x + (int64_t)l2 * nv]);
+            }
+        }
+        out[idx] = v;
+    }
+}
+
+hipError_t launch_pack32(const PanelDesc *d_desc, int npanels, const Geom &g, float *pan32,
+                         hipStream_t st)
+{
+    int64_t blocks = (g.vecsz + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_pack32, dim3((unsigned)blocks, npanels), dim3(256), 0, st, d_desc, g, pan32);
     return hipGetLastError();
 }
 
@@ -786,17 +831,93 @@ __device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restr
 #undef MS
 }
 
+// FP32 form of the two-wave item (k_fused32): fp64_wave's eight products per
+// wave role over FP32 panels on v_mfma_f32_16x16x4_f32, FP32 accumulation.
+// That MFMA's result layout is already row 4*(l/16)+r, column l%16, so the A
+// rows go in natural order.
+__device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+template <int W>
+__device__ __forceinline__ void fp32_wave(flt4 (&facc)[8], const float *__restrict__ pan,
+                                          const Geom &g, const PanelPairs &pp, int bi, int ci,
+                                          int lane)
+{
+    const int64_t vz = g.vecsz;
+    const uint32_t bytes = (uint32_t)(2 * vz * 4);
+    __amdgpu_buffer_rsrc_t rp[4];
+    rp[RP_LI] = buffer_rsrc(reinterpret_cast<const double *>(pan + 2 * (int64_t)pp.li * vz), bytes);
+    rp[RP_LK] = buffer_rsrc(reinterpret_cast<const double *>(pan + 2 * (int64_t)pp.lk * vz), bytes);
+    rp[RP_RIJ] = buffer_rsrc(reinterpret_cast<const double *>(pan + 2 * (int64_t)pp.rij * vz), bytes);
+    rp[RP_RKJ] = buffer_rsrc(reinterpret_cast<const double *>(pan + 2 * (int64_t)pp.rkj * vz), bytes);
+    const uint32_t vsb = (uint32_t)(vz * 4);
+    const uint32_t blkb = (uint32_t)g.nks * 256;
+    const uint32_t ob = bi * blkb, oc = ci * blkb;
+    const uint32_t voff = lane * 4;
+    constexpr int LR[4] = {RP_LI, RP_LI, RP_LK, RP_LK};
+    constexpr int RR[4] = {RP_RKJ, RP_RKJ, RP_RIJ, RP_RIJ};
+    uint32_t ao[4], bo[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        ao[v] = (v & 1) * vsb + ob;
+        bo[v] = (v & 1) * vsb + oc;
+    }
+    const int nks = g.nks;
+    float xa[4], xb[4], ya[4], yb[4];
+#define F3_LOAD(da, db, kstep)                                                                   \
+    _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                             \
+        const int ra = (W == 0) ? LR[v] : RR[v], rb = (W == 0) ? RR[v] : LR[v];                   \
+        da[v] = buffer_load_f32(rp[ra], voff, ao[v] + (uint32_t)(kstep) * 256);                   \
+        db[v] = buffer_load_f32(rp[rb], voff, bo[v] + (uint32_t)(kstep) * 256);                   \
+    }
+#define MS(pp_, ia, ib) facc[pp_] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ia], b[ib], facc[pp_], 0, 0, 0)
+#define F3_MFMA(da, db)                                                                          \
+    {                                                                                            \
+        const float *a = da, *b = db;                                                            \
+        if (W == 0) {                                                                            \
+            MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);                                  \
+            MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);                                  \
+        } else {                                                                                 \
+            MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);                                  \
+            MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);                                  \
+        }                                                                                        \
+    }
+    F3_LOAD(xa, xb, 0)
+    int ks = 0;
+    for (; ks + 1 < nks; ks += 2) {
+        F3_LOAD(ya, yb, ks + 1)
+        F3_MFMA(xa, xb)
+        const int k2 = (ks + 2 < nks) ? ks + 2 : nks - 1;  // clamped prefetch, in bounds
+        F3_LOAD(xa, xb, k2)
+        F3_MFMA(ya, yb)
+    }
+    if (nks & 1) F3_MFMA(xa, xb)
+#undef F3_LOAD
+#undef F3_MFMA
+#undef MS
+}
+
 // Epilogue of a two-wave block-pair item.  On entry wave 0 holds the eight
 // products at (b,c) and wave 1 the eight mirrored ones, both in the layout
 // lane l, element r -> row 4*(l/16)+r, column l%16.  On a diagonal pair
 // (B = C) wave 1 computed nothing: the mirrored products are the direct ones
 // transposed, read back from one LDS tile.
-template <bool TENGY, bool WRITEF>
-__device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, const dbl4 (&facc)[8],
-                                              int bi, int ci, int w, int lane,
-                                              double *__restrict__ part4, double *__restrict__ fo)
+// T = float (the FP32 form) evaluates the energy summand in FP32, with the
+// denominator formed as in gpu-omp/ccsd_trpdrv_omp_cbody.c:178 (FP32 sum, FP64
+// division, rounded to FP32); the per-pair partial leaves as FP64.
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { typedef dbl4 type; };
+template <> struct Vec4<float> { typedef flt4 type; };
+
+template <typename T, bool TENGY, bool WRITEF>
+__device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
+                                              const typename Vec4<T>::type (&facc)[8], int bi, int ci,
+                                              int w, int lane, double *__restrict__ part4,
+                                              double *__restrict__ fo)
 {
-    __shared__ double xch[2][8][2][64];
+    __shared__ T xch[2][8][2][64];
     __shared__ double red[2][4];
     const Geom &g = jd.g;
     const double *__restrict__ ev = jd.ev;
@@ -817,7 +938,7 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, 
 #pragma unroll
                     for (int q = 0; q < 8; ++q) {
                         const int64_t o = (w == 0) ? bx + (int64_t)cx * nv : cx + (int64_t)bx * nv;
-                        fo[mat[q] * vv + o] = facc[q][r];
+                        fo[mat[q] * vv + o] = (double)facc[q][r];
                     }
                 }
             }
@@ -828,13 +949,13 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, 
         // wave 1 keeps rows r = 2,3 and receives the direct values.  Diagonal
         // pairs: wave 0's products go to an LDS tile T[q][row][col] that both
         // waves read as (b,c) and transposed as (c,b).
-        double(*T)[16][16] = reinterpret_cast<double(*)[16][16]>(&xch[0][0][0][0]);
+        T(*Tt)[16][16] = reinterpret_cast<T(*)[16][16]>(&xch[0][0][0][0]);
         if (diag) {
             if (w == 0) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) T[q][row0 + r][lane & 15] = facc[q][r];
+                    for (int r = 0; r < 4; ++r) Tt[q][row0 + r][lane & 15] = facc[q][r];
             }
         } else {
 #pragma unroll
@@ -843,40 +964,41 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, 
                 for (int h = 0; h < 2; ++h) xch[w][q][h][lane] = facc[q][(w == 0 ? 2 : 0) + h];
         }
         __syncthreads();
-        Sum4 s = {0.0, 0.0, 0.0, 0.0};
-        const double ec = ev[cx];
-        const double u1c = vec6[cx], u2c = vec6[nvp + cx];
-        const double x1c = vec6[2 * nvp + cx], x2c = vec6[3 * nvp + cx];
-        const double y1c = vec6[4 * nvp + cx], y2c = vec6[5 * nvp + cx];
+        SumT<T> s = {0, 0, 0, 0};
+        const T ec = (T)ev[cx];
+        const T u1c = (T)vec6[cx], u2c = (T)vec6[nvp + cx];
+        const T x1c = (T)vec6[2 * nvp + cx], x2c = (T)vec6[3 * nvp + cx];
+        const T y1c = (T)vec6[4 * nvp + cx], y2c = (T)vec6[5 * nvp + cx];
+        const T eaijk = std::is_same<T, float>::value ? (T)tk.eaijk32 : (T)tk.eaijk;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int r = (w == 0 ? 0 : 2) + h;
-            double dv[8], mv[8];
+            T dv[8], mv[8];
             if (diag) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    dv[q] = T[q][row0 + r][lane & 15];
-                    mv[q] = T[q][lane & 15][row0 + r];
+                    dv[q] = Tt[q][row0 + r][lane & 15];
+                    mv[q] = Tt[q][lane & 15][row0 + r];
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    const double own = facc[q][r], other = xch[1 - w][q][h][lane];
+                    const T own = facc[q][r], other = xch[1 - w][q][h][lane];
                     dv[q] = (w == 0) ? own : other;
                     mv[q] = (w == 0) ? other : own;
                 }
             }
             const int bx = bi * 16 + row0 + r;
             if (bx < nv && cx < nv) {
-                const double dd = -1.0 / (ev[bx] + ec + tk.eaijk);
-                tengy_point(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
-                            mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
-                            dd, vec6[bx], vec6[nvp + bx], x1c, x2c, y1c, y2c, s);
+                const T dd = (T)(-1.0 / (double)((T)ev[bx] + ec + eaijk));
+                tengy_point<T>(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
+                               mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
+                               dd, (T)vec6[bx], (T)vec6[nvp + bx], x1c, x2c, y1c, y2c, s);
                 if (bi != ci)
-                    tengy_point(mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
-                                dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
-                                dd, u1c, u2c, vec6[2 * nvp + bx], vec6[3 * nvp + bx],
-                                vec6[4 * nvp + bx], vec6[5 * nvp + bx], s);
+                    tengy_point<T>(mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
+                                   dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
+                                   dd, u1c, u2c, (T)vec6[2 * nvp + bx], (T)vec6[3 * nvp + bx],
+                                   (T)vec6[4 * nvp + bx], (T)vec6[5 * nvp + bx], s);
             }
         }
         s.e4i = wave_sum(s.e4i);
@@ -893,36 +1015,49 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk, 
 }
 
 // One two-wave work item: the 16x16 block pair (bi, ci) of one task.  S = 0
-// takes the FP64 panels (k_fused2), S >= 3 the INT8 digit planes of the split
-// form (k_fused_split).
+// takes the FP64 panels (k_fused2), 3..7 the INT8 digit planes of the split
+// form (k_fused_split), kSlicesBf16x3 the BF16 planes (k_fused_bf16x3) and
+// kSlicesFp32 the FP32 panels (k_fused32).
 template <int S, bool TENGY, bool WRITEF>
 __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
                                           int bi, int ci, int w, int lane,
                                           double *__restrict__ part4, double *__restrict__ fo)
 {
-    dbl4 facc[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
     const bool diag = (bi == ci);
-    if constexpr (S == 0) {
+    if constexpr (S == kSlicesFp32) {
+        flt4 f32[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f32[q] = flt4{0.f, 0.f, 0.f, 0.f};
+        const float *pan32 = reinterpret_cast<const float *>(jd.pan);
         if (w == 0)
-            fp64_wave<0>(facc, jd.pan, jd.g, pp, bi, ci, lane);
+            fp32_wave<0>(f32, pan32, jd.g, pp, bi, ci, lane);
         else if (!diag)
-            fp64_wave<1>(facc, jd.pan, jd.g, pp, bi, ci, lane);
+            fp32_wave<1>(f32, pan32, jd.g, pp, bi, ci, lane);
+        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo);
     } else {
-        // INT8 digits (S planes) or the BF16x3 planes
-        constexpr bool BF = (S == kSlicesBf16x3);
-        constexpr int NP = BF ? 3 : S;
-        const SplitGeom &sg = jd.sg;
-        const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
-                                                      (uint32_t)(sg.npanels * NP * sg.slice_bytes));
-        const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
-        if (w == 0)
-            split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
-        else if (!diag)
-            split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+        dbl4 facc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) facc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+        if constexpr (S == 0) {
+            if (w == 0)
+                fp64_wave<0>(facc, jd.pan, jd.g, pp, bi, ci, lane);
+            else if (!diag)
+                fp64_wave<1>(facc, jd.pan, jd.g, pp, bi, ci, lane);
+        } else {
+            // INT8 digits (S planes) or the BF16x3 planes
+            constexpr bool BF = (S == kSlicesBf16x3);
+            constexpr int NP = BF ? 3 : S;
+            const SplitGeom &sg = jd.sg;
+            const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
+                                                          (uint32_t)(sg.npanels * NP * sg.slice_bytes));
+            const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
+            if (w == 0)
+                split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+            else if (!diag)
+                split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+        }
+        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo);
     }
-    pair_epilogue<TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo);
 }
 
 // Item loop of the two-wave kernels (128 threads, one block pair per item).
@@ -972,6 +1107,15 @@ __global__ __launch_bounds__(128) void k_fused_bf16x3(const JobDev *__restrict__
                                                       WorkHdr *hdr, int64_t fstride)
 {
     pair_items<kSlicesBf16x3, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
+}
+
+// The FP32 form (FP32 panels, MFMA and energy summand; FP64 partial sums).
+template <bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(128) void k_fused32(const JobDev *__restrict__ jobs,
+                                                 const Segment *__restrict__ segs,
+                                                 WorkHdr *hdr, int64_t fstride)
+{
+    pair_items<kSlicesFp32, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
 }
 
 // The FP64 form's work item on two waves (the alternative tile of k_fused).
@@ -1029,6 +1173,20 @@ hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *
                               bool writef, hipStream_t st)
 {
     if (grid <= 0) return hipErrorInvalidValue;
+    if (slices == kSlicesFp32) {
+        dim3 g3((unsigned)grid), block(128);
+        if (persist && tengy && !writef)
+            hipLaunchKernelGGL((k_fused32<true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else if (persist)
+            return hipErrorInvalidValue;
+        else if (tengy && writef)
+            hipLaunchKernelGGL((k_fused32<true, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else if (tengy)
+            hipLaunchKernelGGL((k_fused32<true, false, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (slices == kSlicesBf16x3) {
         dim3 g3((unsigned)grid), block(128);
         if (persist && tengy && !writef)
@@ -1067,6 +1225,7 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
     } else {
         switch (slices) {
         case kSlicesBf16x3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_bf16x3<true, false, true>, 128, 0); break;
+        case kSlicesFp32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused32<true, false, true>, 128, 0); break;
         case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<3, true, false, true>, 128, 0); break;
         case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<4, true, false, true>, 128, 0); break;
         case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<5, true, false, true>, 128, 0); break;
@@ -1363,6 +1522,14 @@ __global__ __launch_bounds__(64) void k_mfma_i8_probe(const int *a, const int *b
         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
+    } else if (shape == 3) {
+        // v_mfma_f32_16x16x4_f32: the first word of each lane's operand as a float
+        const flt4 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, av[0]),
+                                                              __builtin_bit_cast(float, bv[0]),
+                                                              flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        float *df = reinterpret_cast<float *>(d);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) df[l * 4 + r] = acc[r];
     } else if (shape == 2) {
         // the 16 bytes per lane as 8 BF16 values; FP32 results returned as bits
         const flt4 acc = PlaneMath<true>::mfma(av, bv, flt4{0.f, 0.f, 0.f, 0.f});

@@ -154,6 +154,35 @@ def ref_cbody_skipdgemm_lib():
     return L
 
 
+def ref_cbody_f32_lib():
+    """gpu-omp/ccsd_trpdrv_omp_cbody.c (the FP32 body) with -DSKIP_DGEMM, or None."""
+    L = _ref("libref_cbody_f32_skipdgemm.so")
+    if L is not None:
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ccsd_trpdrv_omp_cbody_.argtypes = [fp] * 9 + [_ip] * 3 + [fp] * 2 + [_ip] * 5 + [fp] * 20
+        L.ccsd_trpdrv_omp_cbody_.restype = None
+    return L
+
+
+def ref_cbody_f32_glue(L, fcm, arrays, ncor, nocc, nvir, a, i, j, k, klo):
+    """The FP32 reference body's eaijk + energy loop + accumulation on given FP32
+    f buffers (column major, body order), all inputs rounded to FP32.  Returns
+    (emp4, emp5) starting from zero, as Python floats of the FP32 results."""
+    fp = ctypes.POINTER(ctypes.c_float)
+
+    def f32(x):
+        return np.ascontiguousarray(x, dtype=np.float32)
+    fs = [f32(x).copy() for x in fcm]
+    ints = [ctypes.c_int(v) for v in (ncor, nocc, nvir, a, i, j, k, klo)]
+    e4, e5 = ctypes.c_float(0.0), ctypes.c_float(0.0)
+    A = [f32(_arr(arrays, n)) for n in ARRAY_NAMES[1:]]
+    L.ccsd_trpdrv_omp_cbody_(*[x.ctypes.data_as(fp) for x in fs], f32(_arr(arrays, "eorb")).ctypes.data_as(fp),
+                             ctypes.byref(ints[0]), ctypes.byref(ints[1]), ctypes.byref(ints[2]),
+                             ctypes.byref(e4), ctypes.byref(e5),
+                             *[ctypes.byref(v) for v in ints[3:]], *[x.ctypes.data_as(fp) for x in A])
+    return float(e4.value), float(e5.value)
+
+
 def ref_tengy_f77_lib():
     """omp/ccsd_tengy_f77.f (ccsd_tengy_omp_, Fortran order), or None."""
     L = _ref("libref_tengy_f77.so")

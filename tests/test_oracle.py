@@ -251,3 +251,30 @@ def test_ik_rule(orc, cc):
     e5 = t[:, 1].sum() + t[ik, 3].sum()
     assert np.isclose(out["emp4"], e4, rtol=1e-13) and np.isclose(out["emp5"], e5, rtol=1e-13)
     assert np.isclose(out["emp4_plain"], out["emp4"], rtol=1e-12)
+
+
+def _round32(arrays):
+    return {n: np.asarray(v, dtype=np.float32).astype(np.float64) for n, v in arrays.items()}
+
+
+def test_fp32_reference_body_tracks_fp64(orc, cc):
+    """The FP32 reference body (gpu-omp, built with its SKIP_DGEMM switch) on
+    FP32-rounded inputs agrees with the FP64 oracle on the same inputs to FP32
+    accuracy, term by term; a dropped or mis-signed term would show at the
+    percent level."""
+    L = orc.ref_cbody_f32_lib()
+    if L is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    nocc, nvir, kchunk = 5, 24, 5
+    arrays = _round32(cc.make_inputs(nocc, nvir, kchunk, 7))
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    for a, i, j, k, klo in trip[::3]:
+        f = orc.fmats(arrays, nocc, nvir, int(k), int(klo))
+        fcm = [np.ascontiguousarray(f[m].T) for m in range(8)]
+        e4, e5 = orc.ref_cbody_f32_glue(L, fcm, arrays, 0, nocc, nvir, *map(int, (a, i, j, k, klo)))
+        r = orc.run_list(arrays, 0, nocc, nvir, np.array([[a, i, j, k, klo]], np.int32), with_abs=True)
+        t, s = r["terms"][0], r["abs4"][0]
+        ik = i != k
+        E4, S4 = t[0] + ik * t[2], s[0] + ik * s[2]
+        E5, S5 = t[1] + ik * t[3], s[1] + ik * s[3]
+        assert abs(e4 - E4) <= 1e-5 * S4 and abs(e5 - E5) <= 1e-5 * S5
