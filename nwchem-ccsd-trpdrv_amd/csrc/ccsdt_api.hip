@@ -7,6 +7,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <deque>
 #include <map>
 #include <cstdarg>
@@ -15,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ccsdt.h"
@@ -1221,7 +1223,7 @@ struct SJob {
     int64_t footprint = 0;
     bool tensors = false;  // full-tensor job
     double t_submit = 0.0, t_admit = 0.0, t_finish = 0.0;
-    int state = 0;  // 0 waiting, 1 running, 2 reducing, 3 finished, -1 failed
+    int state = 0;  // 0 waiting, 4 admitting, 1 running, 2 reducing, 3 finished, -1 failed
     int status = CCSDT_OK;
     hipEvent_t fin = nullptr;
     double emp4 = 0.0, emp5 = 0.0;
@@ -1252,6 +1254,14 @@ struct ccsdt_server_s {
     int64_t next_id = 1;
     Clock::time_point t0;
     std::mutex mu;
+    // Admission runs on its own thread (job creation, uploads and operand
+    // packing on the job's stream) so ticks keep launching meanwhile.  Jobs in
+    // admit_q already hold their budget share and job-table slot.
+    std::thread admitter;
+    std::condition_variable cv_work, cv_ready;
+    std::deque<int64_t> admit_q;
+    int64_t admitting = 0;
+    bool stop = false;
 };
 
 namespace {
@@ -1294,9 +1304,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
         HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
     else
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
-    sj->slot = s->free_slots.back();
-    s->free_slots.pop_back();
-    JobDev &jd = s->h_jobs[sj->slot];
+    JobDev &jd = s->h_jobs[sj->slot];  // the slot reserved for this job
     jd.g = g;
     jd.sg = sg;
     jd.pan = j->d_pan;
@@ -1309,9 +1317,6 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
     HIP_TRY(hipMemcpyAsync(s->d_jobs + sj->slot, &jd, sizeof(JobDev), hipMemcpyHostToDevice, j->st));
     HIP_TRY(hipEventCreate(&sj->fin));
     HIP_TRY(hipStreamSynchronize(j->st));
-    sj->state = 1;
-    sj->t_admit = server_now(s);
-    s->running.push_back(sj->id);
     return CCSDT_OK;
 }
 
@@ -1331,7 +1336,8 @@ void server_fail(ccsdt_server s, SJob *sj, int rc)
     ++s->nfinished;
 }
 
-// Admits waiting jobs in arrival order while they fit.
+// Hands waiting jobs, in arrival order while they fit, to the admission
+// thread: each takes its budget share and a job-table slot now.  Caller holds mu.
 void server_admit_waiting(ccsdt_server s)
 {
     while (!s->waiting.empty() && !s->free_slots.empty()) {
@@ -1339,12 +1345,41 @@ void server_admit_waiting(ccsdt_server s)
         if (s->used + sj->footprint > s->o.budget_bytes) break;
         s->waiting.pop_front();
         s->used += sj->footprint;
+        sj->slot = s->free_slots.back();
+        s->free_slots.pop_back();
+        sj->state = 4;
+        ++s->admitting;
+        s->admit_q.push_back(sj->id);
+        s->cv_work.notify_one();
+    }
+}
+
+// Admission thread: uploads and packs queued jobs (without the server lock,
+// on each job's own stream), then publishes them as running.
+void server_admission_loop(ccsdt_server s)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] { return s->stop || !s->admit_q.empty(); });
+        if (s->stop) return;
+        SJob *sj = s->jobs[s->admit_q.front()];
+        s->admit_q.pop_front();
+        lk.unlock();
         const double *arr[CCSDT_NARRAYS];
         for (int q = 0; q < CCSDT_NARRAYS; ++q) arr[q] = sj->host[q].data();
         const int rc = server_admit(s, sj, arr);
+        lk.lock();
         sj->host.clear();
         sj->host.shrink_to_fit();
-        if (rc) server_fail(s, sj, rc);
+        --s->admitting;
+        if (rc) {
+            server_fail(s, sj, rc);
+        } else {
+            sj->state = 1;
+            sj->t_admit = server_now(s);
+            s->running.push_back(sj->id);
+        }
+        s->cv_ready.notify_all();
     }
 }
 
@@ -1521,35 +1556,30 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
             (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
     }
-    std::lock_guard<std::mutex> lk(s->mu);
+    const int64_t fp = server_footprint(s->o, desc, n, tensors);
+    if (fp > s->o.budget_bytes)
+        return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
+                         (long long)fp, (long long)s->o.budget_bytes);
     SJob *sj = new SJob();
-    sj->id = s->next_id++;
+    sj->t_submit = server_now(s);
     sj->desc = *desc;
     sj->tensors = tensors;
     sj->n = n;
+    sj->footprint = fp;
     sj->trip.assign(triples, triples + 5 * n);
-    sj->footprint = server_footprint(s->o, desc, n, tensors);
-    sj->t_submit = server_now(s);
-    if (sj->footprint > s->o.budget_bytes) {
-        const int64_t sj_fp = sj->footprint;
-        delete sj;
-        return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
-                         (long long)sj_fp, (long long)s->o.budget_bytes);
-    }
+    // the arrays are copied before return (outside the server lock); the
+    // admission thread uploads them
+    int64_t cnt[CCSDT_NARRAYS];
+    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt, tensors);
+    sj->host.resize(CCSDT_NARRAYS);
+    for (int q = 0; q < CCSDT_NARRAYS; ++q)
+        if (cnt[q] > 0) sj->host[q].assign(desc->arrays[q], desc->arrays[q] + cnt[q]);
+    std::lock_guard<std::mutex> lk(s->mu);
+    sj->id = s->next_id++;
     s->jobs[sj->id] = sj;
     if (id) *id = sj->id;
-    if (s->waiting.empty() && !s->free_slots.empty() && s->used + sj->footprint <= s->o.budget_bytes) {
-        s->used += sj->footprint;
-        rc = server_admit(s, sj, desc->arrays);  // straight from the caller's arrays
-        if (rc) server_fail(s, sj, rc);
-    } else {
-        int64_t cnt[CCSDT_NARRAYS];
-        array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt, tensors);
-        sj->host.resize(CCSDT_NARRAYS);
-        for (int q = 0; q < CCSDT_NARRAYS; ++q)
-            if (cnt[q] > 0) sj->host[q].assign(desc->arrays[q], desc->arrays[q] + cnt[q]);
-        s->waiting.push_back(sj->id);
-    }
+    s->waiting.push_back(sj->id);
+    server_admit_waiting(s);
     t_err = CCSDT_OK;
     return CCSDT_OK;
 }
@@ -1708,6 +1738,11 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
             return fail(set_error(CCSDT_E_NOMEM, "tick buffers failed"));
     }
     for (int q = kMaxSlots - 1; q >= 0; --q) s->free_slots.push_back(q);
+    try {
+        s->admitter = std::thread(server_admission_loop, s);
+    } catch (...) {
+        return fail(set_error(CCSDT_E_STATE, "admission thread could not start"));
+    }
     *out = s;
     t_err = CCSDT_OK;
     return CCSDT_OK;
@@ -1730,7 +1765,14 @@ int ccsdt_server_submit_tensors(ccsdt_server s, const ccsdt_tensor_desc *desc, c
 int ccsdt_server_progress(ccsdt_server s, int block)
 {
     if (!s) return set_error(CCSDT_E_STATE, "null server");
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
+    // nothing on the device but jobs being admitted: wait for an admission
+    // (bounded, so a caller polling other work is not held long)
+    if (block && s->order.empty() && s->admitting > 0) {
+        const int64_t before = s->admitting;
+        s->cv_ready.wait_for(lk, std::chrono::milliseconds(100),
+                             [&] { return s->admitting != before; });
+    }
     int rc = server_retire(s, block != 0 && !s->order.empty());
     if (rc) return rc;
     const int nf = server_finish(s);
@@ -1771,7 +1813,7 @@ int ccsdt_server_counts(ccsdt_server s, int64_t *waiting, int64_t *running, int6
     std::lock_guard<std::mutex> lk(s->mu);
     int64_t run = 0;
     for (auto &kv : s->jobs)
-        if (kv.second->state == 1 || kv.second->state == 2) ++run;
+        if (kv.second->state == 1 || kv.second->state == 2 || kv.second->state == 4) ++run;
     if (waiting) *waiting = (int64_t)s->waiting.size();
     if (running) *running = run;
     if (finished) *finished = s->nfinished;
@@ -1784,6 +1826,14 @@ double ccsdt_server_clock(ccsdt_server s) { return s ? server_now(s) : 0.0; }
 int ccsdt_server_destroy(ccsdt_server s)
 {
     if (!s) return set_error(CCSDT_E_STATE, "null server");
+    if (s->admitter.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->admitter.join();  // finishes the admission in progress, if any
+    }
     if (s->st) (void)hipStreamSynchronize(s->st);
     for (auto &kv : s->jobs) {
         SJob *sj = kv.second;
