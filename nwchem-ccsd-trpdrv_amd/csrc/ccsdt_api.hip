@@ -145,6 +145,75 @@ void dev_free(void *p)
     }
 }
 
+// Pinned host buffers and streams of jobs come from caches as well: freeing
+// pinned memory (hipHostFree) and destroying streams can wait for the device,
+// which in a serving loop would stall every job's completion behind the
+// kernels in flight.
+constexpr size_t kHostCacheMax = size_t(2) << 30;
+std::mutex g_host_mu;
+std::multimap<size_t, void *> g_host_free;
+std::map<void *, size_t> g_host_live;
+size_t g_host_cached = 0;
+std::vector<hipStream_t> g_streams;
+
+hipError_t host_alloc(void **p, size_t bytes)
+{
+    const size_t sz = (bytes + 4095) & ~size_t(4095);
+    {
+        std::lock_guard<std::mutex> lk(g_host_mu);
+        auto it = g_host_free.lower_bound(sz);
+        if (it != g_host_free.end() && it->first <= sz + sz / 4) {
+            *p = it->second;
+            g_host_live[*p] = it->first;
+            g_host_cached -= it->first;
+            g_host_free.erase(it);
+            return hipSuccess;
+        }
+    }
+    const hipError_t e = hipHostMalloc(p, sz, 0);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_host_mu);
+    g_host_live[*p] = sz;
+    return hipSuccess;
+}
+
+void host_free(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_host_mu);
+    auto it = g_host_live.find(p);
+    if (it == g_host_live.end()) return;
+    const size_t sz = it->second;
+    g_host_live.erase(it);
+    if (g_host_cached + sz <= kHostCacheMax) {
+        g_host_free.emplace(sz, p);
+        g_host_cached += sz;
+    } else {
+        (void)hipHostFree(p);
+    }
+}
+
+hipError_t stream_get(hipStream_t *st)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_host_mu);
+        if (!g_streams.empty()) {
+            *st = g_streams.back();
+            g_streams.pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreate(st);
+}
+
+// The stream must be idle (its work synchronized) when it is returned.
+void stream_put(hipStream_t st)
+{
+    if (!st) return;
+    std::lock_guard<std::mutex> lk(g_host_mu);
+    g_streams.push_back(st);
+}
+
 // ------------------------------------------------------------------ device
 std::mutex g_init_mu;
 int g_device = -1;
@@ -345,22 +414,25 @@ void job_free(ccsdt_job j)
     if (j->d_jobdev) dev_free(j->d_jobdev);
     if (j->d_seg) dev_free(j->d_seg);
     if (j->d_hdr) dev_free(j->d_hdr);
-    if (j->h_jobdev) (void)hipHostFree(j->h_jobdev);
-    if (j->h_seg) (void)hipHostFree(j->h_seg);
-    if (j->h_hdr) (void)hipHostFree(j->h_hdr);
-    if (j->h_terms) (void)hipHostFree(j->h_terms);
+    host_free(j->h_jobdev);
+    host_free(j->h_seg);
+    host_free(j->h_hdr);
+    host_free(j->h_terms);
     if (j->d_pairs) dev_free(j->d_pairs);
     if (j->d_tasks) dev_free(j->d_tasks);
-    if (j->h_tasks) (void)hipHostFree(j->h_tasks);
+    host_free(j->h_tasks);
     if (j->d_order) dev_free(j->d_order);
-    if (j->h_order) (void)hipHostFree(j->h_order);
-    if (j->h_stage) (void)hipHostFree(j->h_stage);
-    if (j->h_vec) (void)hipHostFree(j->h_vec);
-    if (j->h_fstage) (void)hipHostFree(j->h_fstage);
-    if (j->h_out) (void)hipHostFree(j->h_out);
+    host_free(j->h_order);
+    host_free(j->h_stage);
+    host_free(j->h_vec);
+    host_free(j->h_fstage);
+    host_free(j->h_out);
     for (hipEvent_t &e : j->ev)
         if (e) (void)hipEventDestroy(e);
-    if (j->st) (void)hipStreamDestroy(j->st);
+    if (j->st) {
+        (void)hipStreamSynchronize(j->st);
+        stream_put(j->st);
+    }
     delete j;
 }
 
@@ -442,9 +514,9 @@ int ensure_tasks(ccsdt_job j, int64_t n)
 {
     if (n <= j->cap_tasks) return CCSDT_OK;
     if (j->d_tasks) dev_free(j->d_tasks);
-    if (j->h_tasks) (void)hipHostFree(j->h_tasks);
+    host_free(j->h_tasks);
     if (j->d_order) dev_free(j->d_order);
-    if (j->h_order) (void)hipHostFree(j->h_order);
+    host_free(j->h_order);
     j->d_tasks = nullptr;
     j->h_tasks = nullptr;
     j->d_order = nullptr;
@@ -453,8 +525,8 @@ int ensure_tasks(ccsdt_job j, int64_t n)
     int rc = dev_alloc((void **)&j->d_tasks, (size_t)n * sizeof(Task));
     if (rc) return rc;
     if ((rc = dev_alloc((void **)&j->d_order, (size_t)n * sizeof(int)))) return rc;
-    HIP_TRY(hipHostMalloc((void **)&j->h_tasks, (size_t)n * sizeof(Task), 0));
-    HIP_TRY(hipHostMalloc((void **)&j->h_order, (size_t)n * sizeof(int), 0));
+    HIP_TRY(host_alloc((void **)&j->h_tasks, (size_t)n * sizeof(Task)));
+    HIP_TRY(host_alloc((void **)&j->h_order, (size_t)n * sizeof(int)));
     j->cap_tasks = n;
     return CCSDT_OK;
 }
@@ -495,21 +567,21 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, bool ten
             return set_error(CCSDT_E_DEVICE, "%s failed: %s", #x, hipGetErrorString(e_));      \
         }                                                                                      \
     } while (0)
-    JOB_HIP(hipStreamCreate(&j->st));
+    JOB_HIP(stream_get(&j->st));
     for (hipEvent_t &e : j->ev) JOB_HIP(hipEventCreate(&e));
     JOB_TRY(dev_alloc((void **)&j->d_raw, (size_t)j->nraw * sizeof(double)));
-    if (staged) JOB_HIP(hipHostMalloc((void **)&j->h_stage, (size_t)j->nraw * sizeof(double), 0));
-    JOB_HIP(hipHostMalloc((void **)&j->h_vec, (size_t)7 * g.nvp * sizeof(double), 0));
-    JOB_HIP(hipHostMalloc((void **)&j->h_out, 64 * sizeof(double), 0));
+    if (staged) JOB_HIP(host_alloc((void **)&j->h_stage, (size_t)j->nraw * sizeof(double)));
+    JOB_HIP(host_alloc((void **)&j->h_vec, (size_t)7 * g.nvp * sizeof(double)));
+    JOB_HIP(host_alloc((void **)&j->h_out, 64 * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_ev, (size_t)g.nvp * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_vec6, (size_t)6 * g.nvp * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_out2, 8 * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_jobdev, sizeof(JobDev)));
     JOB_TRY(dev_alloc((void **)&j->d_seg, sizeof(Segment)));
     JOB_TRY(dev_alloc((void **)&j->d_hdr, sizeof(WorkHdr)));
-    JOB_HIP(hipHostMalloc((void **)&j->h_jobdev, sizeof(JobDev), 0));
-    JOB_HIP(hipHostMalloc((void **)&j->h_seg, sizeof(Segment), 0));
-    JOB_HIP(hipHostMalloc((void **)&j->h_hdr, sizeof(WorkHdr), 0));
+    JOB_HIP(host_alloc((void **)&j->h_jobdev, sizeof(JobDev)));
+    JOB_HIP(host_alloc((void **)&j->h_seg, sizeof(Segment)));
+    JOB_HIP(host_alloc((void **)&j->h_hdr, sizeof(WorkHdr)));
     j->npanels = (int)job_npanels(g.nocc, j->kchunk, tensors);
     JOB_TRY(dev_alloc((void **)&j->d_pan, (size_t)j->npanels * g.vecsz * sizeof(double)));
 
@@ -812,10 +884,10 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         fill_tables(j, sg, n, fouts != nullptr, order == CCSDT_ORDER_PAIR);
         if (use_graph) {
             if (terms && n > j->cap_hterms) {
-                if (j->h_terms) (void)hipHostFree(j->h_terms);
+                host_free(j->h_terms);
                 j->h_terms = nullptr;
                 j->cap_hterms = 0;
-                HIP_TRY(hipHostMalloc((void **)&j->h_terms, (size_t)4 * n * sizeof(double), 0));
+                HIP_TRY(host_alloc((void **)&j->h_terms, (size_t)4 * n * sizeof(double)));
                 j->cap_hterms = n;
             }
             // the signature, and every buffer the captured nodes reference (a run
@@ -1165,7 +1237,7 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
         if (!rc) rc = grow(&j->d_terms, &j->cap_terms, 4);
         if (!rc) rc = ensure_tasks(j, 1);
         if (!rc && !j->h_fstage &&
-            hipHostMalloc((void **)&j->h_fstage, (size_t)8 * vv * sizeof(double), 0) != hipSuccess) {
+            host_alloc((void **)&j->h_fstage, (size_t)8 * vv * sizeof(double)) != hipSuccess) {
             j->h_fstage = nullptr;
             rc = set_error(CCSDT_E_NOMEM, "pinned staging allocation failed");
         }
