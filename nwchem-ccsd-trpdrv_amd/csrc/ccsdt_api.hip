@@ -149,7 +149,7 @@ void dev_free(void *p)
 // pinned memory (hipHostFree) and destroying streams can wait for the device,
 // which in a serving loop would stall every job's completion behind the
 // kernels in flight.
-constexpr size_t kHostCacheMax = size_t(2) << 30;
+constexpr size_t kHostCacheMax = size_t(8) << 30;
 std::mutex g_host_mu;
 std::multimap<size_t, void *> g_host_free;
 std::map<void *, size_t> g_host_live;
@@ -161,8 +161,10 @@ hipError_t host_alloc(void **p, size_t bytes)
     const size_t sz = (bytes + 4095) & ~size_t(4095);
     {
         std::lock_guard<std::mutex> lk(g_host_mu);
+        // pinning is the expensive part, so a cached block up to twice the
+        // request is reused
         auto it = g_host_free.lower_bound(sz);
-        if (it != g_host_free.end() && it->first <= sz + sz / 4) {
+        if (it != g_host_free.end() && it->first <= 2 * sz) {
             *p = it->second;
             g_host_live[*p] = it->first;
             g_host_cached -= it->first;
@@ -1288,7 +1290,10 @@ struct SJob {
     int64_t id = 0;
     ccsdt_job job = nullptr;
     ccsdt_job_desc desc{};
-    std::vector<std::vector<double>> host;  // owned copies of the arrays while waiting
+    // owned copies of the arrays until admission, in pinned memory so the
+    // upload is a direct DMA (array q at hbuf + hoff[q])
+    double *hbuf = nullptr;
+    int64_t hoff[CCSDT_NARRAYS] = {};
     std::vector<int> trip;
     int64_t n = 0, next = 0, done = 0;
     int slot = -1;
@@ -1438,11 +1443,11 @@ void server_admission_loop(ccsdt_server s)
         s->admit_q.pop_front();
         lk.unlock();
         const double *arr[CCSDT_NARRAYS];
-        for (int q = 0; q < CCSDT_NARRAYS; ++q) arr[q] = sj->host[q].data();
+        for (int q = 0; q < CCSDT_NARRAYS; ++q) arr[q] = sj->hbuf + sj->hoff[q];
         const int rc = server_admit(s, sj, arr);
         lk.lock();
-        sj->host.clear();
-        sj->host.shrink_to_fit();
+        host_free(sj->hbuf);
+        sj->hbuf = nullptr;
         --s->admitting;
         if (rc) {
             server_fail(s, sj, rc);
@@ -1643,9 +1648,17 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
     // admission thread uploads them
     int64_t cnt[CCSDT_NARRAYS];
     array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt, tensors);
-    sj->host.resize(CCSDT_NARRAYS);
+    int64_t tot = 0;
+    for (int q = 0; q < CCSDT_NARRAYS; ++q) {
+        sj->hoff[q] = tot;
+        tot += round_up(cnt[q], 32);
+    }
+    if (host_alloc((void **)&sj->hbuf, (size_t)std::max<int64_t>(tot, 1) * sizeof(double)) != hipSuccess) {
+        delete sj;
+        return set_error(CCSDT_E_NOMEM, "pinned copy of the job's arrays failed");
+    }
     for (int q = 0; q < CCSDT_NARRAYS; ++q)
-        if (cnt[q] > 0) sj->host[q].assign(desc->arrays[q], desc->arrays[q] + cnt[q]);
+        if (cnt[q] > 0) memcpy(sj->hbuf + sj->hoff[q], desc->arrays[q], (size_t)cnt[q] * sizeof(double));
     std::lock_guard<std::mutex> lk(s->mu);
     sj->id = s->next_id++;
     s->jobs[sj->id] = sj;
@@ -1909,6 +1922,7 @@ int ccsdt_server_destroy(ccsdt_server s)
     if (s->st) (void)hipStreamSynchronize(s->st);
     for (auto &kv : s->jobs) {
         SJob *sj = kv.second;
+        host_free(sj->hbuf);
         if (sj->job) {
             (void)hipStreamSynchronize(sj->job->st);
             job_free(sj->job);
