@@ -486,57 +486,9 @@ hipError_t launch_fused(const JobDev *jobs, const Segment *segs, WorkHdr *hdr, i
 // per level L = s+q in INT32, then recombined in FP64 with the power-of-two
 // scales.  The K = nocc half and the energy epilogue stay FP64.
 
-// One wave per panel row: row maximum, exponent, digits, and the FP64 tail.
-// Digit plane layout per 16-row block xb and 64-wide K step ks: lane l holds 16
-// bytes for x = 16*xb + l%16, k = 64*ks + 16*(l/16) + j (j = byte), which is the
-// operand layout of v_mfma_i32_16x16x64_i8 (row = lane%16, 16 K bytes per lane).
-template <int S>
-__global__ __launch_bounds__(64) void k_split_pack(const PanelDesc *__restrict__ descs, Geom g,
-                                                  SplitGeom sg, int8_t *__restrict__ digits,
-                                                  double *__restrict__ scales,
-                                                  double *__restrict__ tail)
-{
-    const int pnl = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
-    const PanelDesc d = descs[pnl];
-    const int nv = g.nvir, no = g.nocc;
-    auto val = [&](int k) -> double {
-        if (x >= nv || k >= nv) return 0.0;
-        return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
-    };
-    double m = 0.0;
-    for (int k = lane; k < nv; k += 64) m = fmax(m, fabs(val(k)));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    int e = 0;
-    if (m > 0.0) (void)frexp(m * (128.0 / 127.0), &e);
-    const double inv = ldexp(1.0, -e);
-    if (lane == 0) scales[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
-    const int xb = x >> 4, xi = x & 15;
-    for (int k = lane; k < sg.kv; k += 64) {
-        double v = val(k) * inv;
-        const int64_t o = ((int64_t)(xb * sg.nkv + (k >> 6)) * 64 + ((k & 63) >> 4) * 16 + xi) * 16 + (k & 15);
-#pragma unroll
-        for (int q = 0; q < S; ++q) {
-            v *= 128.0;
-            const double dg = rint(v);
-            v -= dg;
-            digits[((int64_t)pnl * S + q) * sg.slice_bytes + o] = (int8_t)dg;
-        }
-    }
-    for (int kk = lane; kk < sg.kt; kk += 64) {
-        double t = 0.0;
-        if (x < nv && kk < no)
-            t = d.sgn * (d.second_t ? d.Q[kk + (int64_t)x * no] : d.Q[x + (int64_t)kk * nv]);
-        tail[(int64_t)pnl * sg.tail_sz + ((int64_t)(xb * sg.nkt + (kk >> 2)) * 64 + xi + 16 * (kk & 3))] = t;
-    }
-}
-
 // BF16x3 form: the row is scaled by 2^-e(x) so its largest entry is below 1,
 // then written as three BF16 planes h1 + h2 + h3 (h1 = RN(v), h2 = RN(v - h1),
-// h3 = RN(v - h1 - h2); the residual is below 2^-24 |v|).  Plane layout per
-// 16-row block xb and 32-wide K step ks: lane l holds 8 BF16 values (16 bytes)
-// for x = 16*xb + l%16, k = 32*ks + 8*(l/16) + j, the operand layout of
-// v_mfma_f32_16x16x32_bf16.  The FP64 tail is as in k_split_pack.
+// h3 = RN(v - h1 - h2); the residual is below 2^-24 |v|).
 __device__ __forceinline__ double bf16_rn(double v, uint16_t &bits)
 {
     // to FP32 by round-to-nearest, then to BF16 by round-to-nearest-even (the
@@ -547,43 +499,93 @@ __device__ __forceinline__ double bf16_rn(double v, uint16_t &bits)
     return (double)__builtin_bit_cast(float, r);
 }
 
-__global__ __launch_bounds__(64) void k_split_pack_bf16(const PanelDesc *__restrict__ descs, Geom g,
-                                                       SplitGeom sg, int8_t *__restrict__ planes,
-                                                       double *__restrict__ scales,
-                                                       double *__restrict__ tail)
+// Split-form packing, one workgroup (4 waves) per 16-row block of one panel.
+// Plane layout per 16-row block xb and K step ks of the MFMA (KS = 64 INT8
+// digits or 32 BF16 values): lane l holds 16 bytes for row x = 16*xb + l%16 at
+// K offset ks*KS + (KS/4)*(l/16) -- the operand layout of
+// v_mfma_i32_16x16x64_i8 and v_mfma_f32_16x16x32_bf16 -- so one K step of one
+// plane is 1 KiB.  Thread t of a step handles row r = (t/4)%16, K group
+// (t/4)/16 (one per wave) and a quarter of that lane's 16 bytes: its reads of
+// P are 128-byte runs whichever way P is transposed, and its 4-byte stores
+// are contiguous across the workgroup.  Pass 1 takes each row's maximum (lane
+// shuffles within a wave, then LDS across the 4 waves), pass 2 writes the
+// planes, then the FP64 tail (K = nocc) is written in the tail layout.
+template <int S, bool BF>
+__global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict__ descs, Geom g,
+                                                   SplitGeom sg, int8_t *__restrict__ planes,
+                                                   double *__restrict__ scales,
+                                                   double *__restrict__ tail)
 {
-    const int pnl = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
+    constexpr int KS = BF ? 32 : 64;     // K per MFMA step
+    constexpr int EPT = BF ? 2 : 4;      // values per thread per step (4 bytes per plane)
+    constexpr int NPL = BF ? 3 : S;      // planes
+    __shared__ double rmax[4][16];
+    const int pnl = blockIdx.y, xb = blockIdx.x, t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const int r = (lane >> 2) & 15, c = lane & 3;
+    const int x = xb * 16 + r;
     const PanelDesc d = descs[pnl];
     const int nv = g.nvir, no = g.nocc;
     auto val = [&](int k) -> double {
         if (x >= nv || k >= nv) return 0.0;
         return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
     };
+    // K index of value j of this thread in step ks
+    auto kidx = [&](int ks, int j) { return ks * KS + w * (KS / 4) + c * EPT + j; };
+
     double m = 0.0;
-    for (int k = lane; k < nv; k += 64) m = fmax(m, fabs(val(k)));
+    for (int ks = 0; ks < sg.nkv; ++ks)
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        for (int j = 0; j < EPT; ++j) m = fmax(m, fabs(val(kidx(ks, j))));
+    m = fmax(m, __shfl_xor(m, 1, 64));
+    m = fmax(m, __shfl_xor(m, 2, 64));
+    if (c == 0) rmax[w][r] = m;
+    __syncthreads();
+    m = fmax(fmax(rmax[0][r], rmax[1][r]), fmax(rmax[2][r], rmax[3][r]));
     int e = 0;
-    if (m > 0.0) (void)frexp(m, &e);  // m * 2^-e in [0.5, 1)
+    // INT8: largest entry at most 127/128 after scaling; BF16: below 1
+    if (m > 0.0) (void)frexp(BF ? m : m * (128.0 / 127.0), &e);
     const double inv = ldexp(1.0, -e);
-    if (lane == 0) scales[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
-    const int xb = x >> 4, xi = x & 15;
-    uint16_t *pl = reinterpret_cast<uint16_t *>(planes);
-    for (int k = lane; k < sg.kv; k += 64) {
-        double v = val(k) * inv;
-        const int64_t o = ((int64_t)(xb * sg.nkv + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + xi) * 8 + (k & 7);
+    if (w == 0 && c == 0) scales[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
+
+    uint32_t *pl = reinterpret_cast<uint32_t *>(planes);
+    const int64_t plane_words = sg.slice_bytes / 4;
+    for (int ks = 0; ks < sg.nkv; ++ks) {
+        uint32_t word[NPL];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            uint16_t b;
-            v -= bf16_rn(v, b);
-            pl[((int64_t)pnl * 3 + q) * (sg.slice_bytes / 2) + o] = b;
+        for (int q = 0; q < NPL; ++q) word[q] = 0;
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            double v = val(kidx(ks, j)) * inv;
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) {
+                if (BF) {
+                    uint16_t b;
+                    v -= bf16_rn(v, b);
+                    word[q] |= (uint32_t)b << (16 * j);
+                } else {
+                    v *= 128.0;
+                    const double dg = rint(v);
+                    v -= dg;
+                    word[q] |= (uint32_t)(uint8_t)(int8_t)dg << (8 * j);
+                }
+            }
         }
+        // byte offset of this thread's 4 bytes: step base + lane-group (w*16 + r)
+        // * 16 + c * 4
+        const int64_t o = ((int64_t)(xb * sg.nkv + ks) * 1024 + (w * 16 + r) * 16 + c * 4) / 4;
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) pl[((int64_t)pnl * NPL + q) * plane_words + o] = word[q];
     }
-    for (int kk = lane; kk < sg.kt; kk += 64) {
-        double t = 0.0;
-        if (x < nv && kk < no)
-            t = d.sgn * (d.second_t ? d.Q[kk + (int64_t)x * no] : d.Q[x + (int64_t)kk * nv]);
-        tail[(int64_t)pnl * sg.tail_sz + ((int64_t)(xb * sg.nkt + (kk >> 2)) * 64 + xi + 16 * (kk & 3))] = t;
+    // FP64 tail: element e = step*64 + l with l = x%16 + 16*(k%4)
+    double *tp = tail + (int64_t)pnl * sg.tail_sz + (int64_t)xb * sg.nkt * 64;
+    for (int e2 = t; e2 < sg.nkt * 64; e2 += 256) {
+        const int kk = 4 * (e2 >> 6) + ((e2 & 63) >> 4);
+        const int xx = xb * 16 + (e2 & 15);
+        double v = 0.0;
+        if (xx < nv && kk < no)
+            v = d.sgn * (d.second_t ? d.Q[kk + (int64_t)xx * no] : d.Q[xx + (int64_t)kk * nv]);
+        tp[e2] = v;
     }
 }
 
@@ -591,17 +593,17 @@ hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g
                              const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
                              hipStream_t st)
 {
-    dim3 grid(g.nvp, npanels), block(64);
+    dim3 grid(g.nb, npanels), block(256);
     if (sg.bf16) {
-        hipLaunchKernelGGL(k_split_pack_bf16, grid, block, 0, st, d_desc, g, sg, digits, scales, tail);
+        hipLaunchKernelGGL((k_split_pack<3, true>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail);
         return hipGetLastError();
     }
     switch (sg.slices) {
-    case 3: hipLaunchKernelGGL(k_split_pack<3>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
-    case 4: hipLaunchKernelGGL(k_split_pack<4>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
-    case 5: hipLaunchKernelGGL(k_split_pack<5>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
-    case 6: hipLaunchKernelGGL(k_split_pack<6>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
-    case 7: hipLaunchKernelGGL(k_split_pack<7>, grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 3: hipLaunchKernelGGL((k_split_pack<3, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 4: hipLaunchKernelGGL((k_split_pack<4, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 5: hipLaunchKernelGGL((k_split_pack<5, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 6: hipLaunchKernelGGL((k_split_pack<6, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
+    case 7: hipLaunchKernelGGL((k_split_pack<7, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
