@@ -240,7 +240,9 @@ int ccsdt_job_destroy(ccsdt_job job);
  * launch configuration of the job's triple list -- item order x tile variant x
  * schedule (one workgroup per item, or persistent grids at the occupancy limit
  * and at half of it) --
- * with base's mode, slices and graph flag, and writes the fastest to *best.
+ * with base's mode, slices and graph flag, and writes the fastest to *best
+ * (candidates within 1% of the fastest count as equal; the first of them in the
+ * order order x tile x schedule is taken, so repeated tunings agree).
  * Each candidate runs once untimed, then `reps` times; its time is the median
  * of ms_total.  table (may be NULL) receives up to cap entries, *count the
  * number of candidates tried.  Candidates the device rejects are recorded with

@@ -1004,6 +1004,8 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
 // E[T] agreement with the FP64 form the slice-count choice asks for: ten times
 // under the split form's 1e-8 bound, since the tuning list is a sample.
 constexpr double kSliceTol = 1e-9;
+// Relative time difference below which two launch configurations count as equal.
+constexpr double kTuneTie = 0.01;
 
 // Times every launch configuration of one triple list (item order x tile
 // variant x schedule) and keeps the fastest.  The schedules are one workgroup
@@ -1065,6 +1067,7 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     }
     int nbest = -1;
     double tbest = 0.0;
+    std::vector<double> meds(cand.size(), -1.0);
     for (size_t c = 0; c < cand.size(); ++c) {
         std::vector<double> ms;
         ccsdt_run_result r{};
@@ -1081,6 +1084,7 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
                 nbest = (int)c;
                 tbest = med;
             }
+            meds[c] = med;
         } else if (rc == CCSDT_E_ARG || rc == CCSDT_E_STATE) {
             return rc;  // the list or the options are wrong, not the candidate
         }
@@ -1093,6 +1097,13 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     }
     if (count) *count = (int)cand.size();
     if (nbest < 0) return rc ? rc : set_error(CCSDT_E_DEVICE, "no launch configuration ran");
+    // candidates within kTuneTie of the fastest are timing noise apart: take the
+    // first of them in enumeration order, so repeated tunings agree
+    for (size_t c = 0; c < cand.size(); ++c)
+        if (meds[c] > 0.0 && meds[c] <= tbest * (1.0 + kTuneTie)) {
+            nbest = (int)c;
+            break;
+        }
     *best = cand[nbest];
     return CCSDT_OK;
 }

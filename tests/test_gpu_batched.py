@@ -123,10 +123,11 @@ def test_autotune_picks_a_working_candidate(cc, mode, slices):
     assert variants == ({0, 1} if mode == "fp64" else {0})
     assert any(row["persistent"] for row in table) and any(not row["persistent"] for row in table)
     assert all(row["status"] == 0 and row["ms"] > 0 for row in table)
-    assert min(row["ms"] for row in table) == min(
-        row["ms"] for row in table if row["variant"] == best["variant"]
-        and row["persistent"] == best["persistent"] and row["wg_per_cu"] == best["wg_per_cu"]
-        and row["order"] == best["order"])
+    # the pick is within the 1% tie band of the fastest candidate
+    picked = [row["ms"] for row in table if row["variant"] == best["variant"]
+              and row["persistent"] == best["persistent"] and row["wg_per_cu"] == best["wg_per_cu"]
+              and row["order"] == best["order"]]
+    assert len(picked) == 1 and picked[0] <= 1.01 * min(row["ms"] for row in table)
     assert {row["order"] for row in table} == {0, 1}
     assert best["graph"]
     tol = 1e-12 if mode == "fp64" else 1e-9
