@@ -173,3 +173,20 @@ def test_pair_major_order_is_bitwise_task_order(cc, mode, slices, variant):
             got = job.run(trip, mode=mode, slices=slices, variant=variant, terms=True, order=1, **kw)
             assert np.array_equal(got["terms"], ref["terms"]), kw
             assert got["emp4"] == ref["emp4"] and got["emp5"] == ref["emp5"]
+
+
+def test_batched_production_shape_sampled_oracle(cc, orc):
+    """nocc=40, nvir=400 with uneven k blocks: all 11480 triples in one
+    pair-major persistent launch; 12 triples spread over the list checked
+    against the oracle term by term (1e-11), the split form's E[T] against the
+    FP64 form's (1e-8)."""
+    arrays, blocks, trip, kchunk = cc.config_case(3)
+    with cc.Job(arrays, 40, 400, kchunk) as job:
+        f = job.run(trip, terms=True, order=1, persistent=True, graph=True)
+        s = job.run(trip, mode="ozaki_int8", slices=4, order=1)
+    pick = np.linspace(0, len(trip) - 1, 12).astype(int)
+    ref = orc.run_list(arrays, 0, 40, 400, trip[pick], with_abs=True)
+    scale = np.maximum(np.abs(ref["terms"]), ref["abs4"])
+    assert np.max(np.abs(f["terms"][pick] - ref["terms"]) / scale) < 1e-11
+    E = f["emp4"] + f["emp5"]
+    assert abs(s["emp4"] + s["emp5"] - E) <= 1e-8 * abs(E)
