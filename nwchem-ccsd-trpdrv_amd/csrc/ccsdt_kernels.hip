@@ -658,12 +658,14 @@ struct PlaneMath<true> {
     static __device__ __forceinline__ double weight(int) { return 1.0; }
 };
 
+// stash: this wave's 8 KiB of LDS, holding group 0's four products while group
+// 1's digit products run, so their 32 VGPRs are free during the INT8 loops.
 template <int S, int W, bool BF = false>
 __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
                                            const __amdgpu_buffer_rsrc_t rt,
                                            const double *__restrict__ scales, const Geom &g,
                                            const SplitGeom &sg, const PanelPairs &pp, int bi,
-                                           int ci, int lane)
+                                           int ci, int lane, double *__restrict__ stash)
 {
     // panel ids: pair q holds panels 2q, 2q+1
     const int LJ = 2 * pp.li, LK = LJ + 1, LJT = 2 * pp.lk, LKT = LJT + 1;
@@ -731,10 +733,18 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
 #pragma unroll
                 for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * PlaneMath<BF>::weight(q);
                 const double sA = scales[(int64_t)pa * nvp + bi * 16 + row0 + r];
-                facc[lp][r] += v * sA * sB;
+                const double f = v * sA * sB;
+                if (gi == 0)
+                    stash[(lp * 4 + r) * 64 + lane] = f;  // lp = 0..3
+                else
+                    facc[lp][r] = f;
             }
         }
     }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) facc[p][r] = stash[(p * 4 + r) * 64 + lane];
 
     // FP64 tail (K = nocc) of the eight products
     const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
@@ -917,9 +927,11 @@ template <typename T, bool TENGY, bool WRITEF>
 __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
                                               const typename Vec4<T>::type (&facc)[8], int bi, int ci,
                                               int w, int lane, double *__restrict__ part4,
-                                              double *__restrict__ fo)
+                                              double *__restrict__ fo, void *lds)
 {
-    __shared__ T xch[2][8][2][64];
+    // the workgroup's 16 KiB exchange buffer (each wave's half first held its
+    // stash, consumed before the products were finished)
+    T(*xch)[8][2][64] = reinterpret_cast<T(*)[8][2][64]>(lds);
     __shared__ double red[2][4];
     const Geom &g = jd.g;
     const double *__restrict__ ev = jd.ev;
@@ -1026,6 +1038,7 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
                                           double *__restrict__ part4, double *__restrict__ fo)
 {
     const bool diag = (bi == ci);
+    __shared__ double lds_x[2 * 8 * 2 * 64];
     if constexpr (S == kSlicesFp32) {
         flt4 f32[8];
 #pragma unroll
@@ -1035,7 +1048,7 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
             fp32_wave<0>(f32, pan32, jd.g, pp, bi, ci, lane);
         else if (!diag)
             fp32_wave<1>(f32, pan32, jd.g, pp, bi, ci, lane);
-        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo);
+        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo, lds_x);
     } else {
         dbl4 facc[8];
 #pragma unroll
@@ -1054,11 +1067,11 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
                                                           (uint32_t)(sg.npanels * NP * sg.slice_bytes));
             const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
             if (w == 0)
-                split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+                split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x);
             else if (!diag)
-                split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane);
+                split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x + 1024);
         }
-        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo);
+        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo, lds_x);
     }
 }
 
