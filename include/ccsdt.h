@@ -274,17 +274,39 @@ int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t nt
  *   Jia = jv + i*vv                 Jka = jv + k*vv        (Kia, Kka from kv alike)
  *   Tia = t1o + i*ov                Tka = t1o + k*ov       (Xia, Xka from xo alike)
  *   Jij = jo + (j*nocc + i)*ov      Jkj = jo + (j*nocc + k)*ov  (Kij, Kkj from ko alike)
- * eorb and the six nvir vectors are as in ccsdt_job_desc.  Triples are run
- * with ccsdt_job_run; their klo is checked (1 <= klo <= k) and otherwise unused. */
+ * eorb and the six nvir vectors are as in ccsdt_job_desc (the same six for
+ * every triple, as the harness passes them, omp/test.c:141-146,167-176).
+ * Triples are run with ccsdt_job_run; their klo is checked (1 <= klo <= k)
+ * and otherwise unused.
+ *
+ * Driver tensors (optional: t1, dja, djx all set, or all null).  The reference
+ * allocates the driver's Dja, Djka, Djia (omp/test.c:137-139) but never uses
+ * them; the six vectors of each triple are sliced from them by NWChem's
+ * driver, which is not in the reference.  Given the driver tensors, the six
+ * vector pointers above are not read and each triple reads, with rows of nvir
+ * doubles and 0-based i, j, k:
+ *   t1v1   = t1  + i*nvir                 t1v2   = t1  + k*nvir
+ *   dintc1 = dja + (j*nocc + k)*nvir      dintc2 = dja + (j*nocc + i)*nvir
+ *   dintx1 = djx + (j*nocc + k)*nvir      dintx2 = djx + (j*nocc + i)*nvir
+ * so that for one j, dja + j*nocc*nvir has the harness's Dja shape (nvir x
+ * nocc), djx + j*nocc*nvir its Djia, and that block's columns klo..khi its Djka
+ * (nvir x kchunk).  This pairing (emp5i: t1 of i with integrals over j, k;
+ * emp5k: t1 of k with integrals over j, i) keeps the i <-> k exchange symmetry
+ * of omp/ccsd_tengy_c99.c:106-111; no reference code fixes it.  Nothing is
+ * copied per triple: the kernels read the rows in place. */
 typedef struct {
     int ncor, nocc, nvir, pad;
     const double *eorb;
     const double *t2, *jv, *kv, *t1o, *xo, *jo, *ko;
     const double *dintc1, *dintx1, *t1v1, *dintc2, *dintx2, *t1v2;
+    const double *t1;   /* nocc*nvir: [m][c] */
+    const double *dja;  /* nocc*nocc*nvir: [j][m][c] */
+    const double *djx;  /* nocc*nocc*nvir: [j][m][c] */
 } ccsdt_tensor_desc;
 
 int ccsdt_job_create_tensors(const ccsdt_tensor_desc *desc, ccsdt_job *out);
-int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriples);
+/* driver = 1 for a job given the driver tensors. */
+int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriples, int driver);
 
 /* ---------------------------------------------------------------- serving */
 /* No counterpart in the reference (whose harness runs one job, omp/test.c).  A

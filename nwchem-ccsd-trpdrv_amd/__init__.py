@@ -38,6 +38,10 @@ TENSOR_STREAM = {"t2": STREAM["tij"], "jv": STREAM["jia"], "kv": STREAM["kia"],
                  "t1o": STREAM["tia"], "xo": STREAM["xia"], "jo": STREAM["jij"],
                  "ko": STREAM["kij"]}
 VECTOR_NAMES = ["dintc1", "dintx1", "t1v1", "dintc2", "dintx2", "t1v2"]
+# Driver tensors of a full-tensor job (optional): t1 [m][c], dja and djx
+# [j][m][c]; the six vectors of each triple are rows of them (ccsdt_tensor_desc).
+DRIVER_NAMES = ["t1", "dja", "djx"]
+DRIVER_STREAM = {"t1": STREAM["t1v1"], "dja": STREAM["dja"], "djx": STREAM["djia"]}
 
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3, "fp32": 4}
 MODE_NAMES = {v: k for k, v in MODES.items()}
@@ -63,14 +67,24 @@ class TensorDesc(ctypes.Structure):
     _fields_ = ([("ncor", ctypes.c_int), ("nocc", ctypes.c_int), ("nvir", ctypes.c_int),
                  ("pad", ctypes.c_int), ("eorb", ctypes.c_void_p)]
                 + [(n, ctypes.c_void_p) for n in TENSOR_NAMES]
-                + [(n, ctypes.c_void_p) for n in VECTOR_NAMES])
+                + [(n, ctypes.c_void_p) for n in VECTOR_NAMES]
+                + [(n, ctypes.c_void_p) for n in DRIVER_NAMES])
+
+
+def has_driver_tensors(t):
+    return any(t.get(name) is not None for name in DRIVER_NAMES)
 
 
 def _tensor_desc(t, nocc, nvir, ncor):
-    """TensorDesc over contiguous float64 copies of t (returned to keep alive)."""
+    """TensorDesc over contiguous float64 copies of t (returned to keep alive).
+    With driver tensors the six vectors are left null (they are not read)."""
     d = TensorDesc(ncor, nocc, nvir, 0)
     keep = []
-    for name in ["eorb"] + TENSOR_NAMES + VECTOR_NAMES:
+    names = ["eorb"] + TENSOR_NAMES
+    names += DRIVER_NAMES if has_driver_tensors(t) else VECTOR_NAMES
+    for name in names:
+        if t.get(name) is None:
+            continue
         a = np.ascontiguousarray(t[name], dtype=np.float64)
         keep.append(a)
         setattr(d, name, a.ctypes.data)
@@ -154,7 +168,7 @@ def lib():
                                                   ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
             "ccsdt_job_footprint": (i64, [ctypes.c_int] * 4 + [i64]),
             "ccsdt_job_create_tensors": (ctypes.c_int, [ctypes.POINTER(TensorDesc), ctypes.POINTER(vp)]),
-            "ccsdt_tensor_job_footprint": (i64, [ctypes.c_int] * 3 + [i64]),
+            "ccsdt_tensor_job_footprint": (i64, [ctypes.c_int] * 3 + [i64, ctypes.c_int]),
             "ccsdt_server_submit_tensors": (ctypes.c_int, [vp, ctypes.POINTER(TensorDesc), vp, i64,
                                                            ctypes.POINTER(i64)]),
             "ccsdt_selftest_mfma": (ctypes.c_int, [vp, vp, vp]),
@@ -260,11 +274,13 @@ def make_inputs(nocc, nvir, kchunk, seed, ncor=0, illcond=False):
     return arrays
 
 
-def make_tensor_inputs(nocc, nvir, seed, ncor=0, illcond=False):
+def make_tensor_inputs(nocc, nvir, seed, ncor=0, illcond=False, driver=False):
     """Seeded full-tensor job inputs (see ccsdt_tensor_desc): eorb as in
     make_inputs; t2 (nocc*nocc*nvir^2), jv, kv (nocc*nvir^2), t1o, xo
     (nocc*nocc*nvir), jo, ko (nocc*nocc*nocc*nvir) and the six nvir vectors, all
-    U(-0.05,0.05).  ``illcond`` scales jv, kv, jo, ko by 10**u, u ~ U(-6,0)."""
+    U(-0.05,0.05).  ``illcond`` scales jv, kv, jo, ko by 10**u, u ~ U(-6,0).
+    ``driver`` replaces the six vectors by the driver tensors t1 (nocc*nvir),
+    dja and djx (nocc*nocc*nvir), U(-0.05,0.05)."""
     vv, ov = nvir * nvir, nocc * nvir
     base = make_inputs(nocc, nvir, 1, seed, ncor=ncor)
     sizes = {"t2": nocc * nocc * vv, "jv": nocc * vv, "kv": nocc * vv, "t1o": nocc * ov,
@@ -275,9 +291,30 @@ def make_tensor_inputs(nocc, nvir, seed, ncor=0, illcond=False):
         if illcond and name in ("jv", "kv", "jo", "ko"):
             e = pcg_uniform(seed, ILLCOND_STREAM_BASE + TENSOR_STREAM[name], -6.0, 0.0, n)
             t[name] = t[name] * np.power(10.0, e)
+    if driver:
+        sizes = {"t1": nocc * nvir, "dja": nocc * nocc * nvir, "djx": nocc * nocc * nvir}
+        for name, n in sizes.items():
+            t[name] = pcg_uniform(seed, DRIVER_STREAM[name], -0.05, 0.05, n)
+        return t
     for name in VECTOR_NAMES:
         t[name] = base[name][:nvir].copy()
     return t
+
+
+def triple_vectors(t, nocc, nvir, i, j, k):
+    """The six nvir vectors triple (i,j,k) (1-based) reads: the job's fixed
+    ones, or rows of the driver tensors (ccsdt_tensor_desc):
+    t1v1 = t1[i], t1v2 = t1[k], dintc1 = dja[j][k], dintc2 = dja[j][i],
+    dintx1 = djx[j][k], dintx2 = djx[j][i] (0-based rows)."""
+    if not has_driver_tensors(t):
+        return {name: np.asarray(t[name])[:nvir] for name in VECTOR_NAMES}
+    i0, j0, k0 = i - 1, j - 1, k - 1
+
+    def row(name, r):
+        return np.asarray(t[name])[r * nvir:(r + 1) * nvir]
+    return {"t1v1": row("t1", i0), "t1v2": row("t1", k0),
+            "dintc1": row("dja", j0 * nocc + k0), "dintc2": row("dja", j0 * nocc + i0),
+            "dintx1": row("djx", j0 * nocc + k0), "dintx2": row("djx", j0 * nocc + i0)}
 
 
 def gather_triple(t, nocc, nvir, i, j, k):
@@ -301,14 +338,14 @@ def gather_triple(t, nocc, nvir, i, j, k):
     # the first nocc*nvir (or nvir) elements
     for name in ("tia", "xia", "jij", "kij"):
         a[name] = np.concatenate([a[name], np.zeros(ov * (nocc - 1))])
-    for name in VECTOR_NAMES:
-        a[name] = np.concatenate([t[name], np.zeros(vv - nvir)])
+    for name, v in triple_vectors(t, nocc, nvir, i, j, k).items():
+        a[name] = np.concatenate([v, np.zeros(vv - nvir)])
     return a
 
 
-def tensor_footprint(nocc, nvir, ntriples, ncor=0):
+def tensor_footprint(nocc, nvir, ntriples, ncor=0, driver=False):
     """Device bytes of a full-tensor job (ccsdt_tensor_job_footprint)."""
-    return lib().ccsdt_tensor_job_footprint(ncor, nocc, nvir, ntriples)
+    return lib().ccsdt_tensor_job_footprint(ncor, nocc, nvir, ntriples, 1 if driver else 0)
 
 
 def blocks_uniform(nocc, length):

@@ -282,12 +282,20 @@ SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = fa
     return sg;
 }
 
+// Kinds of job: harness-shaped arrays (ccsdt_job_create), full tensors with one
+// set of the six vectors, and full tensors with the driver tensors t1, dja, djx
+// (ccsdt_job_create_tensors).
+enum { JOB_HARNESS = 0, JOB_TENSORS = 1, JOB_TENSORS_DRIVER = 2 };
+// Array slots of the driver tensors in a full-tensor job's descriptor (k-indexed
+// slots, otherwise unused by such a job).
+enum { TS_T1 = CCSDT_TKA, TS_DJA = CCSDT_JKJ, TS_DJX = CCSDT_KKJ };
+
 // Element counts of the job arrays (omp/test.c:112-146 shapes; tia, xia, jij,
 // kij and the dint/t1v vectors at the lengths the body reads).
-void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n, bool tensors = false)
+void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n, int kind = JOB_HARNESS)
 {
     const int64_t vv = (int64_t)nvir * nvir, ov = (int64_t)nocc * nvir;
-    if (tensors) {
+    if (kind != JOB_HARNESS) {
         // full-tensor job: the i-indexed slots hold every occupied index, the
         // k-indexed slots are unused (ccsdt_tensor_desc)
         for (int q = 0; q < CCSDT_NARRAYS; ++q) n[q] = 0;
@@ -299,7 +307,13 @@ void array_sizes(int ncor, int nocc, int nvir, int kchunk, int64_t *n, bool tens
         n[CCSDT_XIA] = nocc * ov;
         n[CCSDT_JIJ] = (int64_t)nocc * nocc * ov;
         n[CCSDT_KIJ] = (int64_t)nocc * nocc * ov;
-        for (int q = CCSDT_DINTC1; q <= CCSDT_T1V2; ++q) n[q] = nvir;
+        if (kind == JOB_TENSORS_DRIVER) {
+            n[TS_T1] = ov;
+            n[TS_DJA] = nocc * ov;
+            n[TS_DJX] = nocc * ov;
+        } else {
+            for (int q = CCSDT_DINTC1; q <= CCSDT_T1V2; ++q) n[q] = nvir;
+        }
         return;
     }
     n[CCSDT_EORB] = ncor + nocc + nvir;
@@ -326,24 +340,40 @@ constexpr int64_t kFWorkspaceBytes = int64_t(4) << 30;
 
 // Panels of a job: 4 job-wide + 4 per k slice (harness shape), or the L(p)
 // and R(p, j) pair families of a full-tensor job.
-int64_t job_npanels(int nocc, int kchunk, bool tensors)
+int64_t job_npanels(int nocc, int kchunk, int kind)
 {
-    return tensors ? 2 * ((int64_t)nocc + (int64_t)nocc * nocc) : PAN_SLICE0 + 4 * (int64_t)kchunk;
+    return kind != JOB_HARNESS ? 2 * ((int64_t)nocc + (int64_t)nocc * nocc)
+                               : PAN_SLICE0 + 4 * (int64_t)kchunk;
+}
+
+// Rows of the vector pool: the six vectors, or t1 (nocc rows), dja and djx
+// (nocc*nocc rows each) of a job with driver tensors.
+int64_t job_vrows(int nocc, int kind)
+{
+    return kind == JOB_TENSORS_DRIVER ? (int64_t)nocc + 2 * (int64_t)nocc * nocc : 6;
+}
+
+bool is_driver_slot(int kind, int q)
+{
+    return kind == JOB_TENSORS_DRIVER && (q == TS_T1 || q == TS_DJA || q == TS_DJX);
 }
 
 // ------------------------------------------------------------------ job
-// Device bytes of a job: arrays, packed panels, vectors, and per triple the
-// partials, terms, task record and order entry.
-int64_t job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples, bool tensors)
+// Device bytes of a job: arrays (the driver tensors live only in the vector
+// pool), packed panels, eorb and the vector pool, and per triple the partials,
+// terms, task record and order entry.
+int64_t job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples, int kind)
 {
     const Geom g = make_geom(ncor, nocc, nvir);
     int64_t cnt[CCSDT_NARRAYS];
-    array_sizes(ncor, nocc, nvir, kchunk, cnt, tensors);
+    array_sizes(ncor, nocc, nvir, kchunk, cnt, kind);
     int64_t raw = 0;
-    for (int q = 0; q < CCSDT_NARRAYS; ++q) raw += round_up(cnt[q], 32);
-    const int64_t pan = job_npanels(nocc, kchunk, tensors) * g.vecsz;
+    for (int q = 0; q < CCSDT_NARRAYS; ++q)
+        if (!is_driver_slot(kind, q)) raw += round_up(cnt[q], 32);
+    const int64_t pan = job_npanels(nocc, kchunk, kind) * g.vecsz;
     const int64_t per_task = (int64_t)g.npairs * 4 + 4 + (int64_t)(sizeof(Task) + sizeof(int)) / 8 + 1;
-    return (raw + pan + 7 * g.nvp + ntriples * per_task) * (int64_t)sizeof(double);
+    return (raw + pan + (1 + job_vrows(nocc, kind)) * g.nvp + ntriples * per_task) *
+           (int64_t)sizeof(double);
 }
 
 // entries of a captured graph's signature (job_run)
@@ -358,15 +388,17 @@ struct ccsdt_job_s {
     int64_t cnt[CCSDT_NARRAYS] = {};
     double *d_raw = nullptr;
     double *h_stage = nullptr;  // pinned staging for uploads
-    double *h_vec = nullptr;    // pinned staging for the 7 padded vectors
+    double *h_vec = nullptr;    // pinned staging for eorb and the vector pool
     double *h_fstage = nullptr; // pinned staging for f matrices (drop-in energy pass)
     double *d_ev = nullptr;     // virtual orbital energies, padded to nvp
-    double *d_vec6 = nullptr;   // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2, each nvp
+    double *d_vec6 = nullptr;   // vector pool: vrows rows of nvp doubles (Task::vrow)
+    int64_t vrows = 6;
     double *d_pan = nullptr;
     PanelDesc *d_desc = nullptr;
     int npanels = 0;
     short2 *d_pairs = nullptr;
     bool tensors = false;  // full-tensor job (ccsdt_job_create_tensors)
+    int kind = JOB_HARNESS;
     Task *d_tasks = nullptr;
     Task *h_tasks = nullptr;
     int *d_order = nullptr;  // tasks sorted by k slice (pair-major execution order)
@@ -450,7 +482,7 @@ int grow(double **p, int64_t *cap, int64_t need)
     return CCSDT_OK;
 }
 
-int check_desc(const ccsdt_job_desc *d, bool tensors = false)
+int check_desc(const ccsdt_job_desc *d, int kind = JOB_HARNESS)
 {
     if (!d) return set_error(CCSDT_E_ARG, "null job descriptor");
     if (d->ncor < 0 || d->nocc < 1 || d->nvir < 1 || d->kchunk < 1 || d->kchunk > d->nocc)
@@ -459,28 +491,35 @@ int check_desc(const ccsdt_job_desc *d, bool tensors = false)
     if (d->nvir > 8192 || d->nocc > 4096)
         return set_error(CCSDT_E_ARG, "nvir=%d / nocc=%d beyond supported range", d->nvir, d->nocc);
     // the pack kernels put one panel per grid row (at most 65535)
-    if (job_npanels(d->nocc, d->kchunk, tensors) > 65535)
+    if (job_npanels(d->nocc, d->kchunk, kind) > 65535)
         return set_error(CCSDT_E_ARG, "nocc=%d gives %lld operand panels (at most 65535)", d->nocc,
-                         (long long)job_npanels(d->nocc, d->kchunk, tensors));
+                         (long long)job_npanels(d->nocc, d->kchunk, kind));
+    // the kernels address the vector pool with 32-bit offsets
+    if (job_vrows(d->nocc, kind) * round_up(d->nvir, 16) >= (int64_t(1) << 31))
+        return set_error(CCSDT_E_ARG, "vector pool of nocc=%d nvir=%d exceeds 2^31 elements", d->nocc, d->nvir);
     int64_t cnt[CCSDT_NARRAYS];
-    array_sizes(d->ncor, d->nocc, d->nvir, d->kchunk, cnt, tensors);
+    array_sizes(d->ncor, d->nocc, d->nvir, d->kchunk, cnt, kind);
     for (int q = 0; q < CCSDT_NARRAYS; ++q)
         if (cnt[q] > 0 && !d->arrays[q]) return set_error(CCSDT_E_ARG, "array %d of the job is null", q);
     return CCSDT_OK;
 }
 
-// Refreshes eorb and the six nvir vectors (padded copies on the device).
-int job_upload_vectors(ccsdt_job j, const double *eorb, const double *const *vec6)
+// Refreshes eorb and the vector pool (each row padded to nvp on the device).
+// rows[r] points at the nvir values of pool row r, r < j->vrows.
+int job_upload_vectors(ccsdt_job j, const double *eorb, const double *const *rows)
 {
     const Geom &g = j->g;
     j->eorb.assign(eorb, eorb + j->cnt[CCSDT_EORB]);
     double *hv = j->h_vec;
     for (int b = 0; b < g.nvp; ++b) hv[b] = b < g.nvir ? j->eorb[g.ncor + g.nocc + b] : 1.0;
-    for (int q = 0; q < 6; ++q)
-        for (int b = 0; b < g.nvp; ++b) hv[g.nvp * (1 + q) + b] = b < g.nvir ? vec6[q][b] : 0.0;
+    for (int64_t r = 0; r < j->vrows; ++r) {
+        double *o = hv + g.nvp * (1 + r);
+        memcpy(o, rows[r], (size_t)g.nvir * sizeof(double));
+        for (int b = g.nvir; b < g.nvp; ++b) o[b] = 0.0;
+    }
     INJECT_POINT("upload");
     HIP_TRY(hipMemcpyAsync(j->d_ev, hv, (size_t)g.nvp * sizeof(double), hipMemcpyHostToDevice, j->st));
-    HIP_TRY(hipMemcpyAsync(j->d_vec6, hv + g.nvp, (size_t)6 * g.nvp * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(j->d_vec6, hv + g.nvp, (size_t)j->vrows * g.nvp * sizeof(double),
                            hipMemcpyHostToDevice, j->st));
     HIP_TRY(hipStreamSynchronize(j->st));
     return CCSDT_OK;
@@ -504,6 +543,17 @@ int job_upload(ccsdt_job j, const double *const *arrays, const int64_t *slice_of
             if (j->cnt[q] > 0)
                 HIP_TRY(hipMemcpyAsync(j->d_raw + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
                                        (size_t)j->cnt[q] * sizeof(double), hipMemcpyHostToDevice, j->st));
+    }
+    if (j->kind == JOB_TENSORS_DRIVER) {
+        // pool rows: t1[m] (m < nocc), then dja[jj][m] and djx[jj][m]
+        const int no = j->g.nocc, nv = j->g.nvir;
+        std::vector<const double *> rows(j->vrows);
+        for (int m = 0; m < no; ++m) rows[m] = arrays[TS_T1] + (int64_t)m * nv;
+        for (int64_t r = 0; r < (int64_t)no * no; ++r) {
+            rows[no + r] = arrays[TS_DJA] + r * nv;
+            rows[no + (int64_t)no * no + r] = arrays[TS_DJX] + r * nv;
+        }
+        return job_upload_vectors(j, arrays[CCSDT_EORB], rows.data());
     }
     // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2: the order the kernels read
     const double *v6[6] = {arrays[CCSDT_T1V1], arrays[CCSDT_T1V2], arrays[CCSDT_DINTX1],
@@ -533,10 +583,11 @@ int ensure_tasks(ccsdt_job j, int64_t n)
     return CCSDT_OK;
 }
 
-int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, bool tensors = false)
+int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, int kind = JOB_HARNESS)
 {
     *out = nullptr;
-    int rc = check_desc(desc, tensors);
+    const bool tensors = kind != JOB_HARNESS;
+    int rc = check_desc(desc, kind);
     if (rc) return rc;
     rc = ensure_device(-1);
     if (rc) return rc;
@@ -545,8 +596,13 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, bool ten
     j->g = make_geom(desc->ncor, desc->nocc, desc->nvir);
     j->kchunk = desc->kchunk;
     j->tensors = tensors;
+    j->kind = kind;
+    j->vrows = job_vrows(desc->nocc, kind);
     const Geom &g = j->g;
-    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, j->cnt, tensors);
+    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, j->cnt, kind);
+    // the driver tensors go to the vector pool only
+    for (int q = 0; q < CCSDT_NARRAYS; ++q)
+        if (is_driver_slot(kind, q)) j->cnt[q] = 0;
     int64_t o = 0;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) {
         j->off[q] = o;
@@ -573,10 +629,10 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, bool ten
     for (hipEvent_t &e : j->ev) JOB_HIP(hipEventCreate(&e));
     JOB_TRY(dev_alloc((void **)&j->d_raw, (size_t)j->nraw * sizeof(double)));
     if (staged) JOB_HIP(host_alloc((void **)&j->h_stage, (size_t)j->nraw * sizeof(double)));
-    JOB_HIP(host_alloc((void **)&j->h_vec, (size_t)7 * g.nvp * sizeof(double)));
+    JOB_HIP(host_alloc((void **)&j->h_vec, (size_t)(1 + j->vrows) * g.nvp * sizeof(double)));
     JOB_HIP(host_alloc((void **)&j->h_out, 64 * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_ev, (size_t)g.nvp * sizeof(double)));
-    JOB_TRY(dev_alloc((void **)&j->d_vec6, (size_t)6 * g.nvp * sizeof(double)));
+    JOB_TRY(dev_alloc((void **)&j->d_vec6, (size_t)j->vrows * g.nvp * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_out2, 8 * sizeof(double)));
     JOB_TRY(dev_alloc((void **)&j->d_jobdev, sizeof(JobDev)));
     JOB_TRY(dev_alloc((void **)&j->d_seg, sizeof(Segment)));
@@ -584,7 +640,7 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, bool ten
     JOB_HIP(host_alloc((void **)&j->h_jobdev, sizeof(JobDev)));
     JOB_HIP(host_alloc((void **)&j->h_seg, sizeof(Segment)));
     JOB_HIP(host_alloc((void **)&j->h_hdr, sizeof(WorkHdr)));
-    j->npanels = (int)job_npanels(g.nocc, j->kchunk, tensors);
+    j->npanels = (int)job_npanels(g.nocc, j->kchunk, kind);
     JOB_TRY(dev_alloc((void **)&j->d_pan, (size_t)j->npanels * g.vecsz * sizeof(double)));
 
     // panel descriptors: which raw arrays feed each packed operand
@@ -669,6 +725,21 @@ int job_tasks(ccsdt_job j, const int *trip, int64_t n)
             tk.lk = 3 + 2 * (k - klo);
         }
         tk.ik_differ = (i != k);
+        if (j->kind == JOB_TENSORS_DRIVER) {
+            // t1v1 = t1[i], t1v2 = t1[k]; dintx1 = djx[j][k], dintx2 = djx[j][i];
+            // dintc1 = dja[j][k], dintc2 = dja[j][i] (ccsdt_tensor_desc)
+            const int no = g.nocc, jrow = no + (jj - 1) * no;
+            const int xoff = no * no;
+            tk.vrow[V_T1V1] = i - 1;
+            tk.vrow[V_T1V2] = k - 1;
+            tk.vrow[V_DINTX1] = xoff + jrow + (k - 1);
+            tk.vrow[V_DINTX2] = xoff + jrow + (i - 1);
+            tk.vrow[V_DINTC1] = jrow + (k - 1);
+            tk.vrow[V_DINTC2] = jrow + (i - 1);
+        } else {
+            for (int q = 0; q < 6; ++q) tk.vrow[q] = q;
+        }
+        tk.pad = 0;
         {   // gpu-omp/ccsd_trpdrv_omp_cbody.c:62: FP32 eorb, FP32 arithmetic
             const float ea = (float)e[a - 1], ei = (float)e[g.ncor + i - 1];
             const float ej = (float)e[g.ncor + jj - 1], ek = (float)e[g.ncor + k - 1];
@@ -1261,7 +1332,7 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
                     memcpy(j->h_fstage + m * vv, fs[m], (size_t)vv * sizeof(double));
                 HIP_TRY(hipMemcpyAsync(j->d_f, j->h_fstage, (size_t)8 * vv * sizeof(double),
                                        hipMemcpyHostToDevice, j->st));
-                j->h_tasks[0] = Task{0, 0, 0, 0, 1, (float)eaijk, eaijk};
+                j->h_tasks[0] = Task{0, 0, 0, 0, 1, (float)eaijk, eaijk, {0, 1, 2, 3, 4, 5}, 0};
                 HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, sizeof(Task), hipMemcpyHostToDevice,
                                        j->st));
                 HIP_TRY(launch_tengy(j->d_f, 8 * vv, j->g, j->d_tasks, 1, j->d_ev, j->d_vec6,
@@ -1309,7 +1380,7 @@ struct SJob {
     int64_t n = 0, next = 0, done = 0;
     int slot = -1;
     int64_t footprint = 0;
-    bool tensors = false;  // full-tensor job
+    int kind = JOB_HARNESS;
     double t_submit = 0.0, t_admit = 0.0, t_finish = 0.0;
     int state = 0;  // 0 waiting, 4 admitting, 1 running, 2 reducing, 3 finished, -1 failed
     int status = CCSDT_OK;
@@ -1358,12 +1429,12 @@ double server_now(ccsdt_server s)
     return std::chrono::duration<double>(Clock::now() - s->t0).count();
 }
 
-int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n, bool tensors)
+int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n, int kind)
 {
-    int64_t b = job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n, tensors);
+    int64_t b = job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n, kind);
     if (is_split_mode(o.mode)) {
         const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
-        const int npanels = (int)job_npanels(d->nocc, d->kchunk, tensors);
+        const int npanels = (int)job_npanels(d->nocc, d->kchunk, kind);
         const SplitGeom sg = make_split_geom(g, o.slices, npanels, o.mode == CCSDT_MODE_BF16X3);
         b += npanels * (sg.slices * sg.slice_bytes + sg.tail_sz * 8 + (int64_t)g.nvp * 8);
     }
@@ -1375,7 +1446,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
 {
     ccsdt_job_desc d = sj->desc;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q];
-    int rc = job_create(&d, &sj->job, false, sj->tensors);
+    int rc = job_create(&d, &sj->job, false, sj->kind);
     if (rc) return rc;
     ccsdt_job j = sj->job;
     const Geom &g = j->g;
@@ -1594,8 +1665,9 @@ int server_finish(ccsdt_server s)
 }  // namespace
 
 // A full-tensor descriptor in the job descriptor's slots: the i-indexed slots
-// carry the tensors, the k-indexed slots stay empty, kchunk = nocc.
-static ccsdt_job_desc tensor_job_desc(const ccsdt_tensor_desc *t)
+// carry the tensors, three k-indexed slots the driver tensors (when given), the
+// others stay empty; kchunk = nocc.  *kind receives the job kind.
+static ccsdt_job_desc tensor_job_desc(const ccsdt_tensor_desc *t, int *kind)
 {
     ccsdt_job_desc d{};
     d.ncor = t->ncor;
@@ -1610,6 +1682,15 @@ static ccsdt_job_desc tensor_job_desc(const ccsdt_tensor_desc *t)
     d.arrays[CCSDT_XIA] = t->xo;
     d.arrays[CCSDT_JIJ] = t->jo;
     d.arrays[CCSDT_KIJ] = t->ko;
+    if (t->t1 || t->dja || t->djx) {
+        // a missing one of the three fails check_desc
+        *kind = JOB_TENSORS_DRIVER;
+        d.arrays[TS_T1] = t->t1;
+        d.arrays[TS_DJA] = t->dja;
+        d.arrays[TS_DJX] = t->djx;
+        return d;
+    }
+    *kind = JOB_TENSORS;
     d.arrays[CCSDT_DINTC1] = t->dintc1;
     d.arrays[CCSDT_DINTX1] = t->dintx1;
     d.arrays[CCSDT_T1V1] = t->t1v1;
@@ -1622,10 +1703,10 @@ static ccsdt_job_desc tensor_job_desc(const ccsdt_tensor_desc *t)
 // Validates and queues a job (admitting it at once when it fits and nothing
 // waits before it).
 static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples, int64_t n,
-                         int64_t *id, bool tensors)
+                         int64_t *id, int kind)
 {
     if (!s) return set_error(CCSDT_E_STATE, "null server");
-    int rc = check_desc(desc, tensors);
+    int rc = check_desc(desc, kind);
     if (rc) return rc;
     if (n < 1 || !triples) return set_error(CCSDT_E_ARG, "empty triple list");
     const Geom g = make_geom(desc->ncor, desc->nocc, desc->nvir);
@@ -1638,27 +1719,27 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
             return set_error(CCSDT_E_ARG, "triple %lld outside the job's shape", (long long)t);
     }
     if (is_split_mode(s->o.mode)) {
-        const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, tensors),
+        const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, kind),
                                              s->o.mode == CCSDT_MODE_BF16X3);
         if ((int64_t)sg.npanels * sg.slices * sg.slice_bytes >= (int64_t(1) << 32) ||
             (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
     }
-    const int64_t fp = server_footprint(s->o, desc, n, tensors);
+    const int64_t fp = server_footprint(s->o, desc, n, kind);
     if (fp > s->o.budget_bytes)
         return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
                          (long long)fp, (long long)s->o.budget_bytes);
     SJob *sj = new SJob();
     sj->t_submit = server_now(s);
     sj->desc = *desc;
-    sj->tensors = tensors;
+    sj->kind = kind;
     sj->n = n;
     sj->footprint = fp;
     sj->trip.assign(triples, triples + 5 * n);
     // the arrays are copied before return (outside the server lock); the
     // admission thread uploads them
     int64_t cnt[CCSDT_NARRAYS];
-    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt, tensors);
+    array_sizes(desc->ncor, desc->nocc, desc->nvir, desc->kchunk, cnt, kind);
     int64_t tot = 0;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) {
         sj->hoff[q] = tot;
@@ -1736,8 +1817,9 @@ int ccsdt_job_create_tensors(const ccsdt_tensor_desc *desc, ccsdt_job *out)
 {
     if (!out) return set_error(CCSDT_E_ARG, "null output handle");
     if (!desc) return set_error(CCSDT_E_ARG, "null tensor descriptor");
-    const ccsdt_job_desc d = tensor_job_desc(desc);
-    int rc = job_create(&d, out, false, true);
+    int kind = JOB_TENSORS;
+    const ccsdt_job_desc d = tensor_job_desc(desc, &kind);
+    int rc = job_create(&d, out, false, kind);
     if (!rc) t_err = CCSDT_OK;
     return rc;
 }
@@ -1781,13 +1863,13 @@ int ccsdt_job_destroy(ccsdt_job job)
 int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples)
 {
     if (nocc < 1 || nvir < 1 || kchunk < 1 || ncor < 0) return -1;
-    return job_footprint(ncor, nocc, nvir, kchunk, ntriples, false);
+    return job_footprint(ncor, nocc, nvir, kchunk, ntriples, JOB_HARNESS);
 }
 
-int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriples)
+int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriples, int driver)
 {
     if (nocc < 1 || nvir < 1 || ncor < 0) return -1;
-    return job_footprint(ncor, nocc, nvir, nocc, ntriples, true);
+    return job_footprint(ncor, nocc, nvir, nocc, ntriples, driver ? JOB_TENSORS_DRIVER : JOB_TENSORS);
 }
 
 
@@ -1847,15 +1929,16 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
 int ccsdt_server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *triples, int64_t n,
                         int64_t *id)
 {
-    return server_submit(s, desc, triples, n, id, false);
+    return server_submit(s, desc, triples, n, id, JOB_HARNESS);
 }
 
 int ccsdt_server_submit_tensors(ccsdt_server s, const ccsdt_tensor_desc *desc, const int *triples,
                                 int64_t n, int64_t *id)
 {
     if (!desc) return set_error(CCSDT_E_ARG, "null tensor descriptor");
-    const ccsdt_job_desc d = tensor_job_desc(desc);
-    return server_submit(s, &d, triples, n, id, true);
+    int kind = JOB_TENSORS;
+    const ccsdt_job_desc d = tensor_job_desc(desc, &kind);
+    return server_submit(s, &d, triples, n, id, kind);
 }
 
 int ccsdt_server_progress(ccsdt_server s, int block)

@@ -16,12 +16,20 @@ namespace ccsdt {
 // A harness-shaped job (omp/test.c arrays) has li = 0, rij = 1 and, for k
 // slice s = k-klo (omp/ccsd_trpdrv_omp_cbody.c:64-70), rkj = 2+2s, lk = 3+2s;
 // a full-tensor job indexes its L and R families by occupied indices.
+// vrow[q] is the row (nvp doubles) of the job's vector pool holding vector q of
+// the triple, in the kernels' order t1v1 t1v2 dintx1 dintx2 dintc1 dintc2: rows
+// 0..5 when the job has one set of vectors, per-triple rows into the driver
+// tensors otherwise (ccsdt_tensor_desc).
 struct Task {
     int li, lk, rij, rkj;
     int ik_differ;  // 1 when i != k: the k terms enter the totals
     float eaijk32;  // the same in FP32 arithmetic on FP32 eorb (the FP32 form)
     double eaijk;   // eorb[a] - (eorb[ncor+i] + eorb[ncor+j] + eorb[ncor+k])
+    int vrow[6];
+    int pad;
 };
+
+enum { V_T1V1 = 0, V_T1V2 = 1, V_DINTX1 = 2, V_DINTX2 = 3, V_DINTC1 = 4, V_DINTC2 = 5 };
 
 // Shapes of one job after padding.
 struct Geom {
@@ -75,7 +83,7 @@ struct JobDev {
     const double *scales;   // split form: per-row power-of-two scales
     const double *tail;     // split form: FP64 K = nocc panels
     const double *ev;       // virtual orbital energies, padded to nvp
-    const double *vec6;     // t1v1 t1v2 dintx1 dintx2 dintc1 dintc2, each nvp
+    const double *vec6;     // vector pool: rows of nvp doubles (Task::vrow)
     const short2 *pairs;    // lower-triangle 16x16 block pairs
 };
 

@@ -368,6 +368,9 @@ __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, con
     // column l%16 (measured with ccsdt_selftest_mfma).
     const int nv = g.nvir, nvp = g.nvp;
     const int col = lane & 15, row0 = lane >> 4;
+    int vo[6];  // offsets of the task's six vectors in the pool (uniform)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) vo[q] = uniform(tk.vrow[q] * nvp);
     const int cx = ci * 16 + col;
     if (WRITEF) {
         const int64_t vv = (int64_t)nv * nv;
@@ -388,23 +391,23 @@ __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, con
     if (TENGY) {
         Sum4 s = {0.0, 0.0, 0.0, 0.0};
         const double ec = ev[cx];
-        const double u1c = vec6[cx], u2c = vec6[nvp + cx];
-        const double x1c = vec6[2 * nvp + cx], x2c = vec6[3 * nvp + cx];
-        const double y1c = vec6[4 * nvp + cx], y2c = vec6[5 * nvp + cx];
+        const double u1c = vec6[vo[0] + cx], u2c = vec6[vo[1] + cx];
+        const double x1c = vec6[vo[2] + cx], x2c = vec6[vo[3] + cx];
+        const double y1c = vec6[vo[4] + cx], y2c = vec6[vo[5] + cx];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int bx = bi * 16 + row0 + 4 * r;
             if (bx < nv && cx < nv) {
                 const double d = -1.0 / (ev[bx] + ec + tk.eaijk);
-                const double u1b = vec6[bx], u2b = vec6[nvp + bx];
+                const double u1b = vec6[vo[0] + bx], u2b = vec6[vo[1] + bx];
                 tengy_point(acc[0][r], acc[1][r], acc[2][r], acc[3][r],
                             acc[4][r], acc[5][r], acc[6][r], acc[7][r],
                             acc[8][r], acc[9][r], acc[10][r], acc[11][r],
                             acc[12][r], acc[13][r], acc[14][r], acc[15][r],
                             d, u1b, u2b, x1c, x2c, y1c, y2c, s);
                 if (bi != ci) {  // the (c,b) point of the same pair of blocks
-                    const double x1b = vec6[2 * nvp + bx], x2b = vec6[3 * nvp + bx];
-                    const double y1b = vec6[4 * nvp + bx], y2b = vec6[5 * nvp + bx];
+                    const double x1b = vec6[vo[2] + bx], x2b = vec6[vo[3] + bx];
+                    const double y1b = vec6[vo[4] + bx], y2b = vec6[vo[5] + bx];
                     tengy_point(acc[8][r], acc[9][r], acc[10][r], acc[11][r],
                                 acc[12][r], acc[13][r], acc[14][r], acc[15][r],
                                 acc[0][r], acc[1][r], acc[2][r], acc[3][r],
@@ -938,6 +941,9 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
     const double *__restrict__ vec6 = jd.vec6;
     const int nvp = g.nvp;
     const int row0 = 4 * (lane >> 4);
+    int vo[6];  // offsets of the task's six vectors in the pool (uniform)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) vo[q] = uniform(tk.vrow[q] * nvp);
     const int cx = ci * 16 + (lane & 15);
     const bool diag = (bi == ci);
     const int nv = g.nvir;
@@ -980,9 +986,9 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
         __syncthreads();
         SumT<T> s = {0, 0, 0, 0};
         const T ec = (T)ev[cx];
-        const T u1c = (T)vec6[cx], u2c = (T)vec6[nvp + cx];
-        const T x1c = (T)vec6[2 * nvp + cx], x2c = (T)vec6[3 * nvp + cx];
-        const T y1c = (T)vec6[4 * nvp + cx], y2c = (T)vec6[5 * nvp + cx];
+        const T u1c = (T)vec6[vo[0] + cx], u2c = (T)vec6[vo[1] + cx];
+        const T x1c = (T)vec6[vo[2] + cx], x2c = (T)vec6[vo[3] + cx];
+        const T y1c = (T)vec6[vo[4] + cx], y2c = (T)vec6[vo[5] + cx];
         const T eaijk = std::is_same<T, float>::value ? (T)tk.eaijk32 : (T)tk.eaijk;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -1007,12 +1013,12 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
                 const T dd = (T)(-1.0 / (double)((T)ev[bx] + ec + eaijk));
                 tengy_point<T>(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
                                mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
-                               dd, (T)vec6[bx], (T)vec6[nvp + bx], x1c, x2c, y1c, y2c, s);
+                               dd, (T)vec6[vo[0] + bx], (T)vec6[vo[1] + bx], x1c, x2c, y1c, y2c, s);
                 if (bi != ci)
                     tengy_point<T>(mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
                                    dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
-                                   dd, u1c, u2c, (T)vec6[2 * nvp + bx], (T)vec6[3 * nvp + bx],
-                                   (T)vec6[4 * nvp + bx], (T)vec6[5 * nvp + bx], s);
+                                   dd, u1c, u2c, (T)vec6[vo[2] + bx], (T)vec6[vo[3] + bx],
+                                   (T)vec6[vo[4] + bx], (T)vec6[vo[5] + bx], s);
             }
         }
         s.e4i = wave_sum(s.e4i);
@@ -1289,6 +1295,9 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     const int CI = p - BI * (BI + 1) / 2;
     const Task tk = tasks[t];
     const double *ft = f + (int64_t)t * fstride;
+    int vo[6];  // offsets of the task's six vectors in the pool (uniform)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) vo[q] = uniform(tk.vrow[q] * nvp);
     const int64_t vv = (int64_t)nv * nv;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const int bx = BI * 32 + tx;
@@ -1345,13 +1354,13 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
             // body order: 0 f1n, 1 f1t, 2 f2n, 3 f2t, 4 f3n, 5 f3t, 6 f4n, 7 f4t
             tengy_point(dv[0][q], dv[2][q], dv[4][q], dv[6][q], dv[1][q], dv[3][q], dv[5][q], dv[7][q],
                         mv[0][q], mv[2][q], mv[4][q], mv[6][q], mv[1][q], mv[3][q], mv[5][q], mv[7][q],
-                        d, vec6[bx], vec6[nvp + bx], vec6[2 * nvp + cx], vec6[3 * nvp + cx],
-                        vec6[4 * nvp + cx], vec6[5 * nvp + cx], s);
+                        d, vec6[vo[0] + bx], vec6[vo[1] + bx], vec6[vo[2] + cx], vec6[vo[3] + cx],
+                        vec6[vo[4] + cx], vec6[vo[5] + cx], s);
             if (BI != CI)
                 tengy_point(mv[0][q], mv[2][q], mv[4][q], mv[6][q], mv[1][q], mv[3][q], mv[5][q], mv[7][q],
                             dv[0][q], dv[2][q], dv[4][q], dv[6][q], dv[1][q], dv[3][q], dv[5][q], dv[7][q],
-                            d, vec6[cx], vec6[nvp + cx], vec6[2 * nvp + bx], vec6[3 * nvp + bx],
-                            vec6[4 * nvp + bx], vec6[5 * nvp + bx], s);
+                            d, vec6[vo[0] + cx], vec6[vo[1] + cx], vec6[vo[2] + bx], vec6[vo[3] + bx],
+                            vec6[vo[4] + bx], vec6[vo[5] + bx], s);
         }
     }
     s.e4i = wave_sum(s.e4i);

@@ -88,6 +88,11 @@ def test_footprint(cc):
     t = cc.tensor_footprint(40, 400, 11480)
     assert 5e9 < t < 12e9 and t > cc.footprint(40, 400, 40, 11480)
     assert cc.tensor_footprint(0, 400, 1) == -1
+    # driver tensors: nocc + 2 nocc^2 pool rows of nvp doubles replace the six
+    # fixed vectors (six rows, and their 256-byte aligned raw copies)
+    nocc, nvir, nvp = 40, 400, 400
+    d = cc.tensor_footprint(nocc, nvir, 11480, driver=True)
+    assert d - t == ((nocc + 2 * nocc * nocc - 6) * nvp - 6 * 416) * 8
 
 
 def test_gather_triple_reads_the_tensor_slices(cc):
@@ -109,6 +114,26 @@ def test_gather_triple_reads_the_tensor_slices(cc):
     assert a["tia"][0] == i0 and a["xka"][0] == k0
     assert a["jij"][0] == j0 * nocc + i0 and a["kkj"][0] == j0 * nocc + k0
     assert a["tia"].size == ov * nocc and a["t1v1"].size == vv
+
+
+def test_triple_vectors_read_the_driver_rows(cc):
+    """With driver tensors, triple (i,j,k) reads t1 rows i, k and the (j, k),
+    (j, i) rows of dja and djx; without them, the job's fixed vectors."""
+    nocc, nvir = 3, 4
+    t = {"t1": np.repeat(np.arange(nocc, dtype=float), nvir),
+         "dja": np.repeat(np.arange(nocc * nocc, dtype=float), nvir),
+         "djx": np.repeat(100 + np.arange(nocc * nocc, dtype=float), nvir)}
+    v = cc.triple_vectors(t, nocc, nvir, 3, 2, 1)   # i=3, j=2, k=1 (1-based)
+    i0, j0, k0 = 2, 1, 0
+    assert v["t1v1"][0] == i0 and v["t1v2"][0] == k0
+    assert v["dintc1"][0] == j0 * nocc + k0 and v["dintc2"][0] == j0 * nocc + i0
+    assert v["dintx1"][0] == 100 + j0 * nocc + k0 and v["dintx2"][0] == 100 + j0 * nocc + i0
+    assert all(len(x) == nvir for x in v.values())
+    fixed = {name: np.full(nvir, float(q)) for q, name in enumerate(cc.VECTOR_NAMES)}
+    w = cc.triple_vectors(fixed, nocc, nvir, 3, 2, 1)
+    assert all(np.array_equal(w[n], fixed[n]) for n in cc.VECTOR_NAMES)
+    a = cc.gather_triple(dict(cc.make_tensor_inputs(nocc, nvir, 3, driver=True)), nocc, nvir, 3, 2, 1)
+    assert len(a["t1v1"]) == nvir * nvir and not np.any(a["t1v1"][nvir:])
 
 
 def test_tensor_job_without_device(cc):

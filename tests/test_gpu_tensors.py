@@ -80,10 +80,11 @@ def test_constant_tensors_equal_harness_job(cc):
     assert np.array_equal(ra["terms"], rb["terms"])
 
 
+@pytest.mark.parametrize("driver", [False, True])
 @pytest.mark.parametrize("mode,slices", [("fp64", 0), ("ozaki_int8", 3)])
-def test_tensor_job_schedules_bitwise(cc, mode, slices):
+def test_tensor_job_schedules_bitwise(cc, mode, slices, driver):
     nocc, nvir = 6, 40
-    t = cc.make_tensor_inputs(nocc, nvir, 5)
+    t = cc.make_tensor_inputs(nocc, nvir, 5, driver=driver)
     trip = cc.triple_list(nocc, [(1, 3), (4, 6)])
     with cc.Job.from_tensors(t, nocc, nvir) as job:
         ref = job.run(trip, mode=mode, slices=slices, terms=True)
@@ -121,4 +122,69 @@ def test_tensor_and_harness_jobs_serve_together(cc):
     order = [ids[0], ids[2], ids[4], ids[1], ids[3]]  # harness jobs, then tensor jobs
     for jid, a in zip(order, alone):
         m = res[jid]
+        assert m["status"] == 0 and (m["emp4"], m["emp5"]) == (a["emp4"], a["emp5"])
+
+
+# ------------------------------------------------------------------ driver tensors
+@pytest.mark.parametrize("mode,slices,variant", [("fp64", 0, 0), ("fp64", 0, 1), ("fp64_split", 0, 0),
+                                                 ("ozaki_int8", 4, 0), ("bf16x3", 0, 0)])
+def test_driver_tensor_job_against_gathered_oracle(cc, orc, mode, slices, variant):
+    """Per-triple vectors read in place from t1, dja, djx: each triple's terms
+    match the oracle on the host-gathered arrays (rows by the same slicing)."""
+    nocc, nvir = 5, 37
+    t = cc.make_tensor_inputs(nocc, nvir, 808, driver=True)
+    trip = cc.triple_list(nocc, [(1, nocc)])
+    terms, abs4, E = gathered_oracle(cc, orc, t, nocc, nvir, trip)
+    with cc.Job.from_tensors(t, nocc, nvir) as job:
+        r = job.run(trip, mode=mode, slices=slices, variant=variant, terms=True)
+    if mode in ("fp64", "fp64_split"):
+        assert np.max(np.abs(r["terms"] - terms) / np.maximum(np.abs(terms), abs4)) < 1e-11
+        assert abs(r["emp4"] + r["emp5"] - E) <= 1e-11 * abs(E)
+    else:
+        assert abs(r["emp4"] + r["emp5"] - E) <= 1e-8 * abs(E)
+
+
+def test_driver_rows_equal_to_fixed_vectors_bitwise(cc):
+    """Driver tensors whose rows all equal one set of vectors give the fixed-
+    vector tensor job's terms bitwise (the pool rows only change addresses)."""
+    nocc, nvir = 4, 29
+    t = cc.make_tensor_inputs(nocc, nvir, 12)
+    t["t1v2"] = t["t1v1"]
+    t["dintc2"] = t["dintc1"]
+    t["dintx2"] = t["dintx1"]
+    d = {k: v for k, v in t.items() if k not in cc.VECTOR_NAMES}
+    d["t1"] = np.tile(t["t1v1"], nocc)
+    d["dja"] = np.tile(t["dintc1"], nocc * nocc)
+    d["djx"] = np.tile(t["dintx1"], nocc * nocc)
+    trip = cc.triple_list(nocc, [(1, 2), (3, 4)])
+    for mode, slices in (("fp64", 0), ("ozaki_int8", 3), ("fp32", 0)):
+        with cc.Job.from_tensors(t, nocc, nvir) as a, cc.Job.from_tensors(d, nocc, nvir) as b:
+            ra = a.run(trip, mode=mode, slices=slices, terms=True)
+            rb = b.run(trip, mode=mode, slices=slices, terms=True)
+        assert np.array_equal(ra["terms"], rb["terms"]), mode
+
+
+def test_driver_tensor_job_missing_tensor_rejected(cc):
+    nocc, nvir = 3, 20
+    t = cc.make_tensor_inputs(nocc, nvir, 4, driver=True)
+    t["djx"] = None
+    with pytest.raises(cc.CcsdtError) as e:
+        cc.Job.from_tensors(t, nocc, nvir)
+    assert e.value.code == -1
+
+
+def test_driver_tensor_jobs_serve_bitwise(cc):
+    """Driver-tensor jobs in a server mix give their plain runs' totals bitwise."""
+    specs = [(5, 30, 1), (4, 45, 2), (6, 24, 3)]
+    tens = [(no, nv, cc.make_tensor_inputs(no, nv, sd, driver=True)) for no, nv, sd in specs]
+    trips = [cc.triple_list(no, [(1, no)]) for no, nv, t in tens]
+    alone = []
+    for (no, nv, t), tr in zip(tens, trips):
+        with cc.Job.from_tensors(t, no, nv) as job:
+            alone.append(job.run(tr))
+    with cc.Server(items_per_tick=257) as srv:
+        ids = [srv.submit_tensors(t, no, nv, tr) for (no, nv, t), tr in zip(tens, trips)]
+        srv.drain()
+        res = [srv.result(i) for i in ids]
+    for a, m in zip(alone, res):
         assert m["status"] == 0 and (m["emp4"], m["emp5"]) == (a["emp4"], a["emp5"])

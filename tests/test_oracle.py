@@ -223,6 +223,30 @@ def test_nt_swap_symmetry(orc, cc, one_thread):
     assert np.allclose(t1, t0[[2, 3, 0, 1]], rtol=1e-14, atol=0)
 
 
+def test_driver_rows_keep_the_ik_exchange(orc, cc, one_thread):
+    """The driver-tensor slicing of a full-tensor job (t1v1 = t1[i], dintc1 =
+    dja[j][k], dintx1 = djx[j][k], and the same with i, k exchanged for the
+    second set) makes triple (k,j,i) the i <-> k exchange of triple (i,j,k):
+    its (emp4i, emp5i) are the other's (emp4k, emp5k).  A slicing that took a
+    vector from the wrong occupied index breaks this."""
+    nocc, nvir = 4, 14
+    t = cc.make_tensor_inputs(nocc, nvir, 61, driver=True)
+    for i, j, k in [(3, 2, 1), (4, 4, 2), (2, 1, 4)]:
+        tr = np.array([[nocc + 2, i, j, k, k]], np.int32)
+        ts = np.array([[nocc + 2, k, j, i, i]], np.int32)
+        a = orc.run_list(cc.gather_triple(t, nocc, nvir, i, j, k), 0, nocc, nvir, tr, nthreads=1)
+        b = orc.run_list(cc.gather_triple(t, nocc, nvir, k, j, i), 0, nocc, nvir, ts, nthreads=1)
+        assert np.allclose(b["terms"][0], a["terms"][0][[2, 3, 0, 1]], rtol=1e-13, atol=0)
+    # the mistaken slicing dintc1 = dja[j][i] fails the same check
+    i, j, k = 3, 2, 1
+    ga = cc.gather_triple(t, nocc, nvir, i, j, k)
+    gb = cc.gather_triple(t, nocc, nvir, k, j, i)
+    ga["dintc1"], gb["dintc1"] = ga["dintc2"].copy(), gb["dintc2"].copy()
+    a = orc.run_list(ga, 0, nocc, nvir, np.array([[nocc + 2, i, j, k, k]], np.int32), nthreads=1)
+    b = orc.run_list(gb, 0, nocc, nvir, np.array([[nocc + 2, k, j, i, i]], np.int32), nthreads=1)
+    assert not np.allclose(b["terms"][0], a["terms"][0][[2, 3, 0, 1]], rtol=1e-6, atol=0)
+
+
 def test_homogeneity_and_zero(orc, cc, one_thread):
     nocc, nvir = 4, 20
     arrays = cc.make_inputs(nocc, nvir, 1, 8)
