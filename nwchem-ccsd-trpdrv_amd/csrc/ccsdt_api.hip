@@ -863,7 +863,7 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
     else if (f32)
         HIP_TRY(launch_pack32(j->d_desc, j->npanels, g, reinterpret_cast<float *>(j->d_pan), j->st));
     else
-        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, variant == CCSDT_TILE_TWO_WAVE, j->st));
     if (timed) HIP_TRY(hipEventRecord(E[1], j->st));
     const int64_t items = n * g.npairs;
     const int grid = persistent ? persistent_grid(split || f32, code, variant, wg_per_cu) : (int)items;
@@ -1008,7 +1008,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     } else {
         HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)n * sizeof(Task), hipMemcpyHostToDevice, j->st));
         HIP_TRY(hipEventRecord(E[0], j->st));
-        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, variant == CCSDT_TILE_TWO_WAVE, j->st));
         HIP_TRY(hipEventRecord(E[1], j->st));
         launches = 1;
         fill_tables(j, sg, n, true, false);
@@ -1462,7 +1462,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
     if (is_split_mode(s->o.mode))
         HIP_TRY(launch_split_pack(j->d_desc, j->npanels, g, sg, j->d_digits, j->d_scales, j->d_tail, j->st));
     else
-        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, j->st));
+        HIP_TRY(launch_pack(j->d_desc, j->npanels, g, s->o.variant == CCSDT_TILE_TWO_WAVE, j->st));
     JobDev &jd = s->h_jobs[sj->slot];  // the slot reserved for this job
     jd.g = g;
     jd.sg = sg;

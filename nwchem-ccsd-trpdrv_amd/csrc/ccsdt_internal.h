@@ -120,7 +120,9 @@ struct WorkHdr {
 };
 
 // Launchers (ccsdt_kernels.hip).  All enqueue on `st` and return hipError_t.
-hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st);
+// pairs: the step-pair panel layout the two-wave FP64 kernel reads (k_fused2);
+// otherwise the single-step layout of k_fused.
+hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, bool pairs, hipStream_t st);
 // grid = number of workgroups: the item count for a plain launch, or a
 // multiple of 8 for a persistent one (each workgroup then loops over items).
 // variant 0: one wave computes all 16 products of a block pair (k_fused);

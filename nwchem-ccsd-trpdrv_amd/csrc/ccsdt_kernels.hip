@@ -86,6 +86,15 @@ __device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// Two doubles per lane in one 16-byte load.  On gfx950 a wave's 8-byte load
+// takes as long in the texture data path as a 16-byte one (about 16 cycles per
+// CU), so the step-pair panel layout below halves that path's time per MFMA.
+__device__ __forceinline__ dbl2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                  uint32_t soff)
+{
+    return __builtin_bit_cast(dbl2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
 // Spreads consecutive logical tiles over one XCD: the hardware hands block b to
 // XCD b % 8, so logical tile L = (b%8)*q + min(b%8, r) + b/8 keeps a task's
 // tiles (which share operand panels) in the same L2.
@@ -173,16 +182,38 @@ __device__ __forceinline__ int find_segment(const Segment *__restrict__ segs, in
 }
 
 // ------------------------------------------------------------------ packing
+// Panel element order.  Single-step layout (k_fused): per 16-row block and
+// 4-wide K step, element 64*step + l holds row l%16, K index 4*step + l/16.
+// Step-pair layout (PAIRS, the two-wave kernels): the block's steps go in pairs
+// (2s, 2s+1) and element 128*s + 2*l + h holds lane l's value of step 2s+h, so
+// one 16-byte load per lane feeds two steps; an odd last step keeps the
+// single-step order.  A block is nks*64 doubles either way.
+template <bool PAIRS>
 __global__ __launch_bounds__(256) void k_pack(const PanelDesc *__restrict__ descs, Geom g)
 {
     const PanelDesc d = descs[blockIdx.y];
     const int nv = g.nvir, no = g.nocc;
+    const int64_t blk = (int64_t)g.nks * 64;
+    const int64_t paired = (int64_t)(g.nks & ~1) * 64;
     for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < g.vecsz;
          idx += (int64_t)gridDim.x * 256) {
-        const int l = (int)(idx & 63);
-        const int64_t rest = idx >> 6;
-        const int ks = (int)(rest % g.nks);
-        const int xb = (int)(rest / g.nks);
+        int l, ks, xb;
+        if (PAIRS) {
+            xb = (int)(idx / blk);
+            const int64_t e = idx - (int64_t)xb * blk;
+            if (e < paired) {
+                ks = 2 * (int)(e >> 7) + (int)(e & 1);
+                l = (int)((e >> 1) & 63);
+            } else {
+                ks = (int)(e >> 6);
+                l = (int)(e & 63);
+            }
+        } else {
+            l = (int)(idx & 63);
+            const int64_t rest = idx >> 6;
+            ks = (int)(rest % g.nks);
+            xb = (int)(rest / g.nks);
+        }
         const int x = xb * 16 + (l & 15);
         const int kk = ks * 4 + (l >> 4);
         double v = 0.0;
@@ -198,11 +229,14 @@ __global__ __launch_bounds__(256) void k_pack(const PanelDesc *__restrict__ desc
     }
 }
 
-hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, hipStream_t st)
+hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, bool pairs, hipStream_t st)
 {
     int64_t blocks = (g.vecsz + 255) / 256;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_pack, dim3((unsigned)blocks, npanels), dim3(256), 0, st, d_desc, g);
+    if (pairs)
+        hipLaunchKernelGGL(k_pack<true>, dim3((unsigned)blocks, npanels), dim3(256), 0, st, d_desc, g);
+    else
+        hipLaunchKernelGGL(k_pack<false>, dim3((unsigned)blocks, npanels), dim3(256), 0, st, d_desc, g);
     return hipGetLastError();
 }
 
@@ -580,11 +614,13 @@ __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict_
 #pragma unroll
         for (int q = 0; q < NPL; ++q) pl[((int64_t)pnl * NPL + q) * plane_words + o] = word[q];
     }
-    // FP64 tail: element e = step*64 + l with l = x%16 + 16*(k%4)
+    // FP64 tail in the step-pair layout (nkt is even): element 128*s + 2*l + h
+    // holds row l%16, K index 4*(2s+h) + l/16
     double *tp = tail + (int64_t)pnl * sg.tail_sz + (int64_t)xb * sg.nkt * 64;
     for (int e2 = t; e2 < sg.nkt * 64; e2 += 256) {
-        const int kk = 4 * (e2 >> 6) + ((e2 & 63) >> 4);
-        const int xx = xb * 16 + (e2 & 15);
+        const int l = (e2 >> 1) & 63;
+        const int kk = 4 * (2 * (e2 >> 7) + (e2 & 1)) + (l >> 4);
+        const int xx = xb * 16 + (l & 15);
         double v = 0.0;
         if (xx < nv && kk < no)
             v = d.sgn * (d.second_t ? d.Q[kk + (int64_t)xx * no] : d.Q[xx + (int64_t)kk * nv]);
@@ -753,8 +789,9 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
     const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
     const uint32_t blkt = (uint32_t)sg.nkt * 512;
     const int li = lane & 15;
-    const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
-    const uint32_t voffB = lane * 8;
+    // step-pair layout: each lane's two doubles of a step pair are adjacent
+    const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 16;
+    const uint32_t voffB = lane * 16;
     // W=0: A {LJ, LK, LJt, LKt} at b, B {R1, R3, R1t, R3t} at c; W=1 roles swapped
     const int PL[4] = {LJ, LK, LJT, LKT};
     const int PR[4] = {R1, R3, R1T, R3T};
@@ -764,30 +801,40 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
         ao[v] = (W == 0 ? PL[v] : PR[v]) * tsb + bi * blkt;
         bo[v] = (W == 0 ? PR[v] : PL[v]) * tsb + ci * blkt;
     }
-    for (int ks = 0; ks < sg.nkt; ++ks) {
-        double a[4], b[4];
+    for (int sp = 0; sp < sg.nkt / 2; ++sp) {
+        dbl2 a2[4], b2[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            a[v] = buffer_load_f64(rt, voffA, ao[v] + ks * 512);
-            b[v] = buffer_load_f64(rt, voffB, bo[v] + ks * 512);
+            a2[v] = buffer_load_f64x2(rt, voffA, ao[v] + sp * 1024);
+            b2[v] = buffer_load_f64x2(rt, voffB, bo[v] + sp * 1024);
         }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            double a[4], b[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                a[v] = a2[v][h];
+                b[v] = b2[v][h];
+            }
 #define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
-        if (W == 0) {
-            MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);
-            MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);
-        } else {
-            MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);
-            MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);
-        }
+            if (W == 0) {
+                MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);
+                MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);
+            } else {
+                MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);
+                MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);
+            }
 #undef MS
+        }
     }
 }
 
 // FP64 form of the two-wave item (k_fused2): the same eight products per wave
-// role as split_wave, over the full K = nvir + nocc of the FP64 panels.  Each K
-// step loads four A and four B panels and runs eight v_mfma_f64_16x16x4_f64;
-// ping-pong buffers as in fused_mainloop.  The A rows are fed in the order
-// 4*(i%4) + i/4 so the result layout is the one pair_epilogue expects.
+// role as split_wave, over the full K = nvir + nocc of the FP64 panels in the
+// step-pair layout.  Each pair of K steps takes four A and four B 16-byte loads
+// and runs sixteen v_mfma_f64_16x16x4_f64, in step order; ping-pong buffers of
+// step pairs.  The A rows are fed in the order 4*(i%4) + i/4 so the result
+// layout is the one pair_epilogue expects.
 template <int W>
 __device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restrict__ pan,
                                           const Geom &g, const PanelPairs &pp, int bi, int ci,
@@ -799,8 +846,9 @@ __device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restr
     const uint32_t blkb = (uint32_t)g.nks * 512;
     const uint32_t ob = bi * blkb, oc = ci * blkb;
     const int li = lane & 15;
-    const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 8;
-    const uint32_t voffB = lane * 8;
+    const uint32_t la = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48));
+    // panels in the step-pair layout (k_pack<true>): 16 bytes per lane per pair
+    const uint32_t voffA2 = la * 16, voffB2 = lane * 16;
     // L panels {LJ, LK, LJt, LKt} and R panels {R1, R3, R1t, R3t}: their pair,
     // and first (v even) or second panel of it
     constexpr int LR[4] = {RP_LI, RP_LI, RP_LK, RP_LK};
@@ -811,18 +859,17 @@ __device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restr
         ao[v] = (v & 1) * vsb + ob;
         bo[v] = (v & 1) * vsb + oc;
     }
-    const int nks = g.nks;
-    double xa[4], xb[4], ya[4], yb[4];
-#define F2_LOAD(da, db, kstep)                                                                   \
+    const int nks = g.nks, npair = nks >> 1;
+    dbl2 xa[4], xb[4], ya[4], yb[4];
+#define F2_LOAD(da, db, spair)                                                                   \
     _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                             \
         const int ra = (W == 0) ? LR[v] : RR[v], rb = (W == 0) ? RR[v] : LR[v];                   \
-        da[v] = buffer_load_f64(rp[ra], voffA, ao[v] + (uint32_t)(kstep) * 512);                  \
-        db[v] = buffer_load_f64(rp[rb], voffB, bo[v] + (uint32_t)(kstep) * 512);                  \
+        da[v] = buffer_load_f64x2(rp[ra], voffA2, ao[v] + (uint32_t)(spair) * 1024);             \
+        db[v] = buffer_load_f64x2(rp[rb], voffB2, bo[v] + (uint32_t)(spair) * 1024);             \
     }
 #define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
-#define F2_MFMA(da, db)                                                                          \
+#define F2_STEP(a, b)                                                                            \
     {                                                                                            \
-        const double *a = da, *b = db;                                                           \
         if (W == 0) {                                                                            \
             MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);                                  \
             MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);                                  \
@@ -831,18 +878,39 @@ __device__ __forceinline__ void fp64_wave(dbl4 (&facc)[8], const double *__restr
             MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);                                  \
         }                                                                                        \
     }
-    F2_LOAD(xa, xb, 0)
-    int ks = 0;
-    for (; ks + 1 < nks; ks += 2) {
-        F2_LOAD(ya, yb, ks + 1)
+#define F2_MFMA(da, db)                                                                          \
+    _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                             \
+        double a[4], b[4];                                                                       \
+        _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                         \
+            a[v] = da[v][h];                                                                     \
+            b[v] = db[v][h];                                                                     \
+        }                                                                                        \
+        F2_STEP(a, b)                                                                            \
+    }
+    if (npair > 0) F2_LOAD(xa, xb, 0)
+    int sp = 0;
+    for (; sp + 1 < npair; sp += 2) {
+        F2_LOAD(ya, yb, sp + 1)
         F2_MFMA(xa, xb)
-        const int k2 = (ks + 2 < nks) ? ks + 2 : nks - 1;  // clamped prefetch, in bounds
-        F2_LOAD(xa, xb, k2)
+        const int s2 = (sp + 2 < npair) ? sp + 2 : npair - 1;  // clamped prefetch, in bounds
+        F2_LOAD(xa, xb, s2)
         F2_MFMA(ya, yb)
     }
-    if (nks & 1) F2_MFMA(xa, xb)
+    if (npair & 1) F2_MFMA(xa, xb)
+    if (nks & 1) {
+        // odd last step, single-step order
+        double a[4], b[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ra = (W == 0) ? LR[v] : RR[v], rb = (W == 0) ? RR[v] : LR[v];
+            a[v] = buffer_load_f64(rp[ra], la * 8, ao[v] + (uint32_t)npair * 1024);
+            b[v] = buffer_load_f64(rp[rb], lane * 8, bo[v] + (uint32_t)npair * 1024);
+        }
+        F2_STEP(a, b)
+    }
 #undef F2_LOAD
 #undef F2_MFMA
+#undef F2_STEP
 #undef MS
 }
 

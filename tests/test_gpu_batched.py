@@ -72,11 +72,13 @@ def _rel_terms(got, ref):
     return np.max(np.abs(got - ref["terms"]) / scale)
 
 
-@pytest.mark.parametrize("nocc,nvir,kchunk", [(8, 40, 8), (5, 37, 2), (7, 70, 4)])
+@pytest.mark.parametrize("nocc,nvir,kchunk", [(8, 40, 8), (5, 37, 2), (7, 70, 4), (1, 1, 1),
+                                              (2, 5, 1), (3, 14, 3)])
 def test_two_wave_tile_against_oracle(cc, orc, nocc, nvir, kchunk):
     """The two-wave FP64 tile (plain, persistent at two grid sizes, graph) meets
-    the per-term bar against the oracle; shapes cover nvir % 16 != 0 and odd
-    K-step counts (nvir + nocc = 42, 42, 77 -> 11, 11, 20 steps of 4)."""
+    the per-term bar against the oracle; shapes cover nvir % 16 != 0 and the
+    step-pair panel layout with and without an odd last K step (nvir + nocc =
+    48, 42, 77, 2, 7, 17 -> 12, 11, 20, 1, 2, 5 steps of 4: no pair at 1)."""
     arrays = cc.make_inputs(nocc, nvir, kchunk, 301)
     trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
     ref = orc.run_list(arrays, 0, nocc, nvir, trip, with_abs=True)
