@@ -86,6 +86,13 @@ __device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// A 16-byte load through a global (not flat) pointer: a flat load would also
+// count against the LDS counter, tying every LDS wait to its latency.
+__device__ __forceinline__ dbl2 gload2(const double *p)
+{
+    return *(const __attribute__((address_space(1))) dbl2 *)p;
+}
+
 // Two doubles per lane in one 16-byte load.  On gfx950 a wave's 8-byte load
 // takes as long in the texture data path as a 16-byte one (about 16 cycles per
 // CU), so the step-pair panel layout below halves that path's time per MFMA.
@@ -699,17 +706,33 @@ struct PlaneMath<true> {
 
 // stash: this wave's 8 KiB of LDS, holding group 0's four products while group
 // 1's digit products run, so their 32 VGPRs are free during the INT8 loops.
+// sst: this wave's 1 KiB of LDS for the row scales of its eight panels (p = A0,
+// A1 of group 0, A0, A1 of group 1, then B0, B1 of each group; 16 rows each of
+// block bi for A, ci for B), fetched with one 16-byte load per lane.
 template <int S, int W, bool BF = false>
 __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
                                            const __amdgpu_buffer_rsrc_t rt,
                                            const double *__restrict__ scales, const Geom &g,
                                            const SplitGeom &sg, const PanelPairs &pp, int bi,
-                                           int ci, int lane, double *__restrict__ stash)
+                                           int ci, int lane, double *__restrict__ stash,
+                                           double *__restrict__ sst)
 {
     // panel ids: pair q holds panels 2q, 2q+1
     const int LJ = 2 * pp.li, LK = LJ + 1, LJT = 2 * pp.lk, LKT = LJT + 1;
     const int R1 = 2 * pp.rkj, R3 = R1 + 1, R1T = 2 * pp.rij, R3T = R1T + 1;
     const int nvp = g.nvp;
+    dbl2 sreg;
+    {
+        const int sp = lane >> 3;
+        const int L4[4] = {LJ, LK, LJT, LKT}, R4[4] = {R1, R3, R1T, R3T};
+        int pid = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pid = sp == q ? (W == 0 ? L4[q] : R4[q]) : pid;
+            pid = sp == 4 + q ? (W == 0 ? R4[q] : L4[q]) : pid;
+        }
+        sreg = gload2(scales + (int64_t)pid * nvp + (sp < 4 ? bi : ci) * 16 + (lane & 7) * 2);
+    }
     const uint32_t plane = (uint32_t)sg.slice_bytes;
     const uint32_t blk8 = (uint32_t)sg.nkv * 1024;  // bytes of a 16-row block of one plane
     const uint32_t voff8 = lane * 16;
@@ -760,18 +783,18 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
         // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * w(L) (INT8 w(L) =
         // 2^(-7(L+2)), BF16 w = 1); the combination (a, b) is product
         // 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1)
+        if (gi == 0) *reinterpret_cast<dbl2 *>(sst + 2 * lane) = sreg;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int ca = c & 1, cb = c >> 1;
-            const int pa = ca ? A1 : A0, pb = cb ? B1 : B0;
             const int lp = 4 * gi + (W == 0 ? ca + 2 * cb : 2 * ca + cb);
-            const double sB = scales[(int64_t)pb * nvp + cx];
+            const double sB = sst[(4 + 2 * gi + cb) * 16 + (lane & 15)];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 double v = 0.0;
 #pragma unroll
                 for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * PlaneMath<BF>::weight(q);
-                const double sA = scales[(int64_t)pa * nvp + bi * 16 + row0 + r];
+                const double sA = sst[(2 * gi + ca) * 16 + row0 + r];
                 const double f = v * sA * sB;
                 if (gi == 0)
                     stash[(lp * 4 + r) * 64 + lane] = f;  // lp = 0..3
@@ -994,24 +1017,39 @@ template <typename T> struct Vec4;
 template <> struct Vec4<double> { typedef dbl4 type; };
 template <> struct Vec4<float> { typedef flt4 type; };
 
+// The energy vectors of the item's two blocks: vst[q][s][i] holds eorb (q = 0)
+// or vector q-1 (t1v1 t1v2 dintx1 dintx2 dintc1 dintc2) at row s ? c : b block
+// * 16 + i.  Each wave fetches half the table with one 16-byte load per lane at
+// the start of the item (stage_vectors); pair_epilogue writes it to LDS.
+__device__ __forceinline__ dbl2 stage_vectors(const JobDev &jd, const Task &tk, int bi, int ci, int w,
+                                              int lane)
+{
+    dbl2 v = {0.0, 0.0};
+    const int e = w * 112 + 2 * lane;  // element pair of this lane (lanes 0..55)
+    if (lane < 56) {
+        const int q = e >> 5, side = (e >> 4) & 1, i = e & 15;
+        int row = 0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) row = q == r + 1 ? uniform(tk.vrow[r]) : row;
+        const double *src = q == 0 ? jd.ev : jd.vec6 + (int64_t)row * jd.g.nvp;
+        v = gload2(src + (side ? ci : bi) * 16 + i);
+    }
+    return v;
+}
+
 template <typename T, bool TENGY, bool WRITEF>
 __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
                                               const typename Vec4<T>::type (&facc)[8], int bi, int ci,
                                               int w, int lane, double *__restrict__ part4,
-                                              double *__restrict__ fo, void *lds)
+                                              double *__restrict__ fo, void *lds, dbl2 vreg,
+                                              double *__restrict__ vst)
 {
     // the workgroup's 16 KiB exchange buffer (each wave's half first held its
     // stash, consumed before the products were finished)
     T(*xch)[8][2][64] = reinterpret_cast<T(*)[8][2][64]>(lds);
     __shared__ double red[2][4];
     const Geom &g = jd.g;
-    const double *__restrict__ ev = jd.ev;
-    const double *__restrict__ vec6 = jd.vec6;
-    const int nvp = g.nvp;
     const int row0 = 4 * (lane >> 4);
-    int vo[6];  // offsets of the task's six vectors in the pool (uniform)
-#pragma unroll
-    for (int q = 0; q < 6; ++q) vo[q] = uniform(tk.vrow[q] * nvp);
     const int cx = ci * 16 + (lane & 15);
     const bool diag = (bi == ci);
     const int nv = g.nvir;
@@ -1051,12 +1089,17 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
 #pragma unroll
                 for (int h = 0; h < 2; ++h) xch[w][q][h][lane] = facc[q][(w == 0 ? 2 : 0) + h];
         }
+        if (lane < 56) *reinterpret_cast<dbl2 *>(vst + w * 112 + 2 * lane) = vreg;
         __syncthreads();
+        // vst[q][s][i]: q = eorb, t1v1 t1v2 dintx1 dintx2 dintc1 dintc2
+        auto vb = [&](int q, int i) { return (T)vst[q * 32 + i]; };       // b block, row i
+        auto vc = [&](int q, int i) { return (T)vst[q * 32 + 16 + i]; };  // c block, column i
+        const int lc = lane & 15;
         SumT<T> s = {0, 0, 0, 0};
-        const T ec = (T)ev[cx];
-        const T u1c = (T)vec6[vo[0] + cx], u2c = (T)vec6[vo[1] + cx];
-        const T x1c = (T)vec6[vo[2] + cx], x2c = (T)vec6[vo[3] + cx];
-        const T y1c = (T)vec6[vo[4] + cx], y2c = (T)vec6[vo[5] + cx];
+        const T ec = vc(0, lc);
+        const T u1c = vc(1, lc), u2c = vc(2, lc);
+        const T x1c = vc(3, lc), x2c = vc(4, lc);
+        const T y1c = vc(5, lc), y2c = vc(6, lc);
         const T eaijk = std::is_same<T, float>::value ? (T)tk.eaijk32 : (T)tk.eaijk;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -1078,15 +1121,15 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
             }
             const int bx = bi * 16 + row0 + r;
             if (bx < nv && cx < nv) {
-                const T dd = (T)(-1.0 / (double)((T)ev[bx] + ec + eaijk));
+                const int lb = row0 + r;
+                const T dd = (T)(-1.0 / (double)(vb(0, lb) + ec + eaijk));
                 tengy_point<T>(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
                                mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
-                               dd, (T)vec6[vo[0] + bx], (T)vec6[vo[1] + bx], x1c, x2c, y1c, y2c, s);
+                               dd, vb(1, lb), vb(2, lb), x1c, x2c, y1c, y2c, s);
                 if (bi != ci)
                     tengy_point<T>(mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
                                    dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
-                                   dd, u1c, u2c, (T)vec6[vo[2] + bx], (T)vec6[vo[3] + bx],
-                                   (T)vec6[vo[4] + bx], (T)vec6[vo[5] + bx], s);
+                                   dd, u1c, u2c, vb(3, lb), vb(4, lb), vb(5, lb), vb(6, lb), s);
             }
         }
         s.e4i = wave_sum(s.e4i);
@@ -1113,6 +1156,9 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
 {
     const bool diag = (bi == ci);
     __shared__ double lds_x[2 * 8 * 2 * 64];
+    __shared__ __attribute__((aligned(16))) double lds_v[7 * 2 * 16];   // energy vectors
+    __shared__ __attribute__((aligned(16))) double lds_s[2][8 * 16];    // split: row scales
+    const dbl2 vreg = TENGY ? stage_vectors(jd, tk, bi, ci, w, lane) : dbl2{0.0, 0.0};
     if constexpr (S == kSlicesFp32) {
         flt4 f32[8];
 #pragma unroll
@@ -1122,7 +1168,7 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
             fp32_wave<0>(f32, pan32, jd.g, pp, bi, ci, lane);
         else if (!diag)
             fp32_wave<1>(f32, pan32, jd.g, pp, bi, ci, lane);
-        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo, lds_x);
+        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v);
     } else {
         dbl4 facc[8];
 #pragma unroll
@@ -1141,11 +1187,12 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
                                                           (uint32_t)(sg.npanels * NP * sg.slice_bytes));
             const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
             if (w == 0)
-                split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x);
+                split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x, lds_s[0]);
             else if (!diag)
-                split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x + 1024);
+                split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x + 1024,
+                                      lds_s[1]);
         }
-        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo, lds_x);
+        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v);
     }
 }
 
