@@ -86,11 +86,16 @@ __device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-// A 16-byte load through a global (not flat) pointer: a flat load would also
-// count against the LDS counter, tying every LDS wait to its latency.
+// Loads through a global (not flat) pointer: a flat load would also count
+// against the LDS counter, tying every LDS wait to its latency.
 __device__ __forceinline__ dbl2 gload2(const double *p)
 {
     return *(const __attribute__((address_space(1))) dbl2 *)p;
+}
+
+__device__ __forceinline__ double gld(const double *p)
+{
+    return *(const __attribute__((address_space(1))) double *)p;
 }
 
 // Two doubles per lane in one 16-byte load.  On gfx950 a wave's 8-byte load
@@ -138,6 +143,39 @@ __device__ __forceinline__ ItemRange item_range(int total, int b)
 // the caller's item work must end with a barrier before the next call.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// A read of memory no kernel writes during the launch (tasks, execution order,
+// block pairs: all copied from the host before it) through the constant
+// address space, so a wave-uniform address becomes a scalar load.
+template <typename T>
+__device__ __forceinline__ T cread(const T *p)
+{
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// A task record and a block pair read field by field with cread (an aggregate
+// copy would drop the address space).
+__device__ __forceinline__ Task read_task(const Task *p)
+{
+    Task t;
+    t.li = cread(&p->li);
+    t.lk = cread(&p->lk);
+    t.rij = cread(&p->rij);
+    t.rkj = cread(&p->rkj);
+    t.ik_differ = cread(&p->ik_differ);
+    t.eaijk32 = cread(&p->eaijk32);
+    t.eaijk = cread(&p->eaijk);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t.vrow[q] = cread(&p->vrow[q]);
+    t.pad = 0;
+    return t;
+}
+
+__device__ __forceinline__ short2 read_pair(const short2 *p)
+{
+    const int v = cread(reinterpret_cast<const int *>(p));
+    return make_short2((short)(v & 0xffff), (short)(v >> 16));
+}
+
 // Item of workgroup b in a plain launch (grid = item count), or -1.
 __device__ __forceinline__ int plain_item(const ItemRange &ir)
 {
@@ -169,7 +207,7 @@ __device__ __forceinline__ void decode_item(const Segment &sg, int npairs, int i
     if (sg.pmajor) {
         p = uniform(L / sg.ntasks);
         const int tt = L - p * sg.ntasks;
-        t = uniform(sg.order ? sg.order[tt] : tt);
+        t = uniform(sg.order ? cread(sg.order + tt) : tt);
     } else {
         t = uniform(L / npairs);
         p = uniform(L - t * npairs);
@@ -431,24 +469,24 @@ __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, con
     }
     if (TENGY) {
         Sum4 s = {0.0, 0.0, 0.0, 0.0};
-        const double ec = ev[cx];
-        const double u1c = vec6[vo[0] + cx], u2c = vec6[vo[1] + cx];
-        const double x1c = vec6[vo[2] + cx], x2c = vec6[vo[3] + cx];
-        const double y1c = vec6[vo[4] + cx], y2c = vec6[vo[5] + cx];
+        const double ec = gld(ev + cx);
+        const double u1c = gld(vec6 + vo[0] + cx), u2c = gld(vec6 + vo[1] + cx);
+        const double x1c = gld(vec6 + vo[2] + cx), x2c = gld(vec6 + vo[3] + cx);
+        const double y1c = gld(vec6 + vo[4] + cx), y2c = gld(vec6 + vo[5] + cx);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int bx = bi * 16 + row0 + 4 * r;
             if (bx < nv && cx < nv) {
-                const double d = -1.0 / (ev[bx] + ec + tk.eaijk);
-                const double u1b = vec6[vo[0] + bx], u2b = vec6[vo[1] + bx];
+                const double d = -1.0 / (gld(ev + bx) + ec + tk.eaijk);
+                const double u1b = gld(vec6 + vo[0] + bx), u2b = gld(vec6 + vo[1] + bx);
                 tengy_point(acc[0][r], acc[1][r], acc[2][r], acc[3][r],
                             acc[4][r], acc[5][r], acc[6][r], acc[7][r],
                             acc[8][r], acc[9][r], acc[10][r], acc[11][r],
                             acc[12][r], acc[13][r], acc[14][r], acc[15][r],
                             d, u1b, u2b, x1c, x2c, y1c, y2c, s);
                 if (bi != ci) {  // the (c,b) point of the same pair of blocks
-                    const double x1b = vec6[vo[2] + bx], x2b = vec6[vo[3] + bx];
-                    const double y1b = vec6[vo[4] + bx], y2b = vec6[vo[5] + bx];
+                    const double x1b = gld(vec6 + vo[2] + bx), x2b = gld(vec6 + vo[3] + bx);
+                    const double y1b = gld(vec6 + vo[4] + bx), y2b = gld(vec6 + vo[5] + bx);
                     tengy_point(acc[8][r], acc[9][r], acc[10][r], acc[11][r],
                                 acc[12][r], acc[13][r], acc[14][r], acc[15][r],
                                 acc[0][r], acc[1][r], acc[2][r], acc[3][r],
@@ -484,8 +522,8 @@ __global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
         const JobDev jd = jobs[uniform(sg.job)];
         int t, p;
         decode_item(sg, jd.g.npairs, item, t, p);
-        const Task tk = sg.tasks[t];
-        const short2 pr = jd.pairs[p];
+        const Task tk = read_task(sg.tasks + t);
+        const short2 pr = read_pair(jd.pairs + p);
         fused_item<TENGY, WRITEF>(jd, tk, panel_pairs(tk), uniform(pr.x), uniform(pr.y), threadIdx.x,
                                   sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
                                   WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
@@ -1214,8 +1252,8 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
         const JobDev jd = jobs[uniform(sg.job)];
         int t, p;
         decode_item(sg, jd.g.npairs, item, t, p);
-        const Task tk = sg.tasks[t];
-        const short2 pr = jd.pairs[p];
+        const Task tk = read_task(sg.tasks + t);
+        const short2 pr = read_pair(jd.pairs + p);
         // an opaque copy of the lane index: keeps the compiler from hoisting
         // lane-dependent offsets out of the item loop, where they would hold
         // VGPRs through the whole item
