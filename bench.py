@@ -360,6 +360,10 @@ def main():
                          "inputs (nocc=30, nvir=300)")
     ap.add_argument("--batched-slices", type=int, default=0,
                     help="slices of the batched split form (0: chosen by the autotuner)")
+    ap.add_argument("--load-rate", action="store_true",
+                    help="only measure the per-instruction cost of wave-wide loads of 4, 8 "
+                         "and 16 bytes per lane (and 16-byte global->LDS DMA) on L1-resident "
+                         "data")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (reference point for the "
                          "FETCH_SIZE/WRITE_SIZE counters)")
@@ -400,6 +404,19 @@ def main():
     if args.batched:
         print(json.dumps(batched_report(cc, args.steps, args.warmup, args.batched_slices)),
               flush=True)
+        return
+    if args.load_rate:
+        clock_ghz = 2.4  # MI355X peak engine clock
+        rows = []
+        for nbytes, lds in ((4, 0), (8, 0), (16, 0), (16, 1)):
+            ns = cc.lib().ccsdt_debug_load_rate(nbytes, lds)
+            if ns < 0:
+                raise RuntimeError("load-rate probe failed: %s" % cc.lib().ccsdt_last_error_string())
+            rows.append({"bytes_per_lane": nbytes, "to_lds": bool(lds), "ns_per_wave_load_per_cu": ns,
+                         "cycles_at_2.4GHz": ns * clock_ghz,
+                         "bytes_per_cycle_per_cu": 64 * nbytes / (ns * clock_ghz)})
+        print(json.dumps({"metric": "vector-memory cost per wave-wide load (L1-resident, every CU)",
+                          "runs": rows}), flush=True)
         return
     nocc, nvir = 30, 300
     arrays, blocks, trip, kchunk = cc.config_case(1)

@@ -374,6 +374,12 @@ int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d,
  * average milliseconds per copy (or -1): the reference point for HBM counters.
  * No counterpart in the reference. */
 double ccsdt_debug_copy(int64_t bytes, int reps);
+/* Nanoseconds per wave-wide load per CU when every CU streams L1-resident data
+ * with loads of `bytes` (4, 8 or 16) per lane into VGPRs, or (to_lds = 1, 16
+ * bytes) as global->LDS DMA; multiply by the shader clock for cycles.  No
+ * counterpart in the reference: measures the per-instruction cost of the vector
+ * memory path that bounds the split kernel's digit loop.  -1 on failure. */
+double ccsdt_debug_load_rate(int bytes, int to_lds);
 
 #ifdef __cplusplus
 }

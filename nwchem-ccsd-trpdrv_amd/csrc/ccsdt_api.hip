@@ -2068,6 +2068,41 @@ int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d,
     return CCSDT_OK;
 }
 
+double ccsdt_debug_load_rate(int bytes, int to_lds)
+{
+    if (ensure_device(-1)) return -1.0;
+    if (!(bytes == 4 || bytes == 8 || bytes == 16) || (to_lds && bytes != 16)) {
+        set_error(CCSDT_E_ARG, "load-rate probe: bytes must be 4, 8 or 16 (16 with to_lds)");
+        return -1.0;
+    }
+    const int grid = 8 * g_num_cu, iters = 2000;
+    uint32_t *buf = nullptr, *out = nullptr;
+    if (dev_alloc((void **)&buf, 16384 * 64) || dev_alloc((void **)&out, (size_t)grid * 256 * 4)) {
+        if (buf) dev_free(buf);
+        return -1.0;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = -1.f;
+    bool ok = hipMemset(buf, 1, 16384 * 64) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess &&
+              launch_load_rate(buf, out, bytes, to_lds != 0, grid, iters, nullptr) == hipSuccess &&
+              hipEventRecord(e0, nullptr) == hipSuccess &&
+              launch_load_rate(buf, out, bytes, to_lds != 0, grid, iters, nullptr) == hipSuccess &&
+              hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+              hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(buf);
+    dev_free(out);
+    if (!ok) {
+        set_error(CCSDT_E_DEVICE, "load-rate probe failed");
+        return -1.0;
+    }
+    // wave-wide loads per CU: grid/CUs workgroups x 4 waves x iters x 8
+    const double loads_per_cu = (double)grid / g_num_cu * 4.0 * iters * 8.0;
+    return ms * 1e6 / loads_per_cu;
+}
+
 double ccsdt_debug_copy(int64_t bytes, int reps)
 {
     if (ensure_device(-1)) return -1.0;

@@ -1742,4 +1742,55 @@ hipError_t launch_copy(const double *src, double *dst, int64_t n, hipStream_t st
     return hipGetLastError();
 }
 
+// Load-rate probe: every wave streams `iters` x 8 wave-wide loads of W dwords
+// per lane from its workgroup's 16 KiB region of `buf` (64 regions, 1 MiB; the
+// offsets stay below 10 KiB into the region), to VGPRs (LDS = false) or, W = 4
+// only, as 16-byte global->LDS DMA into eight fixed 1 KiB slots of a 32 KiB
+// array.  `stride` (512) comes at run time so no load can be hoisted.
+typedef __attribute__((address_space(3))) void lds_void;
+template <int W, bool LDS>
+__global__ __launch_bounds__(256) void k_load_rate(const uint32_t *__restrict__ buf,
+                                                   uint32_t *__restrict__ out, int iters, int stride)
+{
+    __shared__ __attribute__((aligned(16))) uint32_t slots[4][2048];
+    const __amdgpu_buffer_rsrc_t r = buffer_rsrc(reinterpret_cast<const double *>(buf), 16384u * 64u);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t base = (blockIdx.x & 63) * 16384u;
+    uint32_t acc = 0;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t off = base + lane * (LDS ? 16 : 4 * W) + (uint32_t)(((it * 8 + u) * stride) & 8191);
+            if constexpr (LDS) {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)&slots[0][u * 256], 16, off, 0, 0, 0);
+            } else if constexpr (W == 1) {
+                acc += __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+            } else if constexpr (W == 2) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+                acc += v[0] + 3u * v[1];
+            } else {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+                acc += v[0] + 3u * v[1] + 5u * v[2] + 7u * v[3];
+            }
+        }
+    }
+    if constexpr (LDS) {
+        __builtin_amdgcn_s_waitcnt(0);
+        acc += slots[0][lane];
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+hipError_t launch_load_rate(const uint32_t *buf, uint32_t *out, int bytes, bool lds, int grid,
+                            int iters, hipStream_t st)
+{
+    const dim3 g(grid), b(256);
+    if (lds && bytes == 16) hipLaunchKernelGGL((k_load_rate<4, true>), g, b, 0, st, buf, out, iters, 512);
+    else if (!lds && bytes == 4) hipLaunchKernelGGL((k_load_rate<1, false>), g, b, 0, st, buf, out, iters, 512);
+    else if (!lds && bytes == 8) hipLaunchKernelGGL((k_load_rate<2, false>), g, b, 0, st, buf, out, iters, 512);
+    else if (!lds && bytes == 16) hipLaunchKernelGGL((k_load_rate<4, false>), g, b, 0, st, buf, out, iters, 512);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 }  // namespace ccsdt

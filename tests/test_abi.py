@@ -46,6 +46,7 @@ def test_no_device_is_an_error(cc):
     L = cc.lib()
     assert L.ccsdt_init(0) == -2
     assert "device" in L.ccsdt_last_error_string().decode().lower()
+    assert L.ccsdt_debug_load_rate(16, 0) == -1.0
 
 
 def test_dropin_without_device_reports_nan(cc):

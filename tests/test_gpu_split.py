@@ -175,3 +175,14 @@ def test_bf16x3_full_list_and_schedules(cc):
     E = et(a["emp4"], a["emp5"])
     assert abs(et(b["emp4"], b["emp5"]) - E) / abs(E) < ET_TOL
     assert np.array_equal(b["terms"], c["terms"])
+
+
+def test_load_rate_probe(cc):
+    """The vector-memory probe behind the split kernel's design runs, rejects
+    widths it has no kernel for, and finds a 16-byte load no cheaper per
+    instruction than a 4-byte one."""
+    L = cc.lib()
+    ns = {(b, l): L.ccsdt_debug_load_rate(b, l) for b, l in ((4, 0), (8, 0), (16, 0), (16, 1))}
+    assert all(0 < v < 1e4 for v in ns.values()), ns
+    assert ns[(16, 0)] >= ns[(4, 0)]
+    assert L.ccsdt_debug_load_rate(2, 0) == -1.0 and L.ccsdt_debug_load_rate(8, 1) == -1.0
