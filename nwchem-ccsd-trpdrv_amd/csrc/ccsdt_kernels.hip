@@ -752,7 +752,7 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
                                            const __amdgpu_buffer_rsrc_t rt,
                                            const double *__restrict__ scales, const Geom &g,
                                            const SplitGeom &sg, const PanelPairs &pp, int bi,
-                                           int ci, int lane, double *__restrict__ stash,
+                                           int ci, int lane, double *stash,
                                            double *__restrict__ sst)
 {
     // panel ids: pair q holds panels 2q, 2q+1
@@ -796,28 +796,50 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
         const uint32_t oa1 = (uint32_t)A1 * S * plane + bi * blk8;
         const uint32_t ob0 = (uint32_t)B0 * S * plane + ci * blk8;
         const uint32_t ob1 = (uint32_t)B1 * S * plane + ci * blk8;
-        for (int ks = 0; ks < sg.nkv; ++ks) {
-            const uint32_t ko = ks * 1024;
-            int4v a0[S], a1[S], b0[S], b1[S];
-#pragma unroll
-            for (int q = 0; q < S; ++q) {
-                a0[q] = buffer_load_i8x16(rd, voff8, oa0 + q * plane + ko);
-                a1[q] = buffer_load_i8x16(rd, voff8, oa1 + q * plane + ko);
-                b0[q] = buffer_load_i8x16(rd, voff8, ob0 + q * plane + ko);
-                b1[q] = buffer_load_i8x16(rd, voff8, ob1 + q * plane + ko);
+        // ping-pong operand buffers X and Y, two K steps per trip: each buffer's
+        // planes are in flight while the other buffer's digit products run; an
+        // odd last step runs from X after the loop (steps stay in order)
+        int4v xa0[S], xa1[S], xb0[S], xb1[S], ya0[S], ya1[S], yb0[S], yb1[S];
+#define SPLIT_LOAD(A0_, A1_, B0_, B1_, kstep)                                                    \
+        _Pragma("unroll") for (int q = 0; q < S; ++q) {                                         \
+            const uint32_t ko_ = (uint32_t)(kstep) * 1024;                                        \
+            A0_[q] = buffer_load_i8x16(rd, voff8, oa0 + q * plane + ko_);                        \
+            A1_[q] = buffer_load_i8x16(rd, voff8, oa1 + q * plane + ko_);                        \
+            B0_[q] = buffer_load_i8x16(rd, voff8, ob0 + q * plane + ko_);                        \
+            B1_[q] = buffer_load_i8x16(rd, voff8, ob1 + q * plane + ko_);                        \
+        }
+#define SPLIT_MFMA(A0_, A1_, B0_, B1_)                                                           \
+        _Pragma("unroll") for (int sa = 0; sa < S; ++sa) {                                      \
+            _Pragma("unroll") for (int sb = 0; sb < S - sa; ++sb) {                             \
+                const int lv = sa + sb;                                                          \
+                iacc[0][lv] = PlaneMath<BF>::mfma(A0_[sa], B0_[sb], iacc[0][lv]);                 \
+                iacc[1][lv] = PlaneMath<BF>::mfma(A1_[sa], B0_[sb], iacc[1][lv]);                 \
+                iacc[2][lv] = PlaneMath<BF>::mfma(A0_[sa], B1_[sb], iacc[2][lv]);                 \
+                iacc[3][lv] = PlaneMath<BF>::mfma(A1_[sa], B1_[sb], iacc[3][lv]);                 \
+            }                                                                                    \
+        }
+        const int nkv = sg.nkv;
+        if constexpr (S <= 4 && !BF) {
+            SPLIT_LOAD(xa0, xa1, xb0, xb1, 0)
+            int ks = 0;
+            for (; ks + 1 < nkv; ks += 2) {
+                SPLIT_LOAD(ya0, ya1, yb0, yb1, ks + 1)
+                SPLIT_MFMA(xa0, xa1, xb0, xb1)
+                const int k2 = (ks + 2 < nkv) ? ks + 2 : nkv - 1;  // clamped prefetch, in bounds
+                SPLIT_LOAD(xa0, xa1, xb0, xb1, k2)
+                SPLIT_MFMA(ya0, ya1, yb0, yb1)
             }
-#pragma unroll
-            for (int sa = 0; sa < S; ++sa) {
-#pragma unroll
-                for (int sb = 0; sb < S - sa; ++sb) {
-                    const int lv = sa + sb;
-                    iacc[0][lv] = PlaneMath<BF>::mfma(a0[sa], b0[sb], iacc[0][lv]);
-                    iacc[1][lv] = PlaneMath<BF>::mfma(a1[sa], b0[sb], iacc[1][lv]);
-                    iacc[2][lv] = PlaneMath<BF>::mfma(a0[sa], b1[sb], iacc[2][lv]);
-                    iacc[3][lv] = PlaneMath<BF>::mfma(a1[sa], b1[sb], iacc[3][lv]);
-                }
+            if (nkv & 1) SPLIT_MFMA(xa0, xa1, xb0, xb1)
+        } else {
+            // five or more planes: a second buffer would cost a wave per SIMD
+            // (the BF16 planes measured slower with it)
+            for (int ks = 0; ks < nkv; ++ks) {
+                SPLIT_LOAD(xa0, xa1, xb0, xb1, ks)
+                SPLIT_MFMA(xa0, xa1, xb0, xb1)
             }
         }
+#undef SPLIT_LOAD
+#undef SPLIT_MFMA
         // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * w(L) (INT8 w(L) =
         // 2^(-7(L+2)), BF16 w = 1); the combination (a, b) is product
         // 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1)
@@ -840,6 +862,10 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
                     facc[lp][r] = f;
             }
         }
+        // a compiler fence (stash is not restrict-qualified, so it covers the
+        // stash): without it the stores are forwarded to the reloads below and
+        // group 0's values stay in registers through group 1 after all
+        if (gi == 0) asm volatile("" ::: "memory");
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
