@@ -2107,24 +2107,31 @@ double ccsdt_debug_copy(int64_t bytes, int reps)
 {
     if (ensure_device(-1)) return -1.0;
     const int64_t n = bytes / 16 * 2;
-    double *src = nullptr, *dst = nullptr;
-    if (dev_alloc((void **)&src, n * 8) || dev_alloc((void **)&dst, n * 8)) return -1.0;
-    hipEvent_t e0, e1;
-    float ms = 0.f;
-    if (hipMemset(src, 0, n * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess)
+    if (n <= 0 || reps < 1) {
+        set_error(CCSDT_E_ARG, "copy probe: bytes %lld, reps %d", (long long)bytes, reps);
         return -1.0;
-    (void)hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps; ++r)
-        if (launch_copy(src, dst, n, nullptr) != hipSuccess) return -1.0;
-    (void)hipEventRecord(e1, nullptr);
-    (void)hipEventSynchronize(e1);
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    }
+    double *src = nullptr, *dst = nullptr;
+    if (dev_alloc((void **)&src, n * 8) || dev_alloc((void **)&dst, n * 8)) {
+        if (src) dev_free(src);
+        return -1.0;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    bool ok = hipMemset(src, 0, n * 8) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess && hipEventRecord(e0, nullptr) == hipSuccess;
+    for (int r = 0; r < reps && ok; ++r) ok = launch_copy(src, dst, n, nullptr) == hipSuccess;
+    ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     dev_free(src);
     dev_free(dst);
-    return reps > 0 ? ms / reps : 0.0;
+    if (!ok) {
+        set_error(CCSDT_E_DEVICE, "copy probe failed");
+        return -1.0;
+    }
+    return ms / reps;
 }
 
 void ccsd_trpdrv_omp_cbody_(double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
