@@ -349,6 +349,29 @@ def tensor_footprint(nocc, nvir, ntriples, ncor=0, driver=False):
     return lib().ccsdt_tensor_job_footprint(ncor, nocc, nvir, ntriples, 1 if driver else 0)
 
 
+def make_harness_inputs(nocc, nvir, nkpass=1, ncor=0):
+    """The reference harness's own inputs (omp/test.c:55-62, 94-146): every array
+    filled by make_array, a[i] = 1/(100+i), at the harness's allocation sizes,
+    with kchunk = (nocc-1)/nkpass + 1 (omp/test.c:97).  Returns (arrays, kchunk)."""
+    kchunk = (nocc - 1) // nkpass + 1
+    vv, ov = nvir * nvir, nocc * nvir
+    sizes = {"eorb": ncor + nocc + nvir, "tij": vv, "tkj": kchunk * vv, "tia": ov * nocc,
+             "tka": kchunk * ov, "xia": ov * nocc, "xka": kchunk * ov, "jia": vv,
+             "jka": kchunk * vv, "kia": vv, "kka": kchunk * vv, "jij": ov * nocc,
+             "jkj": kchunk * ov, "kij": ov * nocc, "kkj": kchunk * ov}
+    for name in VECTOR_NAMES:
+        sizes[name] = vv
+    return {name: 1.0 / (100.0 + np.arange(n)) for name, n in sizes.items()}, kchunk
+
+
+def harness_triples(nocc, kchunk, maxiter=100):
+    """The calls of omp/test.c:160-182 in order: k blocks of kchunk, j and i over
+    all occupied, k from klo to min(khi, i), a = 1, stopping after maxiter calls
+    (maxiter < 0: all)."""
+    trip = triple_list(nocc, blocks_uniform(nocc, kchunk), "ref", a=1)
+    return trip if maxiter < 0 else trip[:maxiter]
+
+
 def blocks_uniform(nocc, length):
     """k blocks klo..khi of a fixed length (kchunk, omp/test.c:97,160-161)."""
     length = max(1, min(int(length), nocc))

@@ -6,6 +6,8 @@ summands of that term); totals within 1e-11 relative.  f matrices within 1e-13
 relative to their largest element.
 """
 import os
+import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -13,6 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 TOL = 1e-11
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -206,3 +209,72 @@ def test_config1_full_list_properties(cc):
     with cc.Job(sc, 30, 300, kchunk) as job:
         c = job.run(trip[::37], terms=True)
     assert np.array_equal(c["terms"], a["terms"][::37] * np.array([16.0, 4.0, 16.0, 4.0]))
+
+
+def wrap1000(x):
+    """The harness's guard against overflow, omp/test.c:188-191."""
+    if x > 1000.0:
+        x -= 1000.0
+    if x < -1000.0:
+        x += 1000.0
+    return x
+
+
+def harness_replay(orc, arrays, nocc, nvir, trip):
+    """(emp4, emp5, scale) after each call of omp/test.c's loop: the oracle's
+    terms accumulated with the body's i != k rule
+    (omp/ccsd_trpdrv_omp_cbody.c:133-142) and the harness's wrap; scale is the
+    running sum of the absolute summands of every term."""
+    r = orc.run_list(arrays, 0, nocc, nvir, trip, with_abs=True)
+    e4 = e5 = s = 0.0
+    out = []
+    for q, (a, i, j, k, klo) in enumerate(trip):
+        t = r["terms"][q]
+        e4 += t[0]
+        e5 += t[1]
+        if i != k:
+            e4 += t[2]
+            e5 += t[3]
+        e4, e5 = wrap1000(e4), wrap1000(e5)
+        s += float(np.sum(r["abs4"][q]))
+        out.append((e4, e5, s))
+    return out
+
+
+@pytest.mark.parametrize("nocc,nvir,maxiter,nkpass", [(8, 40, 100, 1), (8, 40, -1, 2), (5, 33, -1, 1)])
+def test_harness_loop_through_dropin(cc, orc, nocc, nvir, maxiter, nkpass):
+    """omp/test.c's main loop with its own inputs (make_array, a = 1, its k
+    blocks, maxiter, the +-1000 wrap), calling the drop-in body: the running
+    totals after every call match the oracle's replay of the same loop."""
+    arrays, kchunk = cc.make_harness_inputs(nocc, nvir, nkpass)
+    trip = cc.harness_triples(nocc, kchunk, maxiter)
+    want = harness_replay(orc, arrays, nocc, nvir, trip)
+    f = [np.zeros(nvir * nvir) for _ in range(8)]
+    e4 = e5 = 0.0
+    for q, (a, i, j, k, klo) in enumerate(trip):
+        e4, e5 = cc.cbody(f, arrays["eorb"], 0, nocc, nvir, e4, e5, int(a), int(i), int(j),
+                          int(k), int(klo), arrays)
+        e4, e5 = wrap1000(e4), wrap1000(e5)
+        w4, w5, scale = want[q]
+        assert abs(e4 - w4) <= TOL * scale and abs(e5 - w5) <= TOL * scale, q
+
+
+@pytest.mark.parametrize("binary", ["test_c99_dropin", "test_f77_dropin"])
+def test_reference_harness_binary(cc, orc, binary):
+    """The reference harness itself (omp/test.c, built by oracle/Makefile against
+    this library; test_f77_dropin calls the Fortran body's symbol) runs and
+    prints the totals of the oracle's replay of its loop, to its %f precision."""
+    exe = os.path.join(ROOT, "oracle", "_ref", binary)
+    if not os.path.exists(exe):
+        pytest.skip("reference harness not built (reference sources absent at build time)")
+    for nocc, nvir, maxiter, nkpass in ((8, 40, -1, 2), (30, 300, 100, 1)):
+        out = subprocess.run([exe, str(nocc), str(nvir), str(maxiter), str(nkpass)],
+                             capture_output=True, text=True, timeout=180)
+        assert out.returncode == 0 and "SUCCESS" in out.stdout, out.stdout + out.stderr
+        m = re.search(r"emp4=(\S+) emp5=(\S+)", out.stdout)
+        got4, got5 = float(m.group(1)), float(m.group(2))
+        arrays, kchunk = cc.make_harness_inputs(nocc, nvir, nkpass)
+        w4, w5, scale = harness_replay(orc, arrays, nocc, nvir,
+                                       cc.harness_triples(nocc, kchunk, maxiter))[-1]
+        assert abs(got4 - w4) <= 6e-7 + TOL * scale and abs(got5 - w5) <= 6e-7 + TOL * scale
+
