@@ -1442,11 +1442,15 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
 
 // ------------------------------------------------------------------ standalone energy pass
 // Reads the eight f matrices of each task from HBM (ld = nvir, body order
-// f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t) in 32x32 tile pairs (BI >= CI).  The direct
-// tile is read coalesced along b; the mirrored tile f(c,b) is read coalesced
-// along c and transposed through LDS.  All 64 loads of a thread are issued
-// before the first transpose, so they are in flight together; the transposes
-// alternate between two LDS tiles (one barrier each).
+// f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t) in 32x32 tile pairs (BI >= CI), one
+// workgroup of 256 threads each.  The direct tile is read coalesced along b,
+// the mirrored tile f(c,b) along c and transposed through LDS; every thread
+// reads two adjacent rows per load (16 bytes), and the vectors of both tiles
+// come through LDS with one 16-byte load per thread: on gfx950 a wave-wide load
+// costs the texture-data path the same ~16 cycles at 8 or 16 bytes per lane,
+// and that path, not HBM, bounded the 8-byte form.  All 32 tile loads of a
+// thread are issued before the first transpose; the transposes alternate
+// between two LDS tiles (one barrier each).
 __host__ __device__ inline int tengy_tiles(int nv) { return (nv + 31) / 32; }
 
 int tengy_npairs(const Geom &g)
@@ -1462,6 +1466,7 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
                                                double *__restrict__ partial)
 {
     __shared__ double tile[2][32][33];
+    __shared__ __attribute__((aligned(16))) double vst[7 * 2 * 32];
     __shared__ double red[4][4];
     const int nv = g.nvir, nvp = g.nvp;
     const int nt = tengy_tiles(nv);
@@ -1472,74 +1477,105 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     int BI = 0;
     while ((BI + 1) * (BI + 2) / 2 <= p) ++BI;
     const int CI = p - BI * (BI + 1) / 2;
-    const Task tk = tasks[t];
+    const Task tk = read_task(tasks + t);
     const double *ft = f + (int64_t)t * fstride;
-    int vo[6];  // offsets of the task's six vectors in the pool (uniform)
-#pragma unroll
-    for (int q = 0; q < 6; ++q) vo[q] = uniform(tk.vrow[q] * nvp);
     const int64_t vv = (int64_t)nv * nv;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const int bx = BI * 32 + tx;
+    // thread (tx, ty): rows 2tx, 2tx+1 of the B tile, columns ty, ty+16 of the C tile
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
 
-    // byte offsets inside one matrix (0 for padding lanes, whose values are
-    // replaced by 0); the matrix is selected by the uniform offset m*vv*8
-    const __amdgpu_buffer_rsrc_t rf = buffer_rsrc(ft, (uint32_t)(8 * vv * 8));
-    uint32_t od[4], om[4];
-    bool okd[4], okm[4];
+    // eorb and the six vectors of both tiles, one 16-byte load per thread:
+    // vst[q*64 + side*32 + i] = vector q (eorb, t1v1 t1v2 dintx1 dintx2 dintc1
+    // dintc2) at row (side ? CI : BI)*32 + i; rows past nvp read as 0
+    dbl2 vreg = {0.0, 0.0};
+    {
+        const int e = 2 * (int)threadIdx.x;
+        if (e < 7 * 64) {
+            const int q = e >> 6, side = (e >> 5) & 1, i = e & 31;
+            const int x = (side ? CI : BI) * 32 + i;
+            int row = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int cx = CI * 32 + ty + 8 * q;
+            for (int r = 0; r < 6; ++r) row = q == r + 1 ? uniform(tk.vrow[r]) : row;
+            const double *src = q == 0 ? ev : vec6 + (int64_t)row * nvp;
+            if (x < nvp) vreg = gload2(src + x);  // nvp is a multiple of 16: x+1 < nvp too
+        }
+    }
+
+    // Direct values d[m][q] = f_m(bx, bx+1; cx), bx = BI*32 + 2tx, cx = CI*32 + ty + 16q;
+    // mirrored m[m][q] = f_m(cc, cc+1; bb), cc = CI*32 + 2tx, bb = BI*32 + ty + 16q,
+    // transposed below.  Byte offsets inside one matrix; the matrix is selected by
+    // the uniform offset m*vv*8.  Out-of-range points load from offset 0 and are
+    // masked; a pair whose second row is past nv (odd nv) loads one double.
+    const __amdgpu_buffer_rsrc_t rf = buffer_rsrc(ft, (uint32_t)(8 * vv * 8));
+    uint32_t od[2], om[2];
+    bool okd[2], okm[2];
+    const int r0 = 2 * tx;
+    const bool two_d = BI * 32 + r0 + 1 < nv, two_m = CI * 32 + r0 + 1 < nv;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int bx = BI * 32 + r0, cx = CI * 32 + ty + 16 * q;
         okd[q] = bx < nv && cx < nv;
         od[q] = okd[q] ? (uint32_t)(bx + cx * nv) * 8 : 0;
-        const int cc = CI * 32 + tx, bb = BI * 32 + ty + 8 * q;
+        const int cc = CI * 32 + r0, bb = BI * 32 + ty + 16 * q;
         okm[q] = cc < nv && bb < nv;
         om[q] = okm[q] ? (uint32_t)(cc + bb * nv) * 8 : 0;
     }
+    auto load_pair = [&](uint32_t off, uint32_t moff, bool two) -> dbl2 {
+        if (two) return buffer_load_f64x2(rf, off, moff);
+        return dbl2{buffer_load_f64(rf, off, moff), 0.0};
+    };
     // the mirrored tiles first: their transposes can start while the direct
     // tiles are still arriving
-    double dv[8][4], mv[8][4];
+    dbl2 dv[8][2], mv[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) mv[m][q] = buffer_load_f64(rf, om[q], (uint32_t)(m * vv * 8));
+        for (int q = 0; q < 2; ++q)
+            mv[m][q] = load_pair(om[q], (uint32_t)(m * vv * 8), okm[q] ? two_m : nv > 1);
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) dv[m][q] = buffer_load_f64(rf, od[q], (uint32_t)(m * vv * 8));
-#pragma unroll
-    for (int m = 0; m < 8; ++m)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            dv[m][q] = okd[q] ? dv[m][q] : 0.0;
-            mv[m][q] = okm[q] ? mv[m][q] : 0.0;
-        }
-    // transpose: thread (tx, ty) loaded f(c = CI*32+tx, b = BI*32+ty+8q) and
-    // needs f(c = CI*32+ty+8q, b = BI*32+tx)
+        for (int q = 0; q < 2; ++q)
+            dv[m][q] = load_pair(od[q], (uint32_t)(m * vv * 8), okd[q] ? two_d : nv > 1);
+    if (2 * (int)threadIdx.x < 7 * 64) *reinterpret_cast<dbl2 *>(vst + 2 * threadIdx.x) = vreg;
+    // transpose: thread (tx, ty) loaded f(c = CI*32 + 2tx + h, b = BI*32 + ty + 16q)
+    // and needs f(c = CI*32 + ty + 16q, b = BI*32 + 2tx + h)
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         double(*T)[33] = tile[m & 1];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) T[ty + 8 * q][tx] = mv[m][q];
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) T[ty + 16 * q][r0 + h] = okm[q] ? mv[m][q][h] : 0.0;
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q) mv[m][q] = T[tx][ty + 8 * q];
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) mv[m][q][h] = T[r0 + h][ty + 16 * q];
     }
+    auto vb = [&](int q, int i) { return vst[q * 64 + i]; };       // B tile, row i
+    auto vc = [&](int q, int i) { return vst[q * 64 + 32 + i]; };  // C tile, column i
     Sum4 s = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int cx = CI * 32 + ty + 8 * q;
-        if (bx < nv && cx < nv) {
-            const double d = -1.0 / (ev[bx] + ev[cx] + tk.eaijk);
-            // body order: 0 f1n, 1 f1t, 2 f2n, 3 f2t, 4 f3n, 5 f3t, 6 f4n, 7 f4t
-            tengy_point(dv[0][q], dv[2][q], dv[4][q], dv[6][q], dv[1][q], dv[3][q], dv[5][q], dv[7][q],
-                        mv[0][q], mv[2][q], mv[4][q], mv[6][q], mv[1][q], mv[3][q], mv[5][q], mv[7][q],
-                        d, vec6[vo[0] + bx], vec6[vo[1] + bx], vec6[vo[2] + cx], vec6[vo[3] + cx],
-                        vec6[vo[4] + cx], vec6[vo[5] + cx], s);
-            if (BI != CI)
-                tengy_point(mv[0][q], mv[2][q], mv[4][q], mv[6][q], mv[1][q], mv[3][q], mv[5][q], mv[7][q],
-                            dv[0][q], dv[2][q], dv[4][q], dv[6][q], dv[1][q], dv[3][q], dv[5][q], dv[7][q],
-                            d, vec6[vo[0] + cx], vec6[vo[1] + cx], vec6[vo[2] + bx], vec6[vo[3] + bx],
-                            vec6[vo[4] + bx], vec6[vo[5] + bx], s);
+    for (int q = 0; q < 2; ++q) {
+        const int lc = ty + 16 * q, cx = CI * 32 + lc;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int lb = r0 + h, bx = BI * 32 + lb;
+            if (bx < nv && cx < nv) {
+                const double d = -1.0 / (vb(0, lb) + vc(0, lc) + tk.eaijk);
+                // body order: 0 f1n, 1 f1t, 2 f2n, 3 f2t, 4 f3n, 5 f3t, 6 f4n, 7 f4t
+                tengy_point(dv[0][q][h], dv[2][q][h], dv[4][q][h], dv[6][q][h], dv[1][q][h],
+                            dv[3][q][h], dv[5][q][h], dv[7][q][h], mv[0][q][h], mv[2][q][h],
+                            mv[4][q][h], mv[6][q][h], mv[1][q][h], mv[3][q][h], mv[5][q][h],
+                            mv[7][q][h], d, vb(1, lb), vb(2, lb), vc(3, lc), vc(4, lc), vc(5, lc),
+                            vc(6, lc), s);
+                if (BI != CI)
+                    tengy_point(mv[0][q][h], mv[2][q][h], mv[4][q][h], mv[6][q][h], mv[1][q][h],
+                                mv[3][q][h], mv[5][q][h], mv[7][q][h], dv[0][q][h], dv[2][q][h],
+                                dv[4][q][h], dv[6][q][h], dv[1][q][h], dv[3][q][h], dv[5][q][h],
+                                dv[7][q][h], d, vc(1, lc), vc(2, lc), vb(3, lb), vb(4, lb),
+                                vb(5, lb), vb(6, lb), s);
+            }
         }
     }
     s.e4i = wave_sum(s.e4i);
