@@ -54,18 +54,19 @@ def load_pkg():
 
 def traffic_from_profile(form, opts):
     """HBM-side bytes per launch of the form's fused kernel from the committed
-    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated),
-    or None when that profile was taken with other run options."""
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated):
+    the entry taken with the same run options, or None when there is none."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_nocc30_nvir300.json" % form)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
     keys = ("order", "variant", "persistent", "slices")
-    prof = d.get("opts", {})
-    if any(prof.get(k) != opts.get(k) for k in keys):
-        return None
-    return d.get("bytes_per_launch")
+    for e in d.get("entries", []):
+        prof = e.get("opts", {})
+        if all(prof.get(k) == opts.get(k) for k in keys):
+            return e.get("bytes_per_launch")
+    return None
 
 
 def kernel_name(opts):

@@ -242,7 +242,8 @@ int ccsdt_job_destroy(ccsdt_job job);
  * and at half of it) --
  * with base's mode, slices and graph flag, and writes the fastest to *best
  * (candidates within 1% of the fastest count as equal; the first of them in the
- * order order x tile x schedule is taken, so repeated tunings agree).
+ * order order x tile x schedule is taken, so repeated tunings agree; pair-major
+ * order comes first).
  * Each candidate runs once untimed, then `reps` times; its time is the median
  * of ms_total.  table (may be NULL) receives up to cap entries, *count the
  * number of candidates tried.  Candidates the device rejects are recorded with

@@ -1118,7 +1118,12 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     const int slices = b0.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
                        : b0.mode == CCSDT_MODE_FP32 ? kSlicesFp32
                        : split ? (b0.slices ? b0.slices : 4) : 0;
-    for (int ord = 0; ord < (fused ? 2 : 1); ++ord) {
+    // pair-major first: candidates within kTuneTie of the fastest count as ties
+    // and the first in this order wins, so a tie goes to the order that keeps a
+    // block pair's panels shared across the triples in flight
+    const int orders[2] = {CCSDT_ORDER_PAIR, CCSDT_ORDER_TASK};
+    for (int oi = 0; oi < (fused ? 2 : 1); ++oi) {
+        const int ord = fused ? orders[oi] : CCSDT_ORDER_TASK;
         for (int v = 0; v < (split ? 1 : 2); ++v) {
             ccsdt_run_opts o = b0;
             o.order = ord;
