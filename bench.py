@@ -7,8 +7,10 @@ case: nocc=30, nvir=300, k blocks of 16, all 4960 triples i >= j >= k, with the
 job's arrays already resident in HBM.  Both forms are timed in the same run:
 
 * the plain FP64 form (FP64 MFMA GEMMs with the fused energy epilogue), and
-* the split form (the K=nvir contractions as Ozaki INT8 split products with S
-  slices, FP64 recombination, FP64 K=nocc part and energy epilogue).
+* the split form (the contractions as Ozaki INT8 split products with S slices
+  and FP64 recombination -- for S <= 3 both halves, K=nvir and K=nocc, each with
+  its own row scales; for larger S the K=nocc half in FP64 -- and the FP64
+  energy epilogue).
 
 Each form runs the configuration the library's autotuner (ccsdt_job_autotune)
 picks on the full list -- work-item order, tile variant, plain or persistent
@@ -465,8 +467,11 @@ def main():
     sp_kernel = kernel_name(sp["opts"])
     # roofline of the dominant kernel of the headline form
     if split_ok:
-        int8_ops = n * 2.0 * 8 * nvir ** 3 * S * (S + 1) / 2  # Ozaki products over K=nvir
-        fp64_part = n * 2.0 * 8 * nvir ** 2 * nocc           # K=nocc products in FP64
+        # Ozaki digit products over K=nvir, and over K=nocc when that half has
+        # digit planes too; otherwise the K=nocc products run in FP64
+        tail8 = S <= cc.TAIL8_MAX_SLICES
+        int8_ops = n * 2.0 * 8 * nvir ** 2 * (nvir + (nocc if tail8 else 0)) * S * (S + 1) / 2
+        fp64_part = 0.0 if tail8 else n * 2.0 * 8 * nvir ** 2 * nocc
         t = sp["kernel_ms"] * 1e-3
         ideal = int8_ops / (INT8_MFMA_PEAK * 1e12) + fp64_part / (FP64_MFMA_PEAK * 1e12)
         roof = {"bound": "mfma", "kernel": sp_kernel,

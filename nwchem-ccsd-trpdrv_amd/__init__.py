@@ -44,6 +44,9 @@ DRIVER_NAMES = ["t1", "dja", "djx"]
 DRIVER_STREAM = {"t1": STREAM["t1v1"], "dja": STREAM["dja"], "djx": STREAM["djia"]}
 
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3, "fp32": 4}
+# Ozaki INT8 slice counts whose K = nocc half also runs as INT8 digit planes (own
+# row scales); above it that half is an FP64 tail (kTail8MaxSlices in the library)
+TAIL8_MAX_SLICES = 3
 MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
 ORDER_TASK, ORDER_PAIR = 0, 1

@@ -274,10 +274,13 @@ SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = fa
     const int kstep = bf16 ? 32 : 64;  // K of one MFMA
     sg.kv = (int)round_up(g.nvir, kstep);
     sg.nkv = sg.kv / kstep;
+    const bool tail8 = !bf16 && sg.slices <= kTail8MaxSlices;
+    sg.kx = tail8 ? (int)round_up(g.nocc, 64) : 0;
+    sg.nkx = sg.kx / 64;
     sg.kt = (int)round_up(g.nocc, 8);
     sg.nkt = sg.kt / 4;
-    sg.slice_bytes = (int64_t)g.nvp * sg.kv * (bf16 ? 2 : 1);
-    sg.tail_sz = (int64_t)g.nvp * sg.kt;
+    sg.slice_bytes = (int64_t)g.nvp * (sg.kv + sg.kx) * (bf16 ? 2 : 1);
+    sg.tail_sz = tail8 ? 0 : (int64_t)g.nvp * sg.kt;
     sg.npanels = npanels;
     return sg;
 }
@@ -768,7 +771,8 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
-// Digit planes, scales and FP64 tail panels of the split form for sg.slices.
+// Digit planes, scales and FP64 tail panels (when the K = nocc half has no digit
+// planes) of the split form for sg.slices.
 int ensure_split(ccsdt_job j, const SplitGeom &sg)
 {
     const int key = sg.slices + (sg.bf16 ? kSlicesBf16x3 : 0);  // INT8 S or BF16x3
@@ -782,8 +786,12 @@ int ensure_split(ccsdt_job j, const SplitGeom &sg)
     j->split_slices = 0;
     int rc;
     if ((rc = dev_alloc((void **)&j->d_digits, (size_t)j->npanels * sg.slices * sg.slice_bytes))) return rc;
-    if ((rc = dev_alloc((void **)&j->d_scales, (size_t)j->npanels * j->g.nvp * sizeof(double)))) return rc;
-    if ((rc = dev_alloc((void **)&j->d_tail, (size_t)j->npanels * sg.tail_sz * sizeof(double)))) return rc;
+    if ((rc = dev_alloc((void **)&j->d_scales,
+                        (size_t)split_scale_sets(sg) * j->npanels * j->g.nvp * sizeof(double))))
+        return rc;
+    if (sg.tail_sz > 0 &&
+        (rc = dev_alloc((void **)&j->d_tail, (size_t)j->npanels * sg.tail_sz * sizeof(double))))
+        return rc;
     j->split_slices = key;
     return CCSDT_OK;
 }
@@ -1441,7 +1449,7 @@ int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, in
         const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
         const int npanels = (int)job_npanels(d->nocc, d->kchunk, kind);
         const SplitGeom sg = make_split_geom(g, o.slices, npanels, o.mode == CCSDT_MODE_BF16X3);
-        b += npanels * (sg.slices * sg.slice_bytes + sg.tail_sz * 8 + (int64_t)g.nvp * 8);
+        b += npanels * (sg.slices * sg.slice_bytes + sg.tail_sz * 8 + split_scale_sets(sg) * (int64_t)g.nvp * 8);
     }
     return b;
 }

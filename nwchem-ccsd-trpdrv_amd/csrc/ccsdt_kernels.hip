@@ -589,9 +589,12 @@ __device__ __forceinline__ double bf16_rn(double v, uint16_t &bits)
 // plane is 1 KiB.  Thread t of a step handles row r = (t/4)%16, K group
 // (t/4)/16 (one per wave) and a quarter of that lane's 16 bytes: its reads of
 // P are 128-byte runs whichever way P is transposed, and its 4-byte stores
-// are contiguous across the workgroup.  Pass 1 takes each row's maximum (lane
-// shuffles within a wave, then LDS across the 4 waves), pass 2 writes the
-// planes, then the FP64 tail (K = nocc) is written in the tail layout.
+// are contiguous across the workgroup.  Each part of the row is split in two
+// passes: pass 1 takes the row's maximum (lane shuffles within a wave, then LDS
+// across the 4 waves), pass 2 writes the planes.  The K = nvir part (P) fills
+// steps 0..nkv-1 of the block.  INT8 with S <= kTail8MaxSlices: the K = nocc
+// part (sgn * Q) fills steps nkv..nkv+nkx-1 with its own row scale (second
+// scale set); otherwise it is written as an FP64 tail panel.
 template <int S, bool BF>
 __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict__ descs, Geom g,
                                                    SplitGeom sg, int8_t *__restrict__ planes,
@@ -601,6 +604,7 @@ __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict_
     constexpr int KS = BF ? 32 : 64;     // K per MFMA step
     constexpr int EPT = BF ? 2 : 4;      // values per thread per step (4 bytes per plane)
     constexpr int NPL = BF ? 3 : S;      // planes
+    constexpr bool T8 = !BF && S <= kTail8MaxSlices;
     __shared__ double rmax[4][16];
     const int pnl = blockIdx.y, xb = blockIdx.x, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
@@ -608,56 +612,67 @@ __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict_
     const int x = xb * 16 + r;
     const PanelDesc d = descs[pnl];
     const int nv = g.nvir, no = g.nocc;
-    auto val = [&](int k) -> double {
-        if (x >= nv || k >= nv) return 0.0;
-        return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
-    };
-    // K index of value j of this thread in step ks
+    // K index of value j of this thread in step ks of a part
     auto kidx = [&](int ks, int j) { return ks * KS + w * (KS / 4) + c * EPT + j; };
-
-    double m = 0.0;
-    for (int ks = 0; ks < sg.nkv; ++ks)
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) m = fmax(m, fabs(val(kidx(ks, j))));
-    m = fmax(m, __shfl_xor(m, 1, 64));
-    m = fmax(m, __shfl_xor(m, 2, 64));
-    if (c == 0) rmax[w][r] = m;
-    __syncthreads();
-    m = fmax(fmax(rmax[0][r], rmax[1][r]), fmax(rmax[2][r], rmax[3][r]));
-    int e = 0;
-    // INT8: largest entry at most 127/128 after scaling; BF16: below 1
-    if (m > 0.0) (void)frexp(BF ? m : m * (128.0 / 127.0), &e);
-    const double inv = ldexp(1.0, -e);
-    if (w == 0 && c == 0) scales[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
-
     uint32_t *pl = reinterpret_cast<uint32_t *>(planes);
     const int64_t plane_words = sg.slice_bytes / 4;
-    for (int ks = 0; ks < sg.nkv; ++ks) {
-        uint32_t word[NPL];
+    const int nsb = sg.nkv + sg.nkx;  // K steps of one block
+
+    // one part of the row: val(k) for k in [0, nsteps * KS), steps step0.. of the block
+    auto split_part = [&](auto val, int nsteps, int step0, double *scale_set) {
+        double m = 0.0;
+        for (int ks = 0; ks < nsteps; ++ks)
 #pragma unroll
-        for (int q = 0; q < NPL; ++q) word[q] = 0;
+            for (int j = 0; j < EPT; ++j) m = fmax(m, fabs(val(kidx(ks, j))));
+        m = fmax(m, __shfl_xor(m, 1, 64));
+        m = fmax(m, __shfl_xor(m, 2, 64));
+        if (c == 0) rmax[w][r] = m;
+        __syncthreads();
+        m = fmax(fmax(rmax[0][r], rmax[1][r]), fmax(rmax[2][r], rmax[3][r]));
+        int e = 0;
+        // INT8: largest entry at most 127/128 after scaling; BF16: below 1
+        if (m > 0.0) (void)frexp(BF ? m : m * (128.0 / 127.0), &e);
+        const double inv = ldexp(1.0, -e);
+        if (w == 0 && c == 0) scale_set[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
+        for (int ks = 0; ks < nsteps; ++ks) {
+            uint32_t word[NPL];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            double v = val(kidx(ks, j)) * inv;
+            for (int q = 0; q < NPL; ++q) word[q] = 0;
 #pragma unroll
-            for (int q = 0; q < NPL; ++q) {
-                if (BF) {
-                    uint16_t b;
-                    v -= bf16_rn(v, b);
-                    word[q] |= (uint32_t)b << (16 * j);
-                } else {
-                    v *= 128.0;
-                    const double dg = rint(v);
-                    v -= dg;
-                    word[q] |= (uint32_t)(uint8_t)(int8_t)dg << (8 * j);
+            for (int j = 0; j < EPT; ++j) {
+                double v = val(kidx(ks, j)) * inv;
+#pragma unroll
+                for (int q = 0; q < NPL; ++q) {
+                    if (BF) {
+                        uint16_t b;
+                        v -= bf16_rn(v, b);
+                        word[q] |= (uint32_t)b << (16 * j);
+                    } else {
+                        v *= 128.0;
+                        const double dg = rint(v);
+                        v -= dg;
+                        word[q] |= (uint32_t)(uint8_t)(int8_t)dg << (8 * j);
+                    }
                 }
             }
-        }
-        // byte offset of this thread's 4 bytes: step base + lane-group (w*16 + r)
-        // * 16 + c * 4
-        const int64_t o = ((int64_t)(xb * sg.nkv + ks) * 1024 + (w * 16 + r) * 16 + c * 4) / 4;
+            // byte offset of this thread's 4 bytes: step base + lane-group (w*16 + r)
+            // * 16 + c * 4
+            const int64_t o = ((int64_t)(xb * nsb + step0 + ks) * 1024 + (w * 16 + r) * 16 + c * 4) / 4;
 #pragma unroll
-        for (int q = 0; q < NPL; ++q) pl[((int64_t)pnl * NPL + q) * plane_words + o] = word[q];
+            for (int q = 0; q < NPL; ++q) pl[((int64_t)pnl * NPL + q) * plane_words + o] = word[q];
+        }
+        __syncthreads();  // rmax is reused by the next part
+    };
+    split_part([&](int k) -> double {
+        if (x >= nv || k >= nv) return 0.0;
+        return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
+    }, sg.nkv, 0, scales);
+    if (T8) {
+        split_part([&](int k) -> double {
+            if (x >= nv || k >= no) return 0.0;
+            return d.sgn * (d.second_t ? d.Q[k + (int64_t)x * no] : d.Q[x + (int64_t)k * nv]);
+        }, sg.nkx, sg.nkv, scales + (int64_t)sg.npanels * g.nvp);
+        return;
     }
     // FP64 tail in the step-pair layout (nkt is even): element 128*s + 2*l + h
     // holds row l%16, K index 4*(2s+h) + l/16
@@ -744,9 +759,13 @@ struct PlaneMath<true> {
 
 // stash: this wave's 8 KiB of LDS, holding group 0's four products while group
 // 1's digit products run, so their 32 VGPRs are free during the INT8 loops.
-// sst: this wave's 1 KiB of LDS for the row scales of its eight panels (p = A0,
+// sst: this wave's 2 KiB of LDS for the row scales of its eight panels (p = A0,
 // A1 of group 0, A0, A1 of group 1, then B0, B1 of each group; 16 rows each of
-// block bi for A, ci for B), fetched with one 16-byte load per lane.
+// block bi for A, ci for B), fetched with one 16-byte load per lane: the K =
+// nvir planes' scales, then (T8) the K = nocc planes' scales.
+// T8 (INT8, S <= kTail8MaxSlices): each group's K = nvir steps are recombined
+// first, then its K = nocc steps (own scales) are added in FP64.  Otherwise the
+// K = nocc half runs as an FP64 tail on v_mfma_f64_16x16x4_f64 after both groups.
 template <int S, int W, bool BF = false>
 __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
                                            const __amdgpu_buffer_rsrc_t rt,
@@ -759,7 +778,8 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
     const int LJ = 2 * pp.li, LK = LJ + 1, LJT = 2 * pp.lk, LKT = LJT + 1;
     const int R1 = 2 * pp.rkj, R3 = R1 + 1, R1T = 2 * pp.rij, R3T = R1T + 1;
     const int nvp = g.nvp;
-    dbl2 sreg;
+    constexpr bool T8 = !BF && S <= kTail8MaxSlices;
+    dbl2 sreg, sreg_t = {0.0, 0.0};
     {
         const int sp = lane >> 3;
         const int L4[4] = {LJ, LK, LJT, LKT}, R4[4] = {R1, R3, R1T, R3T};
@@ -769,10 +789,12 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             pid = sp == q ? (W == 0 ? L4[q] : R4[q]) : pid;
             pid = sp == 4 + q ? (W == 0 ? R4[q] : L4[q]) : pid;
         }
-        sreg = gload2(scales + (int64_t)pid * nvp + (sp < 4 ? bi : ci) * 16 + (lane & 7) * 2);
+        const int64_t so = (int64_t)pid * nvp + (sp < 4 ? bi : ci) * 16 + (lane & 7) * 2;
+        sreg = gload2(scales + so);
+        if (T8) sreg_t = gload2(scales + (int64_t)sg.npanels * nvp + so);
     }
     const uint32_t plane = (uint32_t)sg.slice_bytes;
-    const uint32_t blk8 = (uint32_t)sg.nkv * 1024;  // bytes of a 16-row block of one plane
+    const uint32_t blk8 = (uint32_t)(sg.nkv + sg.nkx) * 1024;  // bytes of a 16-row block of one plane
     const uint32_t voff8 = lane * 16;
     const int row0 = 4 * (lane >> 4), cx = ci * 16 + (lane & 15);
 #pragma unroll
@@ -838,34 +860,58 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
                 SPLIT_MFMA(xa0, xa1, xb0, xb1)
             }
         }
-#undef SPLIT_LOAD
-#undef SPLIT_MFMA
         // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * w(L) (INT8 w(L) =
         // 2^(-7(L+2)), BF16 w = 1); the combination (a, b) is product
-        // 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1)
-        if (gi == 0) *reinterpret_cast<dbl2 *>(sst + 2 * lane) = sreg;
+        // 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1).  With T8 a second pass
+        // adds the K = nocc steps with their own scales (sst + 128).
+        auto recombine = [&](const double *ss, bool add) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int ca = c & 1, cb = c >> 1;
-            const int lp = 4 * gi + (W == 0 ? ca + 2 * cb : 2 * ca + cb);
-            const double sB = sst[(4 + 2 * gi + cb) * 16 + (lane & 15)];
+            for (int c = 0; c < 4; ++c) {
+                const int ca = c & 1, cb = c >> 1;
+                const int lp = 4 * gi + (W == 0 ? ca + 2 * cb : 2 * ca + cb);
+                const double sB = ss[(4 + 2 * gi + cb) * 16 + (lane & 15)];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                double v = 0.0;
+                for (int r = 0; r < 4; ++r) {
+                    double v = 0.0;
 #pragma unroll
-                for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * PlaneMath<BF>::weight(q);
-                const double sA = sst[(2 * gi + ca) * 16 + row0 + r];
-                const double f = v * sA * sB;
-                if (gi == 0)
-                    stash[(lp * 4 + r) * 64 + lane] = f;  // lp = 0..3
-                else
-                    facc[lp][r] = f;
+                    for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * PlaneMath<BF>::weight(q);
+                    const double sA = ss[(2 * gi + ca) * 16 + row0 + r];
+                    const double f = v * sA * sB;
+                    if (gi == 0) {
+                        double &st = stash[(lp * 4 + r) * 64 + lane];  // lp = 0..3
+                        st = add ? st + f : f;
+                    } else {
+                        facc[lp][r] = add ? facc[lp][r] + f : f;
+                    }
+                }
             }
-        }
+        };
+        if (gi == 0) *reinterpret_cast<dbl2 *>(sst + 2 * lane) = sreg;
+        recombine(sst, false);
         // a compiler fence (stash is not restrict-qualified, so it covers the
         // stash): without it the stores are forwarded to the reloads below and
         // group 0's values stay in registers through group 1 after all
         if (gi == 0) asm volatile("" ::: "memory");
+        if constexpr (T8) {
+            if (gi == 0) *reinterpret_cast<dbl2 *>(sst + 128 + 2 * lane) = sreg_t;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int q = 0; q < S; ++q) iacc[c][q] = acc_t{0, 0, 0, 0};
+            // (the scheduling barrier keeps these loads from being hoisted above the
+            // recombination, where their registers would add to its own)
+            __builtin_amdgcn_sched_barrier(0);
+            for (int kx = 0; kx < sg.nkx; ++kx) {
+                SPLIT_LOAD(xa0, xa1, xb0, xb1, nkv + kx)
+                SPLIT_MFMA(xa0, xa1, xb0, xb1)
+            }
+            recombine(sst + 128, true);
+            if (gi == 0) asm volatile("" ::: "memory");
+            // group 1's first loads stay below this group's last recombination
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#undef SPLIT_LOAD
+#undef SPLIT_MFMA
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -873,45 +919,47 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
         for (int r = 0; r < 4; ++r) facc[p][r] = stash[(p * 4 + r) * 64 + lane];
 
     // FP64 tail (K = nocc) of the eight products
-    const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
-    const uint32_t blkt = (uint32_t)sg.nkt * 512;
-    const int li = lane & 15;
-    // step-pair layout: each lane's two doubles of a step pair are adjacent
-    const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 16;
-    const uint32_t voffB = lane * 16;
-    // W=0: A {LJ, LK, LJt, LKt} at b, B {R1, R3, R1t, R3t} at c; W=1 roles swapped
-    const int PL[4] = {LJ, LK, LJT, LKT};
-    const int PR[4] = {R1, R3, R1T, R3T};
-    uint32_t ao[4], bo[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        ao[v] = (W == 0 ? PL[v] : PR[v]) * tsb + bi * blkt;
-        bo[v] = (W == 0 ? PR[v] : PL[v]) * tsb + ci * blkt;
-    }
-    for (int sp = 0; sp < sg.nkt / 2; ++sp) {
-        dbl2 a2[4], b2[4];
+    if constexpr (!T8) {
+        const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
+        const uint32_t blkt = (uint32_t)sg.nkt * 512;
+        const int li = lane & 15;
+        // step-pair layout: each lane's two doubles of a step pair are adjacent
+        const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 16;
+        const uint32_t voffB = lane * 16;
+        // W=0: A {LJ, LK, LJt, LKt} at b, B {R1, R3, R1t, R3t} at c; W=1 roles swapped
+        const int PL[4] = {LJ, LK, LJT, LKT};
+        const int PR[4] = {R1, R3, R1T, R3T};
+        uint32_t ao[4], bo[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            a2[v] = buffer_load_f64x2(rt, voffA, ao[v] + sp * 1024);
-            b2[v] = buffer_load_f64x2(rt, voffB, bo[v] + sp * 1024);
+            ao[v] = (W == 0 ? PL[v] : PR[v]) * tsb + bi * blkt;
+            bo[v] = (W == 0 ? PR[v] : PL[v]) * tsb + ci * blkt;
         }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            double a[4], b[4];
+        for (int sp = 0; sp < sg.nkt / 2; ++sp) {
+            dbl2 a2[4], b2[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                a[v] = a2[v][h];
-                b[v] = b2[v][h];
+                a2[v] = buffer_load_f64x2(rt, voffA, ao[v] + sp * 1024);
+                b2[v] = buffer_load_f64x2(rt, voffB, bo[v] + sp * 1024);
             }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                double a[4], b[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    a[v] = a2[v][h];
+                    b[v] = b2[v][h];
+                }
 #define MS(pp, ia, ib) facc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ia], b[ib], facc[pp], 0, 0, 0)
-            if (W == 0) {
-                MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);
-                MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);
-            } else {
-                MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);
-                MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);
-            }
+                if (W == 0) {
+                    MS(0, 0, 0); MS(1, 1, 0); MS(2, 0, 1); MS(3, 1, 1);
+                    MS(4, 2, 2); MS(5, 3, 2); MS(6, 2, 3); MS(7, 3, 3);
+                } else {
+                    MS(0, 0, 0); MS(1, 0, 1); MS(2, 1, 0); MS(3, 1, 1);
+                    MS(4, 2, 2); MS(5, 2, 3); MS(6, 3, 2); MS(7, 3, 3);
+                }
 #undef MS
+            }
         }
     }
 }
@@ -1221,7 +1269,7 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
     const bool diag = (bi == ci);
     __shared__ double lds_x[2 * 8 * 2 * 64];
     __shared__ __attribute__((aligned(16))) double lds_v[7 * 2 * 16];   // energy vectors
-    __shared__ __attribute__((aligned(16))) double lds_s[2][8 * 16];    // split: row scales
+    __shared__ __attribute__((aligned(16))) double lds_s[2][2 * 8 * 16];  // split: row scales
     const dbl2 vreg = TENGY ? stage_vectors(jd, tk, bi, ci, w, lane) : dbl2{0.0, 0.0};
     if constexpr (S == kSlicesFp32) {
         flt4 f32[8];
@@ -1293,7 +1341,7 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
 
 // The split form's work item (one 16x16 block pair of one task) on two waves.
 template <int S, bool TENGY, bool WRITEF, bool PERSIST>
-__global__ __launch_bounds__(128) void k_fused_split(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(S <= kTail8MaxSlices ? 3 : 1))) void k_fused_split(const JobDev *__restrict__ jobs,
                                                      const Segment *__restrict__ segs,
                                                      WorkHdr *hdr, int64_t fstride)
 {
