@@ -63,10 +63,10 @@ def traffic_from_profile(form, opts):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    keys = ("order", "variant", "persistent", "slices")
+    keys = ("order", "variant", "persistent", "slices", "occ_digits")
     for e in d.get("entries", []):
         prof = e.get("opts", {})
-        if all(prof.get(k) == opts.get(k) for k in keys):
+        if all(prof.get(k, 0) == opts.get(k, 0) for k in keys):
             return e.get("bytes_per_launch")
     return None
 
@@ -75,7 +75,8 @@ def kernel_name(opts):
     """Name of the fused kernel a configuration launches."""
     p = "true" if opts.get("persistent") else "false"
     if opts["mode"] == "ozaki_int8":
-        return "k_fused_split<%d,true,false,%s>" % (opts["slices"], p)
+        # S = 3 with occupied digit planes is template code 3 + 16 (kSlicesOccDigits)
+        return "k_fused_split<%d,true,false,%s>" % (opts["slices"] + 16 * bool(opts.get("occ_digits")), p)
     return ("k_fused" if opts.get("variant", 0) == 0 else "k_fused2") + "<true,false,%s>" % p
 
 
@@ -293,7 +294,8 @@ def tensor_job_report(cc, steps, warmup, slices):
 
 
 def split_sweep_report(cc, steps, warmup):
-    """The split-precision workload at nocc=30, nvir=300: INT8 slice counts 3..7,
+    """The split-precision workload at nocc=30, nvir=300: INT8 slice counts 3..7
+    (and 3 with the K=nocc half as digit planes too),
     the BF16x3 form and, as a precision comparison point, the FP32 form, on the
     well-conditioned inputs and on the ill-conditioned
     variant (J/K entries scaled by 10^u, u ~ U(-6,0)); per case the FP64-eq
@@ -310,14 +312,17 @@ def split_sweep_report(cc, steps, warmup):
             E = f["res"]["emp4"] + f["res"]["emp5"]
             case = {"inputs": "ill-conditioned" if ill else "well-conditioned",
                     "fp64": {"value": flops / (f["ms_per_step"] * 1e-3) / 1e12, "E": E}, "split": []}
-            for S in range(3, 8):
-                r = time_form(job, trip, steps, warmup, mode="ozaki_int8", slices=S, order=1, graph=True)
+            levels = [(3, 1)] + [(S, 0) for S in range(3, 8)]  # (slices, occ_digits)
+            for S, occ in levels:
+                r = time_form(job, trip, steps, warmup, mode="ozaki_int8", slices=S, order=1,
+                              graph=True, occ_digits=occ)
                 e = r["res"]["emp4"] + r["res"]["emp5"]
-                case["split"].append({"slices": S, "value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
+                case["split"].append({"slices": S, "occ_digits": occ,
+                                      "value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
                                       "ms_per_step": r["ms_per_step"], "rel_dE": abs(e - E) / abs(E)})
-                print("bench: split sweep %s S=%d %.1f TFLOP/s rel dE %.2e" % (
-                    case["inputs"], S, case["split"][-1]["value"], case["split"][-1]["rel_dE"]),
-                    file=sys.stderr, flush=True)
+                print("bench: split sweep %s S=%d%s %.1f TFLOP/s rel dE %.2e" % (
+                    case["inputs"], S, " occ_digits" if occ else "", case["split"][-1]["value"],
+                    case["split"][-1]["rel_dE"]), file=sys.stderr, flush=True)
             for mode in ("bf16x3", "fp32"):
                 r = time_form(job, trip, steps, warmup, mode=mode, order=1, graph=True)
                 e = r["res"]["emp4"] + r["res"]["emp5"]
@@ -469,7 +474,7 @@ def main():
     if split_ok:
         # Ozaki digit products over K=nvir, and over K=nocc when that half has
         # digit planes too; otherwise the K=nocc products run in FP64
-        tail8 = S <= cc.TAIL8_MAX_SLICES
+        tail8 = bool(sp["opts"].get("occ_digits"))
         int8_ops = n * 2.0 * 8 * nvir ** 2 * (nvir + (nocc if tail8 else 0)) * S * (S + 1) / 2
         fp64_part = 0.0 if tail8 else n * 2.0 * 8 * nvir ** 2 * nocc
         t = sp["kernel_ms"] * 1e-3

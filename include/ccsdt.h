@@ -211,7 +211,12 @@ typedef struct {
     int variant;    /* CCSDT_TILE_* (FP64 modes) */
     int wg_per_cu;  /* persistent grid: workgroups per CU (0 = as many as fit) */
     int order;      /* CCSDT_ORDER_* (fused modes; the separate-pass mode runs task order) */
-    int reserved[2];
+    int occ_digits; /* CCSDT_MODE_OZAKI_INT8 with slices 3: 1 runs the K=nocc half as INT8
+                       digit planes too, with row scales of its own (faster; adds that
+                       half's split error), 0 as an FP64 tail.  Larger slice counts reject
+                       1 (their kernels have no registers left for it); other modes
+                       ignore it */
+    int reserved;
 } ccsdt_run_opts;
 
 typedef struct {
@@ -248,9 +253,10 @@ int ccsdt_job_destroy(ccsdt_job job);
  * of ms_total.  table (may be NULL) receives up to cap entries, *count the
  * number of candidates tried.  Candidates the device rejects are recorded with
  * their status and skipped.  With mode CCSDT_MODE_OZAKI_INT8 and slices 0 the
- * slice count is chosen first: the smallest S in 3..7 whose E[T] = emp4 + emp5
- * over these triples is within 1e-9 relative of the FP64 form's (ten times
- * under the split form's 1e-8 bound); best->slices reports it. */
+ * precision is chosen first: the first of (S = 3 with occ_digits), S = 3, 4, .., 7
+ * (cheapest first) whose E[T] = emp4 + emp5 over these triples is within 1e-9
+ * relative of the FP64 form's (ten times under the split form's 1e-8 bound);
+ * best->slices and best->occ_digits report it. */
 typedef struct {
     ccsdt_run_opts opts;
     double ms;      /* median ms_total, or -1 if the candidate failed */

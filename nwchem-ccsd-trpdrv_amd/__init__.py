@@ -44,8 +44,9 @@ DRIVER_NAMES = ["t1", "dja", "djx"]
 DRIVER_STREAM = {"t1": STREAM["t1v1"], "dja": STREAM["dja"], "djx": STREAM["djia"]}
 
 MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3, "fp32": 4}
-# Ozaki INT8 slice counts whose K = nocc half also runs as INT8 digit planes (own
-# row scales); above it that half is an FP64 tail (kTail8MaxSlices in the library)
+# Largest Ozaki INT8 slice count whose K = nocc half may also run as INT8 digit
+# planes with row scales of its own (run option occ_digits; kTail8MaxSlices in the
+# library); otherwise that half is an FP64 tail
 TAIL8_MAX_SLICES = 3
 MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
@@ -97,13 +98,13 @@ def _tensor_desc(t, nocc, nvir, ncor):
 class RunOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slices", ctypes.c_int), ("flags", ctypes.c_int),
                 ("variant", ctypes.c_int), ("wg_per_cu", ctypes.c_int), ("order", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2)]
+                ("occ_digits", ctypes.c_int), ("reserved", ctypes.c_int)]
 
     def as_dict(self):
         return {"mode": MODE_NAMES[self.mode], "slices": self.slices,
                 "persistent": bool(self.flags & RUN_PERSISTENT),
                 "graph": bool(self.flags & RUN_GRAPH), "variant": self.variant,
-                "wg_per_cu": self.wg_per_cu, "order": self.order}
+                "wg_per_cu": self.wg_per_cu, "order": self.order, "occ_digits": self.occ_digits}
 
 
 class TuneEntry(ctypes.Structure):
@@ -484,15 +485,16 @@ class Job:
         return self
 
     def run(self, triples, mode="fp64", slices=0, terms=False, fout=False, persistent=False,
-            graph=False, variant=0, wg_per_cu=0, order=0):
+            graph=False, variant=0, wg_per_cu=0, order=0, occ_digits=0):
         """Runs the triples; persistent / graph select CCSDT_RUN_PERSISTENT and
         CCSDT_RUN_GRAPH, variant the FP64 tile (CCSDT_TILE_*), wg_per_cu the
         persistent grid (0 = as many workgroups per CU as fit), order the work
-        item order (ORDER_TASK or ORDER_PAIR)."""
+        item order (ORDER_TASK or ORDER_PAIR), occ_digits (ozaki_int8, slices 3)
+        the K = nocc half as INT8 digit planes instead of an FP64 tail."""
         trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
         n = trip.shape[0]
         flags = (RUN_PERSISTENT if persistent else 0) | (RUN_GRAPH if graph else 0)
-        opts = RunOpts(MODES[mode], slices, flags, variant, wg_per_cu, order)
+        opts = RunOpts(MODES[mode], slices, flags, variant, wg_per_cu, order, int(occ_digits))
         res = RunResult()
         t = np.empty((n, 4)) if terms else None
         f = np.empty((8, self.nvir, self.nvir)) if fout else None
@@ -507,13 +509,14 @@ class Job:
             out["f"] = np.ascontiguousarray(np.swapaxes(f, 1, 2))
         return out
 
-    def autotune(self, triples, mode="fp64", slices=0, graph=True, reps=3):
+    def autotune(self, triples, mode="fp64", slices=0, graph=True, reps=3, occ_digits=0):
         """Times every launch configuration (tile variant x schedule) on these
-        triples (ccsdt_job_autotune).  Returns (best, table): best is a dict of
-        run() keyword arguments, table one dict per candidate with its median
+        triples (ccsdt_job_autotune); with ozaki_int8 and slices 0 it first picks
+        the precision (slices, occ_digits).  Returns (best, table): best is a dict
+        of run() keyword arguments, table one dict per candidate with its median
         ms (-1 when it failed) and status."""
         trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
-        base = RunOpts(MODES[mode], slices, RUN_GRAPH if graph else 0, 0, 0)
+        base = RunOpts(MODES[mode], slices, RUN_GRAPH if graph else 0, 0, 0, 0, int(occ_digits))
         best = RunOpts()
         cap = 16
         table = (TuneEntry * cap)()

@@ -266,7 +266,8 @@ Geom make_geom(int ncor, int nocc, int nvir)
     return g;
 }
 
-SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = false)
+SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = false,
+                          bool occ_digits = false)
 {
     SplitGeom sg;
     sg.slices = bf16 ? 3 : slices;
@@ -274,7 +275,7 @@ SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = fa
     const int kstep = bf16 ? 32 : 64;  // K of one MFMA
     sg.kv = (int)round_up(g.nvir, kstep);
     sg.nkv = sg.kv / kstep;
-    const bool tail8 = !bf16 && sg.slices <= kTail8MaxSlices;
+    const bool tail8 = !bf16 && occ_digits && sg.slices <= kTail8MaxSlices;
     sg.kx = tail8 ? (int)round_up(g.nocc, 64) : 0;
     sg.nkx = sg.kx / 64;
     sg.kt = (int)round_up(g.nocc, 8);
@@ -771,11 +772,18 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// `slices` argument of the split launchers: the INT8 slice count (plus
+// kSlicesOccDigits with occupied digit planes), or the BF16x3 code.
+int slice_code(const SplitGeom &sg)
+{
+    return sg.bf16 ? kSlicesBf16x3 : sg.slices + (sg.nkx > 0 ? kSlicesOccDigits : 0);
+}
+
 // Digit planes, scales and FP64 tail panels (when the K = nocc half has no digit
 // planes) of the split form for sg.slices.
 int ensure_split(ccsdt_job j, const SplitGeom &sg)
 {
-    const int key = sg.slices + (sg.bf16 ? kSlicesBf16x3 : 0);  // INT8 S or BF16x3
+    const int key = slice_code(sg);  // INT8 S (+ occupied digit planes) or BF16x3
     if (j->split_slices == key) return CCSDT_OK;
     if (j->d_digits) dev_free(j->d_digits);
     if (j->d_scales) dev_free(j->d_scales);
@@ -799,8 +807,6 @@ int ensure_split(ccsdt_job j, const SplitGeom &sg)
 // The two split forms: Ozaki INT8 digits and BF16x3 planes.
 bool is_split_mode(int mode) { return mode == CCSDT_MODE_OZAKI_INT8 || mode == CCSDT_MODE_BF16X3; }
 
-// `slices` argument of the split launchers: the INT8 slice count, or the BF16x3 code.
-int slice_code(const SplitGeom &sg) { return sg.bf16 ? kSlicesBf16x3 : sg.slices; }
 
 // Modes that run a two-wave block-pair kernel through launch_fused_split.
 bool is_pair_mode(int mode) { return is_split_mode(mode) || mode == CCSDT_MODE_FP32; }
@@ -913,6 +919,11 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const int slices = (mode == CCSDT_MODE_OZAKI_INT8) ? (opts && opts->slices ? opts->slices : 4) : 0;
     if (mode == CCSDT_MODE_OZAKI_INT8 && (slices < 3 || slices > 7))
         return set_error(CCSDT_E_ARG, "Ozaki slice count %d outside 3..7", slices);
+    const int occ = (mode == CCSDT_MODE_OZAKI_INT8 && opts) ? opts->occ_digits : 0;
+    if (occ != 0 && occ != 1) return set_error(CCSDT_E_ARG, "occ_digits %d is neither 0 nor 1", occ);
+    if (occ && slices > kTail8MaxSlices)
+        return set_error(CCSDT_E_ARG, "occupied digit planes need slices <= %d (got %d)",
+                         kTail8MaxSlices, slices);
     const int variant = (is_pair_mode(mode) || !opts) ? 0 : opts->variant;
     if (variant != CCSDT_TILE_ONE_WAVE && variant != CCSDT_TILE_TWO_WAVE)
         return set_error(CCSDT_E_ARG, "unknown tile variant %d", variant);
@@ -934,7 +945,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const int64_t npart = (int64_t)n * (fused ? g.npairs : tnp) * 4;
     SplitGeom sg{};
     if (is_split_mode(mode)) {
-        sg = make_split_geom(g, slices, j->npanels, mode == CCSDT_MODE_BF16X3);
+        sg = make_split_geom(g, slices, j->npanels, mode == CCSDT_MODE_BF16X3, occ != 0);
         const int64_t dig = (int64_t)j->npanels * sg.slices * sg.slice_bytes;
         const int64_t tb = (int64_t)j->npanels * sg.tail_sz * 8;
         if (dig >= (int64_t(1) << 32) || tb >= (int64_t(1) << 32))
@@ -973,7 +984,8 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             }
             // the signature, and every buffer the captured nodes reference (a run
             // of another size may have reallocated them)
-            const int64_t key[kGraphKey] = {mode, slices, n, persistent, terms != nullptr, variant,
+            const int64_t key[kGraphKey] = {mode, slices + occ * kSlicesOccDigits, n, persistent,
+                                     terms != nullptr, variant,
                                      persistent ? wg_per_cu : 0, order,
                                      (int64_t)(intptr_t)j->d_order, (int64_t)(intptr_t)j->h_order,
                                      (int64_t)(intptr_t)j->d_scales,
@@ -1103,8 +1115,8 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     const bool fused = (b0.mode != CCSDT_MODE_FP64_SPLIT);
     std::vector<ccsdt_run_opts> cand;
     if (b0.mode == CCSDT_MODE_OZAKI_INT8 && b0.slices == 0) {
-        // slice count: the smallest S whose E[T] on this list is within
-        // kSliceTol of the FP64 form's
+        // precision: the cheapest of (S = 3 with occupied digit planes), S = 3,
+        // 4, .., 7 whose E[T] on this list is within kSliceTol of the FP64 form's
         ccsdt_run_opts f = b0;
         f.mode = CCSDT_MODE_FP64;
         f.flags = 0;
@@ -1113,19 +1125,29 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
         if ((rc = job_run(j, trip, n, &f, &rf, nullptr, nullptr))) return rc;
         const double ef = rf.emp4 + rf.emp5;
         for (int S = 3; S <= 7 && b0.slices == 0; ++S) {
-            ccsdt_run_opts o = b0;
-            o.slices = S;
-            o.flags = 0;
-            ccsdt_run_result r{};
-            if ((rc = job_run(j, trip, n, &o, &r, nullptr, nullptr))) return rc;
-            const double dev = std::fabs(r.emp4 + r.emp5 - ef) / std::fabs(ef);
-            if (dev <= kSliceTol) b0.slices = S;
+            for (int occ = (S <= kTail8MaxSlices ? 1 : 0); occ >= 0 && b0.slices == 0; --occ) {
+                ccsdt_run_opts o = b0;
+                o.slices = S;
+                o.occ_digits = occ;
+                o.flags = 0;
+                ccsdt_run_result r{};
+                if ((rc = job_run(j, trip, n, &o, &r, nullptr, nullptr))) return rc;
+                const double dev = std::fabs(r.emp4 + r.emp5 - ef) / std::fabs(ef);
+                if (dev <= kSliceTol) {
+                    b0.slices = S;
+                    b0.occ_digits = occ;
+                }
+            }
         }
-        if (b0.slices == 0) b0.slices = 7;
+        if (b0.slices == 0) {
+            b0.slices = 7;
+            b0.occ_digits = 0;
+        }
     }
     const int slices = b0.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
                        : b0.mode == CCSDT_MODE_FP32 ? kSlicesFp32
-                       : split ? (b0.slices ? b0.slices : 4) : 0;
+                       : split ? (b0.slices ? b0.slices : 4) + (b0.occ_digits ? kSlicesOccDigits : 0)
+                               : 0;
     // pair-major first: candidates within kTuneTie of the fastest count as ties
     // and the first in this order wins, so a tie goes to the order that keeps a
     // block pair's panels shared across the triples in flight

@@ -68,19 +68,23 @@ struct SplitGeom {
     int slices;
     int bf16;             // 1: three BF16 planes (v_mfma_f32_16x16x32_bf16), not INT8 digits
     int kv, nkv;          // kv = round_up(nvir, 64), nkv = kv / 64 (BF16: 32)
-    int kx, nkx;          // INT8 with S <= kTail8MaxSlices: the K = nocc half as digit
-                          // planes of its own, kx = round_up(nocc, 64), nkx = kx / 64; else 0
+    int kx, nkx;          // INT8 with occupied digit planes (S <= kTail8MaxSlices): the K =
+                          // nocc half as planes of its own, kx = round_up(nocc, 64),
+                          // nkx = kx / 64; else 0
     int kt, nkt;          // FP64 K = nocc tail (when nkx = 0): kt = round_up(nocc, 8), nkt = kt / 4 (even)
     int64_t slice_bytes;  // bytes of one digit plane of one panel: nvp * (kv + kx) (BF16: 2 nvp kv)
     int64_t tail_sz;      // doubles of one FP64 tail panel: nvp * kt (0 when nkx > 0)
     int npanels;          // panels of the job (bounds of the buffer resources)
 };
 
-// INT8 slice counts whose K = nocc half runs as digit planes (own row scales)
-// rather than as an FP64 tail.  At S = 3 the digit step costs less than the FP64
-// MFMA tail; at S >= 4 its accumulators and operands would cost the kernel a
-// wave per SIMD (more than 168 VGPRs).
+// INT8 slice counts whose K = nocc half may run as digit planes (own row scales,
+// ccsdt_run_opts::occ_digits) rather than as an FP64 tail.  At S = 3 the digit
+// step costs less than the FP64 MFMA tail; at S >= 4 its accumulators and
+// operands would cost the kernel a wave per SIMD (more than 168 VGPRs).
 constexpr int kTail8MaxSlices = 3;
+// Added to an INT8 slice count in the launchers' `slices` code: the K = nocc half
+// as digit planes (S <= kTail8MaxSlices only).
+constexpr int kSlicesOccDigits = 16;
 
 // Row-scale sets of a split job: the K = nvir planes' scales, then (nkx > 0) the
 // K = nocc planes' scales, each npanels * nvp doubles.

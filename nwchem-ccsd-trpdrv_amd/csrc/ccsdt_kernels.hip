@@ -592,10 +592,10 @@ __device__ __forceinline__ double bf16_rn(double v, uint16_t &bits)
 // are contiguous across the workgroup.  Each part of the row is split in two
 // passes: pass 1 takes the row's maximum (lane shuffles within a wave, then LDS
 // across the 4 waves), pass 2 writes the planes.  The K = nvir part (P) fills
-// steps 0..nkv-1 of the block.  INT8 with S <= kTail8MaxSlices: the K = nocc
-// part (sgn * Q) fills steps nkv..nkv+nkx-1 with its own row scale (second
+// steps 0..nkv-1 of the block.  T8 (INT8 with occupied digit planes): the K =
+// nocc part (sgn * Q) fills steps nkv..nkv+nkx-1 with its own row scale (second
 // scale set); otherwise it is written as an FP64 tail panel.
-template <int S, bool BF>
+template <int S, bool BF, bool T8 = false>
 __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict__ descs, Geom g,
                                                    SplitGeom sg, int8_t *__restrict__ planes,
                                                    double *__restrict__ scales,
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict_
     constexpr int KS = BF ? 32 : 64;     // K per MFMA step
     constexpr int EPT = BF ? 2 : 4;      // values per thread per step (4 bytes per plane)
     constexpr int NPL = BF ? 3 : S;      // planes
-    constexpr bool T8 = !BF && S <= kTail8MaxSlices;
+    static_assert(!T8 || (!BF && S <= kTail8MaxSlices), "occupied digit planes: INT8, S <= 3");
     __shared__ double rmax[4][16];
     const int pnl = blockIdx.y, xb = blockIdx.x, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
@@ -697,6 +697,11 @@ hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g
         hipLaunchKernelGGL((k_split_pack<3, true>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail);
         return hipGetLastError();
     }
+    if (sg.nkx > 0) {
+        if (sg.slices != 3) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_split_pack<3, false, true>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail);
+        return hipGetLastError();
+    }
     switch (sg.slices) {
     case 3: hipLaunchKernelGGL((k_split_pack<3, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
     case 4: hipLaunchKernelGGL((k_split_pack<4, false>), grid, block, 0, st, d_desc, g, sg, digits, scales, tail); break;
@@ -763,10 +768,11 @@ struct PlaneMath<true> {
 // A1 of group 0, A0, A1 of group 1, then B0, B1 of each group; 16 rows each of
 // block bi for A, ci for B), fetched with one 16-byte load per lane: the K =
 // nvir planes' scales, then (T8) the K = nocc planes' scales.
-// T8 (INT8, S <= kTail8MaxSlices): each group's K = nvir steps are recombined
-// first, then its K = nocc steps (own scales) are added in FP64.  Otherwise the
-// K = nocc half runs as an FP64 tail on v_mfma_f64_16x16x4_f64 after both groups.
-template <int S, int W, bool BF = false>
+// T8 (INT8 with occupied digit planes): each group's K = nvir steps are
+// recombined first, then its K = nocc steps (own scales) are added in FP64.
+// Otherwise the K = nocc half runs as an FP64 tail on v_mfma_f64_16x16x4_f64
+// after both groups.
+template <int S, int W, bool BF = false, bool T8 = false>
 __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffer_rsrc_t rd,
                                            const __amdgpu_buffer_rsrc_t rt,
                                            const double *__restrict__ scales, const Geom &g,
@@ -778,7 +784,7 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
     const int LJ = 2 * pp.li, LK = LJ + 1, LJT = 2 * pp.lk, LKT = LJT + 1;
     const int R1 = 2 * pp.rkj, R3 = R1 + 1, R1T = 2 * pp.rij, R3T = R1T + 1;
     const int nvp = g.nvp;
-    constexpr bool T8 = !BF && S <= kTail8MaxSlices;
+    static_assert(!T8 || (!BF && S <= kTail8MaxSlices), "occupied digit planes: INT8, S <= 3");
     dbl2 sreg, sreg_t = {0.0, 0.0};
     {
         const int sp = lane >> 3;
@@ -1259,8 +1265,9 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
 
 // One two-wave work item: the 16x16 block pair (bi, ci) of one task.  S = 0
 // takes the FP64 panels (k_fused2), 3..7 the INT8 digit planes of the split
-// form (k_fused_split), kSlicesBf16x3 the BF16 planes (k_fused_bf16x3) and
-// kSlicesFp32 the FP32 panels (k_fused32).
+// form (k_fused_split; 3 + kSlicesOccDigits with occupied digit planes),
+// kSlicesBf16x3 the BF16 planes (k_fused_bf16x3) and kSlicesFp32 the FP32
+// panels (k_fused32).
 template <int S, bool TENGY, bool WRITEF>
 __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
                                           int bi, int ci, int w, int lane,
@@ -1293,16 +1300,17 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
         } else {
             // INT8 digits (S planes) or the BF16x3 planes
             constexpr bool BF = (S == kSlicesBf16x3);
-            constexpr int NP = BF ? 3 : S;
+            constexpr bool T8 = (S == 3 + kSlicesOccDigits);
+            constexpr int NP = BF ? 3 : T8 ? 3 : S;
             const SplitGeom &sg = jd.sg;
             const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
                                                           (uint32_t)(sg.npanels * NP * sg.slice_bytes));
             const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
             if (w == 0)
-                split_wave<NP, 0, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x, lds_s[0]);
+                split_wave<NP, 0, BF, T8>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x, lds_s[0]);
             else if (!diag)
-                split_wave<NP, 1, BF>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x + 1024,
-                                      lds_s[1]);
+                split_wave<NP, 1, BF, T8>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x + 1024,
+                                          lds_s[1]);
         }
         pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v);
     }
@@ -1340,10 +1348,13 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
 }
 
 // The split form's work item (one 16x16 block pair of one task) on two waves.
+// S = 3..7, or 3 + kSlicesOccDigits (S = 3 with occupied digit planes, held to
+// three waves per SIMD: the compiler's own allocation lands a few VGPRs above
+// the 168 that allow it).
 template <int S, bool TENGY, bool WRITEF, bool PERSIST>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(S <= kTail8MaxSlices ? 3 : 1))) void k_fused_split(const JobDev *__restrict__ jobs,
-                                                     const Segment *__restrict__ segs,
-                                                     WorkHdr *hdr, int64_t fstride)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(S == 3 + kSlicesOccDigits ? 3 : 1)))
+void k_fused_split(const JobDev *__restrict__ jobs, const Segment *__restrict__ segs, WorkHdr *hdr,
+                   int64_t fstride)
 {
     pair_items<S, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
 }
@@ -1452,6 +1463,8 @@ hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *
         return hipGetLastError();
     }
     switch (slices) {
+    case 3 + kSlicesOccDigits:
+        return launch_split_s<3 + kSlicesOccDigits>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
     case 3: return launch_split_s<3>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
     case 4: return launch_split_s<4>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
     case 5: return launch_split_s<5>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
@@ -1474,6 +1487,9 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
         switch (slices) {
         case kSlicesBf16x3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_bf16x3<true, false, true>, 128, 0); break;
         case kSlicesFp32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused32<true, false, true>, 128, 0); break;
+        case 3 + kSlicesOccDigits:
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<3 + kSlicesOccDigits, true, false, true>, 128, 0);
+            break;
         case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<3, true, false, true>, 128, 0); break;
         case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<4, true, false, true>, 128, 0); break;
         case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_split<5, true, false, true>, 128, 0); break;

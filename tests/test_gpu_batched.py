@@ -138,26 +138,28 @@ def test_autotune_picks_a_working_candidate(cc, mode, slices):
 
 @pytest.mark.parametrize("illcond", [False, True])
 def test_autotune_chooses_slice_count(cc, illcond):
-    """slices=0 in split mode: the autotuner picks the smallest slice count whose
-    E[T] is within 1e-9 of the FP64 form's on the list, and that choice meets
-    the split form's 1e-8 bound."""
+    """slices=0 in split mode: the autotuner picks the cheapest precision level --
+    S = 3 with occupied digit planes, then S = 3, 4, .., 7 with the FP64 K = nocc
+    tail -- whose E[T] is within 1e-9 of the FP64 form's on the list, and that
+    choice meets the split form's 1e-8 bound."""
     nocc, nvir, kchunk = 8, 90, 4
     arrays = cc.make_inputs(nocc, nvir, kchunk, 61, illcond=illcond)
     trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    levels = [(3, 1)] + [(S, 0) for S in range(3, 8)]
     with cc.Job(arrays, nocc, nvir, kchunk) as job:
         best, table = job.autotune(trip, mode="ozaki_int8", slices=0, reps=1)
         f = job.run(trip, mode="fp64")
         s = job.run(trip, **best)
         E = f["emp4"] + f["emp5"]
+        pick = levels.index((best["slices"], best["occ_digits"]))
         devs = {}
-        for S in range(3, best["slices"]):
-            r = job.run(trip, mode="ozaki_int8", slices=S)
-            devs[S] = abs(r["emp4"] + r["emp5"] - E)
-    assert 3 <= best["slices"] <= 7
-    assert all(row["slices"] == best["slices"] for row in table)
+        for S, occ in levels[:pick]:
+            r = job.run(trip, mode="ozaki_int8", slices=S, occ_digits=occ)
+            devs[(S, occ)] = abs(r["emp4"] + r["emp5"] - E)
+    assert all((row["slices"], row["occ_digits"]) == levels[pick] for row in table)
     assert abs(s["emp4"] + s["emp5"] - E) <= 1e-9 * abs(E)
-    # no smaller slice count met the 1e-9 selection bound
-    assert all(devs[S] > 1e-9 * abs(E) for S in range(3, best["slices"]))
+    # no cheaper level met the 1e-9 selection bound
+    assert all(d > 1e-9 * abs(E) for d in devs.values())
 
 
 @pytest.mark.parametrize("mode,slices,variant", [("fp64", 0, 0), ("fp64", 0, 1), ("ozaki_int8", 4, 0)])

@@ -70,6 +70,42 @@ def test_ragged_shapes(cc, orc, nocc, nvir, kchunk):
     assert np.max(np.abs(got["terms"] - ref["terms"]) / np.maximum(np.abs(ref["terms"]), ref["abs4"])) < 1e-11
 
 
+@pytest.mark.parametrize("nocc,nvir,kchunk", [(1, 20, 1), (13, 33, 4), (64, 24, 8), (70, 50, 5)])
+def test_s3_occupied_digit_planes(cc, orc, nocc, nvir, kchunk):
+    """occ_digits at S = 3: the K = nocc half also runs as INT8 digit planes, with
+    row scales of its own, one 64-wide K step per 64 occupied orbitals: nocc = 1, a
+    partial step, exactly one step and two steps.  Every term within 1e-6 of its
+    summand scale (a dropped or mis-scaled K = nocc half is off by O(1)), and the
+    same list with the FP64 tail agrees to the same bound."""
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 500 + nocc)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    trip = trip[:: max(1, len(trip) // 60)]
+    ref = orc.run_list(arrays, 0, nocc, nvir, trip, with_abs=True)
+    got = run(cc, arrays, nocc, nvir, kchunk, trip, 3, terms=True, occ_digits=1)
+    assert np.max(np.abs(got["terms"] - ref["terms"]) / ref["abs4"]) < 1e-6
+    E = et(ref["emp4"], ref["emp5"])
+    assert abs(et(got["emp4"], got["emp5"]) - E) <= 1e-6 * np.sum(ref["abs4"])
+    tail = run(cc, arrays, nocc, nvir, kchunk, trip, 3, terms=True)
+    assert np.max(np.abs(got["terms"] - tail["terms"]) / ref["abs4"]) < 1e-6
+
+
+def test_occ_digits_needs_three_slices(cc):
+    nocc, nvir, kchunk = 4, 20, 2
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 5)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        for S in (4, 7):
+            with pytest.raises(cc.CcsdtError) as e:
+                job.run(trip, mode="ozaki_int8", slices=S, occ_digits=1)
+            assert e.value.code == -1
+        with pytest.raises(cc.CcsdtError):
+            job.run(trip, mode="ozaki_int8", slices=3, occ_digits=2)
+        # other modes ignore it
+        a = job.run(trip, mode="fp64")
+        b = job.run(trip, mode="fp64", occ_digits=1)
+        assert (a["emp4"], a["emp5"]) == (b["emp4"], b["emp5"])
+
+
 def test_split_f_matrices(cc, orc):
     nocc, nvir, kchunk = 5, 37, 3
     arrays = cc.make_inputs(nocc, nvir, kchunk, 17)
@@ -91,18 +127,23 @@ def test_production_shape_golden(cc, orc):
 
 def test_full_list_vs_fp64_form(cc):
     """All 4960 triples at 30/300: the split form's E[T] within 1e-8 of the FP64
-    form's at 4 slices and at 3 (the count the autotuner picks at this shape),
-    and bitwise repeatable."""
+    form's at 4 slices, at 3, and at 3 with occupied digit planes (the level the
+    autotuner picks at this shape), and bitwise repeatable."""
     arrays, blocks, trip, kchunk = cc.config_case(1)
     with cc.Job(arrays, 30, 300, kchunk) as job:
         a = job.run(trip, mode="fp64")
         b = job.run(trip, mode="ozaki_int8", slices=4, terms=True)
         c = job.run(trip, mode="ozaki_int8", slices=4, terms=True)
         d = job.run(trip, mode="ozaki_int8", slices=3, order=1)
+        e = job.run(trip, mode="ozaki_int8", slices=3, order=1, occ_digits=1, terms=True)
+        e2 = job.run(trip, mode="ozaki_int8", slices=3, order=0, occ_digits=1, terms=True,
+                     persistent=True, graph=True)
     E = et(a["emp4"], a["emp5"])
     assert abs(et(b["emp4"], b["emp5"]) - E) / abs(E) < ET_TOL
     assert abs(et(d["emp4"], d["emp5"]) - E) / abs(E) < ET_TOL
+    assert abs(et(e["emp4"], e["emp5"]) - E) / abs(E) < ET_TOL
     assert np.array_equal(b["terms"], c["terms"])
+    assert np.array_equal(e["terms"], e2["terms"])
 
 
 def test_bad_slice_count(cc):
