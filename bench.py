@@ -108,10 +108,11 @@ def job_alone(cc, jb):
     return cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"])
 
 
-def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536, budget=0):
+def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536, budget=0, occ_digits=0):
     """Feeds the jobs to a server at their arrival times (seconds after start)
     and returns per-job latency and results."""
-    srv = cc.Server(budget_bytes=budget, mode=mode, slices=slices, items_per_tick=items_per_tick)
+    srv = cc.Server(budget_bytes=budget, mode=mode, slices=slices, items_per_tick=items_per_tick,
+                    occ_digits=occ_digits)
     ids = [None] * len(jobs)
     nxt = 0
     start = srv.clock()
@@ -134,7 +135,7 @@ def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536, budget=0
 
 
 def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, tensors=False,
-                   budget=0, oracle_gflop=20.0, jobs=None):
+                   budget=0, oracle_gflop=20.0, jobs=None, occ_digits=0):
     """One Poisson serving run.  check: every job's E[T] against the same job
     run alone in the FP64 form on the GPU, and the jobs of at most
     oracle_gflop GEMM work against the CPU oracle (on gathered arrays for
@@ -142,9 +143,10 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
     if jobs is None:
         jobs = cc.serving_jobs(njobs, seed, tensors=tensors)
     arrivals = cc.poisson_arrivals(njobs, rate, seed)
-    res, lat, span = run_serving(cc, jobs, arrivals, mode, slices, budget=budget)
+    occ_digits = int(occ_digits) if mode == "ozaki_int8" else 0
+    res, lat, span = run_serving(cc, jobs, arrivals, mode, slices, budget=budget, occ_digits=occ_digits)
     flops = sum(len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) for jb in jobs)
-    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": mode if mode == "fp64" else "%s s=%d" % (mode, slices),
+    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": mode if mode == "fp64" else "%s s=%d%s" % (mode, slices, " occ_digits" if occ_digits else ""),
            "job_format": "tensors" if tensors else "harness",
            "jobs_per_s": njobs / span, "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
@@ -353,6 +355,9 @@ def main():
     ap.add_argument("--serving", default="", metavar="RATES",
                     help="only run the serving loop at these Poisson rates (jobs/s, comma list)")
     ap.add_argument("--serving-jobs", type=int, default=24)
+    ap.add_argument("--occ-digits", action="store_true",
+                    help="serving in the split form with the K=nocc half as INT8 digit planes "
+                         "too (needs --slices 3)")
     ap.add_argument("--tensors", action="store_true",
                     help="serving jobs as full-tensor jobs (every occupied slice resident)")
     ap.add_argument("--budget-gb", type=float, default=0.0,
@@ -388,13 +393,14 @@ def main():
         S = args.slices or 4
         jobs = cc.serving_jobs(args.serving_jobs, 0xC0DE0004, tensors=args.tensors)
         kw = dict(tensors=args.tensors, budget=int(args.budget_gb * 1e9), jobs=jobs)
+        occ = dict(occ_digits=int(args.occ_digits))
         # warm-up: one unmeasured pass of the same jobs, so the measured runs see
         # a server in steady state (kernels loaded, pinned staging and device
         # memory pools populated)
-        serving_report(cc, rates[0], args.serving_jobs, "ozaki_int8", S, check=False, **kw)
+        serving_report(cc, rates[0], args.serving_jobs, "ozaki_int8", S, check=False, **kw, **occ)
         reps = []
         for rate, mode, sl in [(r, "ozaki_int8", S) for r in rates] + [(rates[-1], "fp64", 0)]:
-            r = serving_report(cc, rate, args.serving_jobs, mode, sl, **kw)
+            r = serving_report(cc, rate, args.serving_jobs, mode, sl, **kw, **occ)
             reps.append(r)
             print("bench: serving %s %s rate %.0f: %.1f jobs/s p50 %.1f ms p99 %.1f ms" % (
                 r["job_format"], r["mode"], r["rate_jobs_per_s"], r["jobs_per_s"], r["p50_ms"],
@@ -533,7 +539,11 @@ def main():
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),
                        "gemm_only_ms": sep["ms_main"]},
     }
-    line["serving"] = serving_report(cc, 40.0, 16, "ozaki_int8", S, check=False)
+    # the server runs one split form for every job, whatever its shape, so the
+    # sample uses 4 slices (within 1e-8 in E[T] on every job shape of the serving
+    # sweep; 3 slices are not on the smallest jobs), each job checked
+    line["serving"] = serving_report(cc, 40.0, 16, "ozaki_int8", 4, check=True,
+                                     oracle_gflop=0.0 if args.no_cpu else 20.0)
     if not args.no_cpu:
         cb, ref_terms, done = cpu_baseline(arrays, trip, flops, args.cpu_seconds)
         line["cpu_baseline"] = cb

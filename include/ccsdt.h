@@ -331,6 +331,9 @@ typedef struct {
     int mode;                /* CCSDT_MODE_FP64, CCSDT_MODE_OZAKI_INT8 or CCSDT_MODE_BF16X3 */
     int slices;              /* Ozaki slices (0 = 4) */
     int variant;             /* CCSDT_TILE_* of the FP64 kernel */
+    int occ_digits;          /* CCSDT_MODE_OZAKI_INT8 with slices 3: 1 packs every job's
+                                K=nocc half as INT8 digit planes too (as
+                                ccsdt_run_opts.occ_digits); 0 otherwise */
 } ccsdt_server_opts;
 
 typedef struct {

@@ -125,7 +125,8 @@ class RunResult(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_int64), ("items_per_tick", ctypes.c_int),
-                ("mode", ctypes.c_int), ("slices", ctypes.c_int), ("variant", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("slices", ctypes.c_int), ("variant", ctypes.c_int),
+                ("occ_digits", ctypes.c_int)]
 
 
 class JobResult(ctypes.Structure):
@@ -555,9 +556,10 @@ class Job:
 class Server:
     """Continuous-batching server over the device (see ccsdt_server_* in ccsdt.h)."""
 
-    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0, variant=0):
+    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0, variant=0,
+                 occ_digits=0):
         o = ServerOpts(int(budget_bytes), int(items_per_tick), MODES[mode], int(slices),
-                       int(variant))
+                       int(variant), int(occ_digits))
         h = ctypes.c_void_p()
         _check(lib().ccsdt_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.handle = h

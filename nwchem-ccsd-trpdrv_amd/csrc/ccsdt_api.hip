@@ -1470,7 +1470,7 @@ int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, in
     if (is_split_mode(o.mode)) {
         const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
         const int npanels = (int)job_npanels(d->nocc, d->kchunk, kind);
-        const SplitGeom sg = make_split_geom(g, o.slices, npanels, o.mode == CCSDT_MODE_BF16X3);
+        const SplitGeom sg = make_split_geom(g, o.slices, npanels, o.mode == CCSDT_MODE_BF16X3, o.occ_digits != 0);
         b += npanels * (sg.slices * sg.slice_bytes + sg.tail_sz * 8 + split_scale_sets(sg) * (int64_t)g.nvp * 8);
     }
     return b;
@@ -1490,7 +1490,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
     if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * sj->n))) return rc;
     SplitGeom sg{};
     if (is_split_mode(s->o.mode)) {
-        sg = make_split_geom(g, s->o.slices, j->npanels, s->o.mode == CCSDT_MODE_BF16X3);
+        sg = make_split_geom(g, s->o.slices, j->npanels, s->o.mode == CCSDT_MODE_BF16X3, s->o.occ_digits != 0);
         if ((rc = ensure_split(j, sg))) return rc;
     }
     HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)sj->n * sizeof(Task), hipMemcpyHostToDevice, j->st));
@@ -1625,7 +1625,8 @@ int server_launch_tick(ccsdt_server s, bool *launched)
     HIP_TRY(hipMemcpyAsync(T.d_hdr, T.h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, s->st));
     if (is_split_mode(s->o.mode))
         HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false,
-                                   s->o.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3 : s->o.slices, 0,
+                                   s->o.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
+                                                                  : s->o.slices + s->o.occ_digits * kSlicesOccDigits, 0,
                                    true, false, s->st));
     else
         HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.variant, 0, true,
@@ -1755,7 +1756,7 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
     }
     if (is_split_mode(s->o.mode)) {
         const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, kind),
-                                             s->o.mode == CCSDT_MODE_BF16X3);
+                                             s->o.mode == CCSDT_MODE_BF16X3, s->o.occ_digits != 0);
         if ((int64_t)sg.npanels * sg.slices * sg.slice_bytes >= (int64_t(1) << 32) ||
             (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
@@ -1921,6 +1922,13 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
         if (o.slices == 0) o.slices = 4;
         if (o.slices < 3 || o.slices > 7) return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
         o.variant = 0;
+        if (o.occ_digits != 0 && o.occ_digits != 1)
+            return set_error(CCSDT_E_ARG, "occ_digits %d is neither 0 nor 1", o.occ_digits);
+        if (o.occ_digits && o.slices > kTail8MaxSlices)
+            return set_error(CCSDT_E_ARG, "occupied digit planes need slices <= %d (got %d)",
+                             kTail8MaxSlices, o.slices);
+    } else {
+        o.occ_digits = 0;
     }
     if (o.mode == CCSDT_MODE_BF16X3) {
         o.slices = 3;

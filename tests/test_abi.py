@@ -175,3 +175,30 @@ def test_inputs_shapes_and_ranges(cc):
         assert np.abs(a[n]).max() < 0.05
     ill = cc.make_inputs(nocc, nvir, kc, cc.config_seed(2), illcond=True)
     assert np.abs(ill["jia"]).min() < 1e-7 and np.abs(ill["tij"]).max() < 0.05
+
+
+def test_struct_layouts_match_header(cc, tmp_path):
+    """Every ctypes mirror of an include/ccsdt.h struct has the C size and field
+    offsets (a field added on one side only would shift the rest)."""
+    pairs = [("ccsdt_job_desc", cc.JobDesc), ("ccsdt_tensor_desc", cc.TensorDesc),
+             ("ccsdt_run_opts", cc.RunOpts), ("ccsdt_tune_entry", cc.TuneEntry),
+             ("ccsdt_run_result", cc.RunResult), ("ccsdt_server_opts", cc.ServerOpts),
+             ("ccsdt_job_result", cc.JobResult)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ccsdt.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.splitlines())
+    for cname, py in pairs:
+        assert int(got["%s size" % cname]) == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got["%s %s" % (cname, f[0])]) == getattr(py, f[0]).offset, (cname, f[0])
