@@ -37,8 +37,11 @@ typedef SumT<double> Sum4;
 
 // Energy summand of one (b,c) point.  n*, t* are f?n, f?t at (b,c); nr*, tr*
 // at (c,b); d = -1/(e_b + e_c + eaijk); u = t1v(b); x = dintx(c); y = dintc(c).
-// Contraction is off so the arithmetic follows omp/ccsd_tengy_c99.c:91-111
-// operation for operation.
+// The association follows omp/ccsd_tengy_c99.c:91-111 term for term; products
+// feeding an add are fused (FMA).  That changes each point's rounding by about
+// one ulp of its summands (far below the 1e-11 parity bound) and cuts the
+// epilogue's FP64 instructions by about a quarter: 1.5 % of the split kernel's
+// time at nocc=30, nvir=300 (the FP64 kernel, bound by its MFMAs, is unchanged).
 // T = float evaluates the same operations in FP32, as the FP32 body of
 // gpu-omp/ccsd_trpdrv_omp_cbody.c:204-221 does.
 template <typename T>
@@ -47,7 +50,7 @@ __device__ __forceinline__ void tengy_point(T n1, T n2, T n3, T n4, T t1, T t2, 
                                             T tr4, T d, T u1, T u2, T x1, T x2, T y1, T y2,
                                             SumT<T> &s)
 {
-#pragma clang fp contract(off)
+#pragma clang fp contract(fast)
     s.e4i += d * (t1 + nr1 + tr2 + n3 + nr4) * (t1 - t2 * 2 - t3 * 2 + t4)
            - d * (n1 + tr1 + nr2 + nr3) * (t1 * 2 - t2 - t3 + t4 * 2)
            + d * 3 * (n1 * (n1 + nr3 + tr4 * 2) + n2 * tr2 + n3 * t4);

@@ -146,6 +146,25 @@ def test_full_list_vs_fp64_form(cc):
     assert np.array_equal(e["terms"], e2["terms"])
 
 
+@pytest.mark.parametrize("illcond", [False, True])
+def test_split_precision_workload(cc, illcond):
+    """The split-precision workload at nocc=30, nvir=300 (its own seeds), on the
+    well-conditioned inputs and on the variant whose J/K entries are scaled by
+    10^u, u ~ U(-6,0): E[T] within 1e-8 of the FP64 form's at 3 slices, with the
+    FP64 occupied tail and with occupied digit planes, and the deviation shrinks
+    by at least 100x from 3 to 5 slices."""
+    arrays, blocks, trip, kchunk = cc.config_case(2, illcond=illcond)
+    with cc.Job(arrays, 30, 300, kchunk) as job:
+        a = job.run(trip, mode="fp64", order=1)
+        E = et(a["emp4"], a["emp5"])
+        dev = {}
+        for S, occ in ((3, 1), (3, 0), (5, 0)):
+            r = job.run(trip, mode="ozaki_int8", slices=S, occ_digits=occ, order=1)
+            dev[S, occ] = abs(et(r["emp4"], r["emp5"]) - E) / abs(E)
+    assert dev[3, 1] < ET_TOL and dev[3, 0] < ET_TOL, dev
+    assert dev[5, 0] * 100 < dev[3, 0], dev
+
+
 def test_bad_slice_count(cc):
     arrays, blocks, trip, kchunk = cc.config_case(0)
     with cc.Job(arrays, 8, 40, kchunk) as job:
