@@ -390,6 +390,13 @@ double ccsdt_debug_copy(int64_t bytes, int reps);
  * counterpart in the reference: measures the per-instruction cost of the vector
  * memory path that bounds the split kernel's digit loop.  -1 on failure. */
 double ccsdt_debug_load_rate(int bytes, int to_lds);
+/* Host-to-host copies of n buffers (dst[q] <- src[q], bytes[q] bytes) through the
+ * copy pool the synchronous drop-ins stage their arrays and f matrices with (the
+ * calling thread plus CCSDT_COPY_THREADS workers, default 4).  Needs no device.
+ * Returns the pool's worker count, or -1 for a negative size or null pointer.
+ * No counterpart in the reference (whose body reads host memory in place,
+ * omp/ccsd_trpdrv_omp_cbody.c:71-116). */
+int ccsdt_debug_host_copy(void *const *dst, const void *const *src, const int64_t *bytes, int n);
 
 #ifdef __cplusplus
 }

@@ -195,6 +195,97 @@ void host_free(void *p)
     }
 }
 
+// Host copies of the synchronous drop-ins (caller arrays to pinned staging, and
+// f from pinned staging back to the caller) are split over the calling thread
+// and a few workers: one thread copies at ~10-15 GB/s, below what PCIe moves, so
+// a single-threaded copy was the larger part of a drop-in call.  Workers are
+// started on first use; CCSDT_COPY_THREADS sets how many (0: the caller copies
+// alone; default 4).  Every worker takes part in every parallel copy, so the
+// chunk list is never rebuilt while one of them still reads it.
+struct HostCopy {
+    char *dst;
+    const char *src;
+    size_t n;
+};
+
+class CopyPool {
+public:
+    explicit CopyPool(int nthreads)
+    {
+        for (int t = 0; t < nthreads; ++t) th_.emplace_back([this] { loop(); });
+    }
+    int threads() const { return (int)th_.size(); }
+    // Copies every entry; returns when all bytes are in place.
+    void copy(const HostCopy *c, int n)
+    {
+        size_t total = 0;
+        for (int q = 0; q < n; ++q) total += c[q].n;
+        if (th_.empty() || total < kParallelMin) {
+            for (int q = 0; q < n; ++q)
+                if (c[q].n) memcpy(c[q].dst, c[q].src, c[q].n);
+            return;
+        }
+        std::lock_guard<std::mutex> one(call_mu_);  // one parallel copy at a time
+        chunks_.clear();
+        for (int q = 0; q < n; ++q)
+            for (size_t o = 0; o < c[q].n; o += kChunk)
+                chunks_.push_back({c[q].dst + o, c[q].src + o, std::min(kChunk, c[q].n - o)});
+        next_.store(0);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            finished_ = 0;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return finished_ == (int)th_.size(); });
+    }
+
+private:
+    static constexpr size_t kChunk = size_t(256) << 10;
+    static constexpr size_t kParallelMin = size_t(1) << 20;
+    void work()
+    {
+        const size_t nc = chunks_.size();
+        for (size_t i = next_.fetch_add(1); i < nc; i = next_.fetch_add(1))
+            memcpy(chunks_[i].dst, chunks_[i].src, chunks_[i].n);
+    }
+    void loop()
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (++finished_ == (int)th_.size()) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    std::vector<HostCopy> chunks_;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    int finished_ = 0;
+};
+
+CopyPool &copy_pool()
+{
+    // never destroyed: its workers stay parked until the process ends
+    static CopyPool *pool = [] {
+        int n = 4;
+        const char *e = getenv("CCSDT_COPY_THREADS");
+        if (e && *e) n = std::max(0, std::min(64, atoi(e)));
+        return new CopyPool(n);
+    }();
+    return *pool;
+}
+
 hipError_t stream_get(hipStream_t *st)
 {
     {
@@ -393,7 +484,7 @@ struct ccsdt_job_s {
     double *d_raw = nullptr;
     double *h_stage = nullptr;  // pinned staging for uploads
     double *h_vec = nullptr;    // pinned staging for eorb and the vector pool
-    double *h_fstage = nullptr; // pinned staging for f matrices (drop-in energy pass)
+    double *h_fstage = nullptr; // pinned staging for f matrices (drop-in energy pass and f write-back)
     double *d_ev = nullptr;     // virtual orbital energies, padded to nvp
     double *d_vec6 = nullptr;   // vector pool: vrows rows of nvp doubles (Task::vrow)
     int64_t vrows = 6;
@@ -536,9 +627,12 @@ int job_upload(ccsdt_job j, const double *const *arrays, const int64_t *slice_of
     INJECT_POINT("upload");
     if (j->h_stage) {
         // small, repeated uploads (the drop-in context): one copy from pinned memory
+        HostCopy c[CCSDT_NARRAYS];
         for (int q = 0; q < CCSDT_NARRAYS; ++q)
-            memcpy(j->h_stage + j->off[q], arrays[q] + (slice_off ? slice_off[q] : 0),
-                   (size_t)j->cnt[q] * sizeof(double));
+            c[q] = {reinterpret_cast<char *>(j->h_stage + j->off[q]),
+                    reinterpret_cast<const char *>(arrays[q] + (slice_off ? slice_off[q] : 0)),
+                    (size_t)j->cnt[q] * sizeof(double)};
+        copy_pool().copy(c, CCSDT_NARRAYS);
         HIP_TRY(hipMemcpyAsync(j->d_raw, j->h_stage, (size_t)j->nraw * sizeof(double),
                                hipMemcpyHostToDevice, j->st));
     } else {
@@ -966,6 +1060,9 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     } else if (fouts) {
         if ((rc = grow(&j->d_f, &j->cap_f, 8 * vv))) return rc;
     }
+    // a staged job (the drop-in context) brings f back through pinned memory
+    const bool fstaged = fouts && j->h_stage;
+    if (fstaged && !j->h_fstage) HIP_TRY(host_alloc((void **)&j->h_fstage, (size_t)8 * vv * sizeof(double)));
     hipEvent_t *E = j->ev;
     int64_t launches = 0;
     double ms_main = 0.0, ms_tengy = 0.0;
@@ -1063,13 +1160,25 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             HIP_TRY(hipMemcpyAsync(terms, j->d_terms, (size_t)4 * n * sizeof(double),
                                    hipMemcpyDeviceToHost, j->st));
     }
-    if (fouts) {
+    if (fstaged) {
+        HIP_TRY(hipMemcpyAsync(j->h_fstage, j->d_f, (size_t)8 * vv * sizeof(double),
+                               hipMemcpyDeviceToHost, j->st));
+    } else if (fouts) {
         for (int m = 0; m < 8; ++m)
             if (fouts[m])
                 HIP_TRY(hipMemcpyAsync(fouts[m], j->d_f + m * vv, (size_t)vv * sizeof(double),
                                        hipMemcpyDeviceToHost, j->st));
     }
     HIP_TRY(hipStreamSynchronize(j->st));
+    if (fstaged) {
+        HostCopy c[8];
+        int nc = 0;
+        for (int m = 0; m < 8; ++m)
+            if (fouts[m])
+                c[nc++] = {reinterpret_cast<char *>(fouts[m]),
+                           reinterpret_cast<const char *>(j->h_fstage + m * vv), (size_t)vv * sizeof(double)};
+        copy_pool().copy(c, nc);
+    }
     if (use_graph && terms) memcpy(terms, j->h_terms, (size_t)4 * n * sizeof(double));
     if (res) {
         res->emp4 = j->h_out[0];
@@ -1363,8 +1472,11 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
         if (!rc && injected()) rc = set_error(CCSDT_E_INJECTED, "injected failure at launch");
         if (!rc) {
             auto run = [&]() -> int {
+                HostCopy c[8];
                 for (int m = 0; m < 8; ++m)
-                    memcpy(j->h_fstage + m * vv, fs[m], (size_t)vv * sizeof(double));
+                    c[m] = {reinterpret_cast<char *>(j->h_fstage + m * vv),
+                            reinterpret_cast<const char *>(fs[m]), (size_t)vv * sizeof(double)};
+                copy_pool().copy(c, 8);
                 HIP_TRY(hipMemcpyAsync(j->d_f, j->h_fstage, (size_t)8 * vv * sizeof(double),
                                        hipMemcpyHostToDevice, j->st));
                 j->h_tasks[0] = Task{0, 0, 0, 0, 1, (float)eaijk, eaijk, {0, 1, 2, 3, 4, 5}, 0};
@@ -2144,6 +2256,25 @@ double ccsdt_debug_load_rate(int bytes, int to_lds)
     // wave-wide loads per CU: grid/CUs workgroups x 4 waves x iters x 8
     const double loads_per_cu = (double)grid / g_num_cu * 4.0 * iters * 8.0;
     return ms * 1e6 / loads_per_cu;
+}
+
+int ccsdt_debug_host_copy(void *const *dst, const void *const *src, const int64_t *bytes, int n)
+{
+    if (n < 0 || (n > 0 && (!dst || !src || !bytes))) {
+        set_error(CCSDT_E_ARG, "host copy: null list");
+        return -1;
+    }
+    std::vector<HostCopy> c(n);
+    for (int q = 0; q < n; ++q) {
+        if (bytes[q] < 0 || (bytes[q] > 0 && (!dst[q] || !src[q]))) {
+            set_error(CCSDT_E_ARG, "host copy: entry %d", q);
+            return -1;
+        }
+        c[q] = {static_cast<char *>(dst[q]), static_cast<const char *>(src[q]), (size_t)bytes[q]};
+    }
+    CopyPool &pool = copy_pool();
+    pool.copy(c.data(), n);
+    return pool.threads();
 }
 
 double ccsdt_debug_copy(int64_t bytes, int reps)

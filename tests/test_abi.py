@@ -202,3 +202,53 @@ def test_struct_layouts_match_header(cc, tmp_path):
         assert int(got["%s size" % cname]) == ctypes.sizeof(py), cname
         for f in py._fields_:
             assert int(got["%s %s" % (cname, f[0])]) == getattr(py, f[0]).offset, (cname, f[0])
+
+
+def _host_copy_check(cc, rng, sizes):
+    L = cc.lib()
+    src = [rng.integers(0, 256, n, dtype=np.uint8) for n in sizes]
+    dst = [np.zeros(n, np.uint8) for n in sizes]
+    vp = ctypes.c_void_p
+    D = (vp * len(sizes))(*[d.ctypes.data for d in dst])
+    S = (vp * len(sizes))(*[s.ctypes.data for s in src])
+    B = (ctypes.c_int64 * len(sizes))(*sizes)
+    nthreads = L.ccsdt_debug_host_copy(D, S, B, len(sizes))
+    assert nthreads >= 0
+    for d, s in zip(dst, src):
+        assert np.array_equal(d, s)
+    return nthreads
+
+
+def test_host_copy_pool(cc):
+    """The drop-ins' host copy pool (no device): every byte lands, for lists of
+    many odd-sized buffers above and below the parallel threshold, over many
+    calls in a row (a chunk list reused while a worker still read it would show
+    as a wrong byte here)."""
+    rng = np.random.default_rng(7)
+    for it in range(40):
+        n = int(rng.integers(1, 22))
+        sizes = [int(x) for x in rng.integers(0, 1 << int(rng.integers(4, 22)), n)]
+        _host_copy_check(cc, rng, sizes)
+    # the drop-in's shape at nocc=30, nvir=300: 21 arrays, 4.9 MB in all
+    sizes = [8 * x for x in (330, 90000, 90000, 9000, 9000, 9000, 9000, 90000, 90000, 90000, 90000,
+                             9000, 9000, 9000, 9000, 300, 300, 300, 300, 300, 300)]
+    assert _host_copy_check(cc, rng, sizes) == 4  # default worker count
+    L = cc.lib()
+    assert L.ccsdt_debug_host_copy(None, None, None, 1) == -1
+    neg = (ctypes.c_int64 * 1)(-1)
+    one = (ctypes.c_void_p * 1)(1)
+    assert L.ccsdt_debug_host_copy(one, one, neg, 1) == -1
+
+
+def test_host_copy_pool_thread_count(cc):
+    """CCSDT_COPY_THREADS sets the worker count (0: the caller copies alone)."""
+    import subprocess
+    import sys
+    code = ("import ctypes, numpy as np, sys; sys.path.insert(0, %r); import __graft_entry__ as g; "
+            "cc = g.package(); L = cc.lib(); a = np.arange(1 << 19, dtype=np.float64); b = np.zeros_like(a); "
+            "vp = ctypes.c_void_p; n = L.ccsdt_debug_host_copy((vp*1)(b.ctypes.data), (vp*1)(a.ctypes.data), "
+            "(ctypes.c_int64*1)(a.nbytes), 1); assert np.array_equal(a, b); print(n)" % ROOT)
+    for env, want in (("0", 0), ("3", 3)):
+        out = subprocess.run([sys.executable, "-c", code], check=True, capture_output=True, text=True,
+                             env=dict(os.environ, CCSDT_COPY_THREADS=env)).stdout.split()
+        assert int(out[-1]) == want
