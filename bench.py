@@ -247,14 +247,15 @@ def dropin_report(cc, arrays, trip, nocc, nvir, flops, ncalls=24):
     (the reference leaves f1n..f4t filled), copies 8 nvir^2 doubles back."""
     out = {}
     f = [np.zeros(nvir * nvir) for _ in range(8)]
+    # pointer arguments built once, as a compiled caller has them
+    body = cc.BodyCall(f, arrays["eorb"], 0, nocc, nvir, arrays)
     for wb in (1, 0):
         cc.lib().ccsdt_set_f_writeback(wb)
-        e4 = e5 = 0.0
-        for a, i, j, k, klo in trip[:2]:
-            e4, e5 = cc.cbody(f, arrays["eorb"], 0, nocc, nvir, e4, e5, a, i, j, k, klo, arrays)
+        for t in trip[:2]:
+            body.call(*t)
         t0 = time.perf_counter()
-        for a, i, j, k, klo in trip[:ncalls]:
-            e4, e5 = cc.cbody(f, arrays["eorb"], 0, nocc, nvir, e4, e5, a, i, j, k, klo, arrays)
+        for t in trip[:ncalls]:
+            body.call(*t)
         dt = (time.perf_counter() - t0) / ncalls
         out["f_writeback" if wb else "no_f_writeback"] = {
             "ms_per_call": dt * 1e3, "triples_per_s": 1.0 / dt,

@@ -680,6 +680,39 @@ def cbody(f, eorb, ncor, nocc, nvir, emp4, emp5, a, i, j, k, klo, arrays,
     return e4.value, e5.value
 
 
+class BodyCall:
+    """The drop-in body with its pointer arguments built once, for callers that
+    make many calls on the same buffers (the timing loop of a C or Fortran caller
+    pays no per-call argument conversion either).  ``call(a, i, j, k, klo)``
+    accumulates into ``emp4``/``emp5`` as the body does."""
+
+    def __init__(self, f, eorb, ncor, nocc, nvir, arrays, symbol="ccsd_trpdrv_omp_cbody_"):
+        self._keep = [np.ascontiguousarray(arrays[n], dtype=np.float64) for n in ARRAY_NAMES[1:]]
+        self._keep.append(np.ascontiguousarray(eorb, dtype=np.float64))
+        self._f = f
+        self._ints = [ctypes.c_int(v) for v in (ncor, nocc, nvir, 0, 0, 0, 0, 0)]
+        self._e4, self._e5 = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        self._fn = getattr(lib(), symbol)
+        self._args = ([_dptr(x) for x in f] + [_dptr(self._keep[-1])]
+                      + [ctypes.byref(v) for v in self._ints[:3]]
+                      + [ctypes.byref(self._e4), ctypes.byref(self._e5)]
+                      + [ctypes.byref(v) for v in self._ints[3:]]
+                      + [_dptr(x) for x in self._keep[:-1]])
+
+    @property
+    def emp4(self):
+        return self._e4.value
+
+    @property
+    def emp5(self):
+        return self._e5.value
+
+    def call(self, a, i, j, k, klo):
+        for v, x in zip(self._ints[3:], (a, i, j, k, klo)):
+            v.value = int(x)
+        self._fn(*self._args)
+
+
 def tengy(f, vec, eorb, eaijk, ncor, nocc, nvir, symbol="ccsd_tengy_omp"):
     """Drop-in energy reduction; f as in cbody, vec = dict of the six vectors.
 

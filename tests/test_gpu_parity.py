@@ -126,6 +126,23 @@ def test_dropin_body(cc, orc):
     assert abs(e4 - r4) <= TOL * abs(r4) and abs(e5 - r5) <= TOL * abs(r5)
 
 
+def test_prepared_body_call_equals_cbody(cc):
+    """BodyCall (pointer arguments built once, as bench.py times the drop-in)
+    gives bitwise the f matrices and running totals of per-call cbody, over a
+    sequence of triples at the 30/300 shapes with uploads of new contents."""
+    arrays, blocks, trip, kchunk = cc.config_case(1)
+    trip = trip[::613][:8]
+    fa = [np.zeros(300 * 300) for _ in range(8)]
+    fb = [np.zeros(300 * 300) for _ in range(8)]
+    body = cc.BodyCall(fb, arrays["eorb"], 0, 30, 300, arrays)
+    e4 = e5 = 0.0
+    for t in trip:
+        e4, e5 = cc.cbody(fa, arrays["eorb"], 0, 30, 300, e4, e5, *t, arrays)
+        body.call(*t)
+        assert (body.emp4, body.emp5) == (e4, e5)
+        assert all(np.array_equal(x, y) for x, y in zip(fa, fb))
+
+
 @pytest.mark.parametrize("symbol", ["ccsd_trpdrv_cbody_", "ccsd_trpdrv_omp_fbody_"])
 def test_dropin_aliases(cc, symbol):
     nocc, nvir = 4, 20
