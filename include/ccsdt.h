@@ -390,6 +390,12 @@ double ccsdt_debug_copy(int64_t bytes, int reps);
  * counterpart in the reference: measures the per-instruction cost of the vector
  * memory path that bounds the split kernel's digit loop.  -1 on failure. */
 double ccsdt_debug_load_rate(int bytes, int to_lds);
+/* on = 0: jobs whose runs start after this call pack the K = nocc digit planes of
+ * CCSDT_MODE_OZAKI_INT8 with occ_digits one digit per 64-wide K step even when
+ * nocc <= 32 (the default, on = 1, puts two digits in each step there: fewer loads
+ * and MFMAs, the same integer products, bitwise the same results).  For tests
+ * that compare the two layouts.  No counterpart in the reference. */
+void ccsdt_debug_occ_fold(int on);
 /* Host-to-host copies of n buffers (dst[q] <- src[q], bytes[q] bytes) through the
  * copy pool the synchronous drop-ins stage their arrays and f matrices with (the
  * calling thread plus CCSDT_COPY_THREADS workers, default 4).  Needs no device.

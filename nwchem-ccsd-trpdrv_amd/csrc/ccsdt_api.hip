@@ -357,6 +357,10 @@ Geom make_geom(int ncor, int nocc, int nvir)
     return g;
 }
 
+// Folded occupied digit slots for nocc <= 32 (ccsdt_debug_occ_fold switches them
+// off, to compare both layouts on one job).
+std::atomic<int> g_occ_fold{1};
+
 SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = false,
                           bool occ_digits = false)
 {
@@ -367,7 +371,10 @@ SplitGeom make_split_geom(const Geom &g, int slices, int npanels, bool bf16 = fa
     sg.kv = (int)round_up(g.nvir, kstep);
     sg.nkv = sg.kv / kstep;
     const bool tail8 = !bf16 && occ_digits && sg.slices <= kTail8MaxSlices;
-    sg.kx = tail8 ? (int)round_up(g.nocc, 64) : 0;
+    // nocc <= 32 fills at most half of a 64-wide K step: two digits share each
+    // step (five slots in two steps of the three planes)
+    sg.fold = (tail8 && g.nocc <= 32 && g_occ_fold.load()) ? 1 : 0;
+    sg.kx = tail8 ? (sg.fold ? 128 : (int)round_up(g.nocc, 64)) : 0;
     sg.nkx = sg.kx / 64;
     sg.kt = (int)round_up(g.nocc, 8);
     sg.nkt = sg.kt / 4;
@@ -877,7 +884,7 @@ int slice_code(const SplitGeom &sg)
 // planes) of the split form for sg.slices.
 int ensure_split(ccsdt_job j, const SplitGeom &sg)
 {
-    const int key = slice_code(sg);  // INT8 S (+ occupied digit planes) or BF16x3
+    const int key = slice_code(sg) + 1024 * sg.fold;  // INT8 S (+ occupied digit planes, folded) or BF16x3
     if (j->split_slices == key) return CCSDT_OK;
     if (j->d_digits) dev_free(j->d_digits);
     if (j->d_scales) dev_free(j->d_scales);
@@ -1081,7 +1088,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             }
             // the signature, and every buffer the captured nodes reference (a run
             // of another size may have reallocated them)
-            const int64_t key[kGraphKey] = {mode, slices + occ * kSlicesOccDigits, n, persistent,
+            const int64_t key[kGraphKey] = {mode, slices + occ * kSlicesOccDigits + 1024 * sg.fold, n, persistent,
                                      terms != nullptr, variant,
                                      persistent ? wg_per_cu : 0, order,
                                      (int64_t)(intptr_t)j->d_order, (int64_t)(intptr_t)j->h_order,
@@ -2256,6 +2263,11 @@ double ccsdt_debug_load_rate(int bytes, int to_lds)
     // wave-wide loads per CU: grid/CUs workgroups x 4 waves x iters x 8
     const double loads_per_cu = (double)grid / g_num_cu * 4.0 * iters * 8.0;
     return ms * 1e6 / loads_per_cu;
+}
+
+void ccsdt_debug_occ_fold(int on)
+{
+    g_occ_fold.store(on ? 1 : 0);
 }
 
 int ccsdt_debug_host_copy(void *const *dst, const void *const *src, const int64_t *bytes, int n)

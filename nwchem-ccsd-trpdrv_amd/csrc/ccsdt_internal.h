@@ -70,7 +70,9 @@ struct SplitGeom {
     int kv, nkv;          // kv = round_up(nvir, 64), nkv = kv / 64 (BF16: 32)
     int kx, nkx;          // INT8 with occupied digit planes (S <= kTail8MaxSlices): the K =
                           // nocc half as planes of its own, kx = round_up(nocc, 64),
-                          // nkx = kx / 64; else 0
+                          // nkx = kx / 64; folded (below): kx = 128, nkx = 2; else 0
+    int fold;             // 1: occupied digit planes with nocc <= 32, two digits per
+                          // 64-wide K step (the folded slots of k_split_pack)
     int kt, nkt;          // FP64 K = nocc tail (when nkx = 0): kt = round_up(nocc, 8), nkt = kt / 4 (even)
     int64_t slice_bytes;  // bytes of one digit plane of one panel: nvp * (kv + kx) (BF16: 2 nvp kv)
     int64_t tail_sz;      // doubles of one FP64 tail panel: nvp * kt (0 when nkx > 0)

@@ -671,10 +671,61 @@ __global__ __launch_bounds__(256) void k_split_pack(const PanelDesc *__restrict_
         return d.first_t ? d.P[k + (int64_t)x * nv] : d.P[x + (int64_t)k * nv];
     }, sg.nkv, 0, scales);
     if (T8) {
-        split_part([&](int k) -> double {
+        auto qv = [&](int k) -> double {
             if (x >= nv || k >= no) return 0.0;
             return d.sgn * (d.second_t ? d.Q[k + (int64_t)x * no] : d.Q[x + (int64_t)k * nv]);
-        }, sg.nkx, sg.nkv, scales + (int64_t)sg.npanels * g.nvp);
+        };
+        double *sset = scales + (int64_t)sg.npanels * g.nvp;
+        if (!sg.fold) {
+            split_part(qv, sg.nkx, sg.nkv, sset);
+            return;
+        }
+        // Folded (nocc <= 32): a 64-wide K step holds two digits of the same
+        // element k < 32, one in K 0..31 and one in K 32..63, so a product of two
+        // such slots sums two digit products of one level.  A panel is an A
+        // operand in one wave and a B operand in the other, so both forms are
+        // stored: A: [d1|d2] (plane 0, step nkv), [d3|0] (plane 1, step nkv);
+        // B: [d1|0] (plane 2, step nkv), [d2|d1] (plane 0, step nkv+1), [d3|d2]
+        // (plane 1, step nkv+1); plane 2 of step nkv+1 is zero.  Then level 0 =
+        // [d1|d2].[d1|0], level 1 = [d1|d2].[d2|d1], level 2 = [d1|d2].[d3|d2] +
+        // [d3|0].[d1|0].  The digits are those the unfolded form computes.
+        const bool lo = w < 2;
+        const int kb = (w & 1) * 16 + c * 4;  // this thread's four elements
+        double m = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m = fmax(m, fabs(qv(kb + j)));
+        m = fmax(m, __shfl_xor(m, 1, 64));
+        m = fmax(m, __shfl_xor(m, 2, 64));
+        if (c == 0) rmax[w][r] = m;
+        __syncthreads();
+        m = fmax(fmax(rmax[0][r], rmax[1][r]), fmax(rmax[2][r], rmax[3][r]));
+        int e = 0;
+        if (m > 0.0) (void)frexp(m * (128.0 / 127.0), &e);
+        const double inv = ldexp(1.0, -e);
+        if (w == 0 && c == 0) sset[(int64_t)pnl * g.nvp + x] = ldexp(1.0, e);
+        uint32_t word[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double v = qv(kb + j) * inv;
+            int dg[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                v *= 128.0;
+                const double t = rint(v);
+                v -= t;
+                dg[q] = (int)t;
+            }
+            const int slot[6] = {lo ? dg[0] : dg[1], lo ? dg[2] : 0, lo ? dg[0] : 0,
+                                 lo ? dg[1] : dg[0], lo ? dg[2] : dg[1], 0};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) word[q] |= (uint32_t)(uint8_t)(int8_t)slot[q] << (8 * j);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int plane_q = q % 3, step = sg.nkv + q / 3;
+            const int64_t o = ((int64_t)(xb * nsb + step) * 1024 + (w * 16 + r) * 16 + c * 4) / 4;
+            pl[((int64_t)pnl * NPL + plane_q) * plane_words + o] = word[q];
+        }
         return;
     }
     // FP64 tail in the step-pair layout (nkt is even): element 128*s + 2*l + h
@@ -910,9 +961,37 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             // (the scheduling barrier keeps these loads from being hoisted above the
             // recombination, where their registers would add to its own)
             __builtin_amdgcn_sched_barrier(0);
-            for (int kx = 0; kx < sg.nkx; ++kx) {
-                SPLIT_LOAD(xa0, xa1, xb0, xb1, nkv + kx)
-                SPLIT_MFMA(xa0, xa1, xb0, xb1)
+            if (sg.fold) {
+                // folded slots (k_split_pack, nocc <= 32): A [d1|d2], [d3|0] and
+                // B [d1|0], [d2|d1], [d3|d2]; four MFMAs per product for the
+                // three levels, 5 loads per operand pair instead of 6
+                const uint32_t k0 = (uint32_t)nkv * 1024, k1 = k0 + 1024;
+                xa0[0] = buffer_load_i8x16(rd, voff8, oa0 + k0);
+                xa1[0] = buffer_load_i8x16(rd, voff8, oa1 + k0);
+                xa0[1] = buffer_load_i8x16(rd, voff8, oa0 + plane + k0);
+                xa1[1] = buffer_load_i8x16(rd, voff8, oa1 + plane + k0);
+                xb0[0] = buffer_load_i8x16(rd, voff8, ob0 + 2 * plane + k0);
+                xb1[0] = buffer_load_i8x16(rd, voff8, ob1 + 2 * plane + k0);
+                xb0[1] = buffer_load_i8x16(rd, voff8, ob0 + k1);
+                xb1[1] = buffer_load_i8x16(rd, voff8, ob1 + k1);
+                xb0[2] = buffer_load_i8x16(rd, voff8, ob0 + plane + k1);
+                xb1[2] = buffer_load_i8x16(rd, voff8, ob1 + plane + k1);
+#pragma unroll
+                for (int lv = 0; lv < 3; ++lv) {
+                    iacc[0][lv] = PlaneMath<BF>::mfma(xa0[0], xb0[lv], iacc[0][lv]);
+                    iacc[1][lv] = PlaneMath<BF>::mfma(xa1[0], xb0[lv], iacc[1][lv]);
+                    iacc[2][lv] = PlaneMath<BF>::mfma(xa0[0], xb1[lv], iacc[2][lv]);
+                    iacc[3][lv] = PlaneMath<BF>::mfma(xa1[0], xb1[lv], iacc[3][lv]);
+                }
+                iacc[0][2] = PlaneMath<BF>::mfma(xa0[1], xb0[0], iacc[0][2]);
+                iacc[1][2] = PlaneMath<BF>::mfma(xa1[1], xb0[0], iacc[1][2]);
+                iacc[2][2] = PlaneMath<BF>::mfma(xa0[1], xb1[0], iacc[2][2]);
+                iacc[3][2] = PlaneMath<BF>::mfma(xa1[1], xb1[0], iacc[3][2]);
+            } else {
+                for (int kx = 0; kx < sg.nkx; ++kx) {
+                    SPLIT_LOAD(xa0, xa1, xb0, xb1, nkv + kx)
+                    SPLIT_MFMA(xa0, xa1, xb0, xb1)
+                }
             }
             recombine(sst + 128, true);
             if (gi == 0) asm volatile("" ::: "memory");

@@ -63,6 +63,27 @@ def test_budget_makes_jobs_wait(cc):
     assert admits == sorted(admits)
 
 
+def test_occ_digits_folded_and_plain_jobs_serve_together(cc):
+    """Occupied digit planes in serving: jobs with nocc <= 32 (two digits per K
+    step) and nocc > 32 (one) share ticks; each job's totals equal its plain run
+    bitwise."""
+    jobs = cc.serving_jobs(4, 77, nocc_range=(28, 37), nvir_range=(16, 24))
+    assert min(jb["nocc"] for jb in jobs) <= 32 < max(jb["nocc"] for jb in jobs)
+    kw = dict(mode="ozaki_int8", slices=3)
+    alone = []
+    for jb in jobs:
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            alone.append(job.run(jb["triples"], occ_digits=1, **kw))
+    with cc.Server(occ_digits=1, items_per_tick=4096, **kw) as srv:
+        ids = [srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+               for jb in jobs]
+        srv.drain()
+        mixed = [srv.result(i) for i in ids]
+    for a, m in zip(alone, mixed):
+        assert m is not None and m["status"] == 0
+        assert (a["emp4"], a["emp5"]) == (m["emp4"], m["emp5"])
+
+
 def test_server_occ_digits_needs_three_slices(cc):
     for kw in (dict(slices=4, occ_digits=1), dict(slices=3, occ_digits=2)):
         with pytest.raises(cc.CcsdtError):

@@ -89,6 +89,32 @@ def test_s3_occupied_digit_planes(cc, orc, nocc, nvir, kchunk):
     assert np.max(np.abs(got["terms"] - tail["terms"]) / ref["abs4"]) < 1e-6
 
 
+@pytest.mark.parametrize("nocc,nvir,kchunk", [(1, 20, 1), (13, 33, 4), (30, 64, 16), (32, 40, 8)])
+def test_folded_occupied_planes_bitwise(cc, nocc, nvir, kchunk):
+    """nocc <= 32: two occupied digits share each 64-wide K step (fewer loads and
+    MFMAs).  The integer digit products are the same, so every term equals the
+    one-digit-per-step layout's bitwise; switching the layout inside one job
+    re-packs, and a graph-replayed persistent run agrees too."""
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 900 + nocc)
+    trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+    trip = trip[:: max(1, len(trip) // 80)]
+    L = cc.lib()
+    kw = dict(mode="ozaki_int8", slices=3, occ_digits=1, terms=True)
+    with cc.Job(arrays, nocc, nvir, kchunk) as job:
+        folded = job.run(trip, **kw)
+        try:
+            L.ccsdt_debug_occ_fold(0)
+            plain = job.run(trip, **kw)
+            plain_g = job.run(trip, order=1, graph=True, persistent=True, **kw)
+        finally:
+            L.ccsdt_debug_occ_fold(1)
+        folded_g = job.run(trip, order=1, graph=True, persistent=True, **kw)
+    assert np.array_equal(folded["terms"], plain["terms"])
+    assert np.array_equal(folded_g["terms"], plain["terms"])
+    assert np.array_equal(plain_g["terms"], plain["terms"])
+    assert (folded["emp4"], folded["emp5"]) == (plain["emp4"], plain["emp5"])
+
+
 def test_occ_digits_needs_three_slices(cc):
     nocc, nvir, kchunk = 4, 20, 2
     arrays = cc.make_inputs(nocc, nvir, kchunk, 5)
