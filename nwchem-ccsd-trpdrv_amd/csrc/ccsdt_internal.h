@@ -62,8 +62,11 @@ enum { SL_R1 = 0, SL_R3 = 1, SL_LJT = 2, SL_LKT = 3 };
 
 // Geometry of the Ozaki INT8 split form: the K = nvir half of every panel as
 // `slices` int8 digit planes (kv = nvir rounded up to 64, one
-// v_mfma_i32_16x16x64_i8 K step per 64), and the K = nocc half as an FP64 tail
-// panel (kt = nocc rounded up to 8).
+// v_mfma_i32_16x16x64_i8 K step per 64), and the K = nocc half either as an
+// FP64 tail panel (kt = nocc rounded up to 8) or, at 3 slices with occupied
+// digit planes, as more K steps of the same planes under row scales of their
+// own (folded two digits per step when nocc <= 32).  BF16x3 uses the same
+// fields with 32-wide K steps.
 struct SplitGeom {
     int slices;
     int bf16;             // 1: three BF16 planes (v_mfma_f32_16x16x32_bf16), not INT8 digits

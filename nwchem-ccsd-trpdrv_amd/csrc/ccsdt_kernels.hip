@@ -780,7 +780,8 @@ __device__ __forceinline__ int4v buffer_load_i8x16(__amdgpu_buffer_rsrc_t r, uin
 // B panels.  After the INT8 groups each wave adds its FP64 tail (K = nocc) on
 // v_mfma_f64_16x16x4_f64 with the A rows fed in the order 4*(i%4) + i/4, which
 // makes its result layout (lane l, element r -> row 4*(l/16)+r, column l%16)
-// equal that of the INT8 MFMA.  The energy epilogue swaps half the values
+// equal that of the INT8 MFMA; with occupied digit planes (T8) each group
+// instead runs the K = nocc steps as INT8 digit products too.  The energy epilogue swaps half the values
 // through LDS and each wave finishes two of the four rows per lane.
 // The eight products of one wave role (W = 0: (b,c); W = 1: mirrored (c,b)),
 // panel ids resolved at compile time so every operand offset stays scalar.
