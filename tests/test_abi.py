@@ -252,3 +252,18 @@ def test_host_copy_pool_thread_count(cc):
         out = subprocess.run([sys.executable, "-c", code], check=True, capture_output=True, text=True,
                              env=dict(os.environ, CCSDT_COPY_THREADS=env)).stdout.split()
         assert int(out[-1]) == want
+
+
+def test_server_options_validated_before_device(cc):
+    """Server options are checked on the host before any device work: occupied
+    digit planes need the INT8 form at 3 slices (CCSDT_E_ARG otherwise); valid
+    options get past that check (then, without a GPU, fail as a device error)."""
+    L = cc.lib()
+    h = ctypes.c_void_p()
+    for mode, slices, occ in (("ozaki_int8", 4, 1), ("ozaki_int8", 3, 2), ("ozaki_int8", 9, 0)):
+        o = cc.ServerOpts(0, 0, cc.MODES[mode], slices, 0, occ)
+        assert L.ccsdt_server_create(ctypes.byref(o), ctypes.byref(h)) == -1, (mode, slices, occ)
+    if not _has_gpu(cc):
+        for mode, slices, occ in (("ozaki_int8", 3, 1), ("fp64", 0, 1)):
+            o = cc.ServerOpts(0, 0, cc.MODES[mode], slices, 0, occ)
+            assert L.ccsdt_server_create(ctypes.byref(o), ctypes.byref(h)) == -2, (mode, slices, occ)
