@@ -20,10 +20,19 @@ grid, and for the split form the slice count (the smallest whose E[T] is within
 
 `value` is the split form's FP64-equivalent TFLOP/s (16*nvir^2*(nvir+nocc) flops
 per triple, 2 per multiply-add) when its E[T] = emp4+emp5 agrees with the FP64
-form's to 1e-8 relative, else the FP64 form's.  The JSON line also carries the
-roofline of the dominant kernel (HIP events on the library's stream), per-step
-p50/p99 latency, the separate energy pass's HBM rate, and the CPU oracle timed
-on the host's cores over a bounded sample of the same list.
+form's to 1e-8 relative, else the FP64 form's.  The JSON line also carries:
+
+* the roofline of the dominant kernel (HIP events on the library's stream);
+* per-step p50/p99 latency (step_p50_ms, step_p99_ms) and the harness's own
+  OPS/PERF figures (omp/test.c:216-226) for both forms;
+* the separate energy pass's HBM rate and the synchronous drop-in's
+  PCIe-inclusive rate;
+* a serving sample (48 jobs of the serving workload, Poisson at 40 jobs/s,
+  after one unmeasured warm pass): serving_job_p50_ms / serving_job_p99_ms;
+* the same list as a full-tensor job (distinct slices per occupied index);
+* the CPU oracle timed on the host's cores over the whole list after a
+  warm-up call (cpu_baseline), and from the same run the oracle's E[T], each
+  form's rel_dE_vs_oracle and the FP64 form's largest per-term error.
 """
 import argparse
 import json
@@ -181,18 +190,42 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
 
 
 def cpu_baseline(arrays, trip, flops, seconds):
+    """The oracle body (the reference's operations, one call per triple, MKL
+    DGEMM and the energy loop threaded over all host cores) over the same list,
+    after one untimed warm-up call; per-call wall time summed as in
+    omp/test.c:166-179.  seconds > 0 stops after that much CPU time (a bounded
+    sample), 0 runs the whole list.  Also returns the oracle's terms, summand
+    scales and compensated totals of the triples it ran."""
     import oracle
     oracle.blas_select(0)
     cores = oracle.max_threads()
-    out = oracle.run_list(arrays, 0, 30, 300, trip, max_seconds=seconds)
+    oracle.run_list(arrays, 0, 30, 300, trip[:1])  # warm-up: MKL and OpenMP start-up
+    t0 = time.perf_counter()
+    out = oracle.run_list(arrays, 0, 30, 300, trip, with_abs=True, max_seconds=seconds)
+    wall = time.perf_counter() - t0
     done = int(out["done"])
     secs = float(np.sum(out["secs"]))
+    what = "all %d triples" % done if done == len(trip) else "first %d of the %d triples" % (done, len(trip))
     return {"value": done * flops / secs / 1e12, "unit": "TFLOP/s", "cores": cores,
             "kind": "port",
-            "sample": "first %d of the %d triples through the oracle body (%s DGEMM, %d OpenMP "
-                      "threads), per-call wall time summed as in omp/test.c:166-179"
-                      % (done, len(trip), oracle.blas_name(), cores),
-            "triples_per_s": done / secs}, out["terms"], done
+            "sample": "%s through the oracle body (%s DGEMM, %d OpenMP threads) after one "
+                      "warm-up call, per-call wall time summed as in omp/test.c:166-179"
+                      % (what, oracle.blas_name(), cores),
+            "triples_per_s": done / secs, "seconds": secs, "wall_s": wall}, out
+
+
+def reference_perf(nocc, nvir, ms_per_triple):
+    """The harness's own OPS/PERF figures (omp/test.c:216-226) for an average
+    time per triple: dgemm_flops = 8 nvir^2 (nvir+nocc) (one flop per
+    multiply-add), dgemm_mops = 8 (4 nvir^2 + 2 nvir nocc), tengy_ops = 94 nvir^2;
+    GF/s = (dgemm_flops + tengy_ops) / tavg, GB/s = 8 (dgemm_mops + tengy_ops) / tavg."""
+    dgemm_flops = 8.0 * nvir * nvir * (nvir + nocc)
+    dgemm_mops = 8.0 * (4.0 * nvir * nvir + 2.0 * nvir * nocc)
+    tengy_ops = 94.0 * nvir * nvir
+    tavg = ms_per_triple * 1e-3
+    return {"tavg_s": tavg, "dgemm_flops": dgemm_flops, "dgemm_mops": dgemm_mops,
+            "tengy_ops": tengy_ops, "GF_per_s": 1e-9 * (dgemm_flops + tengy_ops) / tavg,
+            "GB_per_s": 8e-9 * (dgemm_mops + tengy_ops) / tavg}
 
 
 def batched_report(cc, steps, warmup, slices):
@@ -351,7 +384,10 @@ def main():
                          "\"wg_per_cu\": 0}'")
     ap.add_argument("--split-opts", default="", metavar="JSON",
                     help="the same for the split form (may hold \"slices\")")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=0.0,
+                    help="bound the CPU baseline to this many seconds (0: the whole list)")
+    ap.add_argument("--serving-jobs-sample", type=int, default=48,
+                    help="jobs of the default line's serving sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", default="", metavar="RATES",
                     help="only run the serving loop at these Poisson rates (jobs/s, comma list)")
@@ -518,7 +554,12 @@ def main():
                    "nocc": nocc, "nvir": nvir, "kchunk": kchunk, "ntriples": n,
                    "mode": ("ozaki_int8 s=%d" % S) if split_ok else "fp64"},
         "roofline": roof,
-        "job_p50_ms": head["p50_ms"], "job_p99_ms": head["p99_ms"],
+        "step_p50_ms": head["p50_ms"], "step_p99_ms": head["p99_ms"],
+        "reference_perf_convention": {
+            "fp64_form": reference_perf(nocc, nvir, fp["ms_per_step"] / n),
+            "split_form": reference_perf(nocc, nvir, sp["ms_per_step"] / n),
+            "note": "omp/test.c:216-226 figures for the average time per triple of one "
+                    "step (GEMM flops counted once per multiply-add)"},
         "fp64_form": {"value": tf(fp["ms_per_step"]), "ms_per_step": fp["ms_per_step"],
                       "kernel": fp_kernel, "kernel_ms": fp["kernel_ms"],
                       "kernel_tflops": tf(fp["kernel_ms"]),
@@ -540,19 +581,42 @@ def main():
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),
                        "gemm_only_ms": sep["ms_main"]},
     }
-    # the server runs one split form for every job, whatever its shape, so the
-    # sample uses 4 slices (within 1e-8 in E[T] on every job shape of the serving
-    # sweep; 3 slices are not on the smallest jobs), each job checked
-    line["serving"] = serving_report(cc, 40.0, 16, "ozaki_int8", 4, check=True,
-                                     oracle_gflop=0.0 if args.no_cpu else 20.0)
+    # serving sample: one unmeasured pass of the same jobs first (kernels loaded,
+    # pinned and device pools populated), then the measured pass, every job
+    # checked against its FP64 run alone and the small ones against the oracle
+    sjobs = cc.serving_jobs(args.serving_jobs_sample, 0xC0DE0004)
+    serving_report(cc, 40.0, len(sjobs), "ozaki_int8", 4, check=False, jobs=sjobs)
+    srv = serving_report(cc, 40.0, len(sjobs), "ozaki_int8", 4, check=True, jobs=sjobs,
+                         oracle_gflop=0.0 if args.no_cpu else 20.0)
+    srv["warm"] = True
+    line["serving"] = srv
+    line["serving_job_p50_ms"] = srv["p50_ms"]
+    line["serving_job_p99_ms"] = srv["p99_ms"]
+    print("bench: serving %d jobs at 40/s: p50 %.1f ms p99 %.1f ms, %.1f jobs/s" % (
+        len(sjobs), srv["p50_ms"], srv["p99_ms"], srv["jobs_per_s"]), file=sys.stderr, flush=True)
+    line["tensor_job"] = tensor_job_report(cc, max(3, args.steps // 4), 1, args.slices)
     if not args.no_cpu:
-        cb, ref_terms, done = cpu_baseline(arrays, trip, flops, args.cpu_seconds)
+        cb, ref = cpu_baseline(arrays, trip, flops, args.cpu_seconds)
         line["cpu_baseline"] = cb
+        done = int(ref["done"])
+        print("bench: cpu baseline %.4f TFLOP/s over %d triples (%.1f s)" % (
+            cb["value"], done, cb["seconds"]), file=sys.stderr, flush=True)
         got = fp["res"]["terms"][:done]
-        line["parity_sample"] = {
-            "triples": done,
-            "fp64_max_rel_err_terms": float(np.max(np.abs(got - ref_terms) /
-                                                   np.maximum(np.abs(ref_terms), 1e-300)))}
+        scale = np.maximum(np.abs(ref["terms"]), ref["abs4"])
+        par = {"triples": done,
+               "fp64_max_rel_err_terms": float(np.max(np.abs(got - ref["terms"]) / scale)),
+               "term_tolerance": 1e-11}
+        if done == n:
+            e_or = ref["emp4"] + ref["emp5"]
+            par.update({"oracle_emp4": ref["emp4"], "oracle_emp5": ref["emp5"],
+                        "oracle_E": e_or,
+                        "fp64_rel_dE_vs_oracle": abs(e_fp - e_or) / abs(e_or),
+                        "split_rel_dE_vs_oracle": abs(e_sp - e_or) / abs(e_or),
+                        "split_max_rel_err_terms": float(np.max(
+                            np.abs(sp["res"]["terms"] - ref["terms"]) / scale))})
+            line["split_form"]["rel_dE_vs_oracle"] = par["split_rel_dE_vs_oracle"]
+            line["fp64_form"]["rel_dE_vs_oracle"] = par["fp64_rel_dE_vs_oracle"]
+        line["parity_oracle"] = par
     else:
         line["cpu_baseline"] = None
     if args.copy_calibration > 0:

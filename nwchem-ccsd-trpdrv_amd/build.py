@@ -28,6 +28,9 @@ FLAGS = [
     # registers between the two files on every K step.
     "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-I" + os.path.join(ROOT, "include"),
+    # roctx ranges around job runs, drop-in calls and serving ticks
+    # (rocprofv3 --marker-trace)
+    "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx",
 ]
 
 

@@ -2,6 +2,7 @@
 // include/ccsdt.h (drop-ins with the reference's argument lists, the job API,
 // error reporting and the seeded input generator).
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <atomic>
@@ -23,6 +24,21 @@
 #include "ccsdt_internal.h"
 
 using namespace ccsdt;
+
+// ------------------------------------------------------------------ trace ranges
+namespace {
+// A roctx range over a host-side phase of the library (a job run, a drop-in
+// call, an autotuning, a serving admission), seen by rocprofv3 --marker-trace:
+// the counterpart of the VTune resume/pause the reference harness puts around
+// its timed loop (omp/test.c:156-158,200-202).  Without a profiler attached
+// roctx calls return at once.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
+}  // namespace
 
 // ------------------------------------------------------------------ errors
 namespace {
@@ -1008,6 +1024,7 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
 int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             ccsdt_run_result *res, double *terms, double *const *fouts)
 {
+    TraceRange trace("ccsdt_job_run");
     if (!j) return set_error(CCSDT_E_STATE, "null job");
     if (n < 1 || !trip) return set_error(CCSDT_E_ARG, "empty triple list");
     if (fouts && n != 1) return set_error(CCSDT_E_ARG, "f output needs a single triple");
@@ -1221,6 +1238,7 @@ constexpr double kTuneTie = 0.01;
 int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *base, int reps,
                  ccsdt_run_opts *best, ccsdt_tune_entry *table, int cap, int *count)
 {
+    TraceRange trace("ccsdt_job_autotune");
     if (!j) return set_error(CCSDT_E_STATE, "null job");
     if (!best) return set_error(CCSDT_E_ARG, "null output options");
     if (reps < 1 || reps > 100) return set_error(CCSDT_E_ARG, "reps %d outside 1..100", reps);
@@ -1382,6 +1400,7 @@ void body_impl(const char *who, double *f1n, double *f1t, double *f2n, double *f
                double *jkj, double *kij, double *kkj, double *dintc1, double *dintx1,
                double *t1v1, double *dintc2, double *dintx2, double *t1v2)
 {
+    TraceRange trace(who);
     int rc = CCSDT_OK;
     double terms[4] = {0, 0, 0, 0};
     if (!ncor_ || !nocc_ || !nvir_ || !a_ || !i_ || !j_ || !k_ || !klo_ || !emp4_ || !emp5_ ||
@@ -1441,6 +1460,7 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
                 double eaijk, double *emp4i, double *emp5i, double *emp4k, double *emp5k, int ncor,
                 int nocc, int nvir)
 {
+    TraceRange trace(who);
     int rc = CCSDT_OK;
     double terms[4] = {0, 0, 0, 0};
     const double *fs[8] = {f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t};
@@ -1547,6 +1567,7 @@ struct TickBuf {
     WorkHdr *h_hdr = nullptr, *d_hdr = nullptr;
     hipEvent_t done = nullptr;
     bool inflight = false;
+    roctx_range_id_t trace = 0;  // roctx range from launch to retirement
     std::vector<std::pair<int64_t, int64_t>> spans;  // (job id, tasks) of the tick
 };
 constexpr int kMaxSlots = 512;
@@ -1598,6 +1619,7 @@ int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, in
 // Uploads a job, packs its operands and publishes it in the job table.
 int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
 {
+    TraceRange trace("ccsdt_server_admit");
     ccsdt_job_desc d = sj->desc;
     for (int q = 0; q < CCSDT_NARRAYS; ++q) d.arrays[q] = arrays[q];
     int rc = job_create(&d, &sj->job, false, sj->kind);
@@ -1752,6 +1774,7 @@ int server_launch_tick(ccsdt_server s, bool *launched)
                              false, s->st));
     HIP_TRY(launch_reduce_segments(s->d_jobs, T.d_seg, T.d_hdr, (int)tasks, s->st));
     HIP_TRY(hipEventRecord(T.done, s->st));
+    T.trace = roctxRangeStartA("ccsdt_server_tick");
     T.inflight = true;
     s->order.push_back(tb);
     *launched = true;
@@ -1784,6 +1807,7 @@ int server_retire(ccsdt_server s, bool block)
                 s->running.erase(std::remove(s->running.begin(), s->running.end(), sj->id), s->running.end());
             }
         }
+        roctxRangeStop(T.trace);
         T.inflight = false;
         s->order.pop_front();
     }

@@ -16,6 +16,7 @@ keeps checksums of them.
 """
 import ctypes
 import ctypes.util
+import hashlib
 import os
 import sys
 
@@ -43,10 +44,15 @@ def omp_threads(n):
 
 
 def checksums(arrays):
+    """Plain and position-weighted sums of each input, and a SHA-256 of its
+    bytes (which pins every bit of the regenerated array)."""
     names = sorted(arrays)
-    return names, np.array([[float(np.sum(arrays[n])),
-                             float(np.sum(arrays[n] * np.arange(arrays[n].size) % 7))]
-                            for n in names])
+    sums = np.array([[float(np.sum(arrays[n])),
+                      float(np.sum(arrays[n] * (np.arange(arrays[n].size) % 7)))]
+                     for n in names])
+    digests = np.array([hashlib.sha256(np.ascontiguousarray(arrays[n], dtype="<f8").tobytes())
+                        .hexdigest() for n in names])
+    return names, sums, digests
 
 
 def reference_terms(arrays, nocc, nvir, triples, Lt, Lb):
@@ -73,10 +79,10 @@ def main():
     omp_threads(1)
 
     arrays, blocks, trip, kchunk = cc.config_case(0)
-    names, sums = checksums(arrays)
+    names, sums, digests = checksums(arrays)
     terms, emp, keep = reference_terms(arrays, 8, 40, trip, Lt, Lb)
     np.savez_compressed(os.path.join(HERE, "config0.npz"), triples=trip, terms=terms, emp=emp,
-                        f7=keep[7], names=np.array(names), sums=sums,
+                        f7=keep[7], names=np.array(names), sums=sums, digests=digests,
                         provider=np.array(oracle.blas_name()))
 
     arrays, blocks, trip, kchunk = cc.config_case(1)
@@ -84,11 +90,11 @@ def main():
     for klo, _ in blocks:
         pick += [q for q in range(len(trip)) if trip[q, 4] == klo][:6]
     sub = trip[pick]
-    names, sums = checksums(arrays)
+    names, sums, digests = checksums(arrays)
     omp_threads(8)
     terms, emp, _ = reference_terms(arrays, 30, 300, sub, Lt, Lb)
     np.savez_compressed(os.path.join(HERE, "config1_head.npz"), triples=sub, index=np.array(pick),
-                        terms=terms, names=np.array(names), sums=sums,
+                        terms=terms, names=np.array(names), sums=sums, digests=digests,
                         provider=np.array(oracle.blas_name()))
     print("wrote config0.npz, config1_head.npz with", oracle.blas_name())
 

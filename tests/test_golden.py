@@ -1,6 +1,7 @@
 """The committed golden fixtures (terms computed by the reference's own tengy
 and body glue, see tests/golden/make_golden.py) against the regenerated inputs
 and the oracle."""
+import hashlib
 import os
 
 import numpy as np
@@ -14,14 +15,33 @@ def load(name):
 
 def checksums(arrays, names):
     return np.array([[float(np.sum(arrays[n])),
-                      float(np.sum(arrays[n] * np.arange(arrays[n].size) % 7))] for n in names])
+                      float(np.sum(arrays[n] * (np.arange(arrays[n].size) % 7)))] for n in names])
+
+
+def digests(arrays, names):
+    return [hashlib.sha256(np.ascontiguousarray(arrays[n], dtype="<f8").tobytes()).hexdigest()
+            for n in names]
 
 
 def test_inputs_regenerate(cc):
     for cfg, fname in [(0, "config0.npz"), (1, "config1_head.npz")]:
         g = load(fname)
         arrays = cc.config_case(cfg)[0]
-        assert np.array_equal(checksums(arrays, list(g["names"])), g["sums"])
+        names = list(g["names"])
+        assert np.array_equal(checksums(arrays, names), g["sums"])
+        # every bit of every regenerated input
+        assert digests(arrays, names) == list(g["digests"])
+
+
+def test_checksum_sees_a_one_ulp_change(cc):
+    """The fixture's pins catch a change of one input element by one ulp."""
+    g = load("config0.npz")
+    arrays = dict(cc.config_case(0)[0])
+    names = list(g["names"])
+    a = arrays["tkj"].copy()
+    a[77] = np.nextafter(a[77], np.inf)
+    arrays["tkj"] = a
+    assert digests(arrays, names) != list(g["digests"])
 
 
 def test_oracle_matches_config0(cc, orc):

@@ -118,3 +118,61 @@ def test_arrivals_while_running(cc):
         with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
             a = job.run(jb["triples"])
         assert (a["emp4"], a["emp5"]) == (r["emp4"], r["emp5"])
+
+
+def capped(cc, jb, gflop):
+    """The job with its triple list cut to the first triples whose GEMM work fits
+    in `gflop` (at least one), so the oracle checks it in about a second."""
+    per = cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"])
+    n = max(1, min(len(jb["triples"]), int(gflop * 1e9 // per)))
+    return dict(jb, triples=jb["triples"][:n])
+
+
+def serve_at_arrivals(cc, srv, jobs, arrivals):
+    """Submits each job at its arrival time (seconds after the first call) while
+    driving the server, then drains it; returns the job ids."""
+    ids, nxt = [None] * len(jobs), 0
+    start = srv.clock()
+    while nxt < len(jobs):
+        now = srv.clock() - start
+        while nxt < len(jobs) and arrivals[nxt] <= now:
+            jb = jobs[nxt]
+            ids[nxt] = srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+            nxt += 1
+        srv.progress()
+    srv.drain()
+    return ids
+
+
+@pytest.mark.parametrize("mode,slices,tol", [("fp64", 0, 1e-11), ("ozaki_int8", 4, 1e-8)])
+def test_serving_workload_poisson(cc, orc, mode, slices, tol):
+    """The serving workload at its own shapes: ten jobs with nocc log-uniform in
+    [5, 40] and nvir log-uniform in [50, 400] (serving_jobs' defaults), each job
+    seeded as the 8/40 case and cut to about 12 GFLOP of GEMM work, arriving as
+    a seeded Poisson stream at the server (200 GB budget).  Every job's totals
+    equal its alone run bitwise, and its E[T] is within `tol` of the oracle
+    replaying the reference body's loop over the same triples
+    (omp/test.c:160-176).  Small ticks and a fast stream make the jobs share
+    ticks, which the admission and finish times must show."""
+    jobs = [capped(cc, jb, 12.0) for jb in cc.serving_jobs(10, 0xC0DE0004)]
+    assert min(jb["nocc"] for jb in jobs) >= 5 and max(jb["nocc"] for jb in jobs) <= 40
+    assert min(jb["nvir"] for jb in jobs) >= 50 and max(jb["nvir"] for jb in jobs) <= 400
+    arrivals = cc.poisson_arrivals(len(jobs), 2000.0, 0xC0DE0004)
+    alone = []
+    for jb in jobs:
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            alone.append(job.run(jb["triples"], mode=mode, slices=slices))
+    with cc.Server(mode=mode, slices=slices, items_per_tick=1024) as srv:
+        ids = serve_at_arrivals(cc, srv, jobs, arrivals)
+        mixed = [srv.result(i) for i in ids]
+    overlap = 0
+    for p, a in enumerate(mixed):
+        assert a is not None and a["status"] == 0
+        for b in mixed[p + 1:]:
+            overlap += a["t_admit"] < b["t_finish"] and b["t_admit"] < a["t_finish"]
+    assert overlap >= 1, "no two jobs were resident at the same time"
+    for jb, a, m in zip(jobs, alone, mixed):
+        assert (a["emp4"], a["emp5"]) == (m["emp4"], m["emp5"])
+        ref = orc.run_list(jb["arrays"], 0, jb["nocc"], jb["nvir"], jb["triples"])
+        E = ref["emp4"] + ref["emp5"]
+        assert abs(m["emp4"] + m["emp5"] - E) <= tol * abs(E), (jb["nocc"], jb["nvir"])
