@@ -85,7 +85,8 @@ def kernel_name(opts):
     p = "true" if opts.get("persistent") else "false"
     if opts["mode"] == "ozaki_int8":
         # S = 3 with occupied digit planes is template code 3 + 16 (kSlicesOccDigits)
-        return "k_fused_split<%d,true,false,%s>" % (opts["slices"] + 16 * bool(opts.get("occ_digits")), p)
+        kern = "k_split_rows" if opts.get("variant", 0) == 2 else "k_fused_split"
+        return "%s<%d,true,false,%s>" % (kern, opts["slices"] + 16 * bool(opts.get("occ_digits")), p)
     return ("k_fused" if opts.get("variant", 0) == 0 else "k_fused2") + "<true,false,%s>" % p
 
 
@@ -384,6 +385,9 @@ def main():
                          "\"wg_per_cu\": 0}'")
     ap.add_argument("--split-opts", default="", metavar="JSON",
                     help="the same for the split form (may hold \"slices\")")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="time the two forms only (no drop-in, energy-pass, serving, full-tensor "
+                         "or CPU legs): for profiling runs")
     ap.add_argument("--cpu-seconds", type=float, default=0.0,
                     help="bound the CPU baseline to this many seconds (0: the whole list)")
     ap.add_argument("--serving-jobs-sample", type=int, default=48,
@@ -498,9 +502,13 @@ def main():
               r["kernel_ms"], best), file=sys.stderr, flush=True)
     fp, sp = forms["fp64"], forms["ozaki_int8"]
     S = sp["opts"]["slices"]
-    sep = job.run(trip, mode="fp64_split")  # GEMM writes f to HBM, separate energy pass
+    if args.no_extras:
+        sep = {"ms_tengy": float("nan"), "ms_main": float("nan")}
+        dropin = None
+    else:
+        sep = job.run(trip, mode="fp64_split")  # GEMM writes f to HBM, separate energy pass
+        dropin = dropin_report(cc, arrays, trip, nocc, nvir, flops)
     job.close()
-    dropin = dropin_report(cc, arrays, trip, nocc, nvir, flops)
 
     e_fp = fp["res"]["emp4"] + fp["res"]["emp5"]
     e_sp = sp["res"]["emp4"] + sp["res"]["emp5"]
@@ -581,6 +589,10 @@ def main():
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),
                        "gemm_only_ms": sep["ms_main"]},
     }
+    if args.no_extras:
+        line["cpu_baseline"] = None
+        print(json.dumps(line), flush=True)
+        return
     # serving sample: one unmeasured pass of the same jobs first (kernels loaded,
     # pinned and device pools populated), then the measured pass, every job
     # checked against its FP64 run alone and the small ones against the oracle

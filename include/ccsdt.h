@@ -190,10 +190,16 @@ enum {
                                   hipGraph once per signature and replay it */
 };
 
-/* Tile variants of the FP64 fused kernel (the split form has one). */
+/* Tile variants of the fused kernels.  FP64 modes: ONE_WAVE or TWO_WAVE.  The
+ * split and FP32 modes run their two-wave block-pair kernel for 0 or TWO_WAVE;
+ * Ozaki INT8 with at most 4 slices also runs ROW_PAIR. */
 enum {
     CCSDT_TILE_ONE_WAVE = 0,  /* one wave computes all 16 products of a 16x16 block pair */
-    CCSDT_TILE_TWO_WAVE = 1   /* two waves share the pair, eight products each */
+    CCSDT_TILE_TWO_WAVE = 1,  /* two waves share the pair, eight products each */
+    CCSDT_TILE_ROW_PAIR = 2   /* four waves share two block pairs stacked in one column
+                                 block (16x16 blocks (r, c) and (r+1, c)), one product
+                                 group of four each: fewer operand loads per matrix
+                                 operation.  Results are bitwise those of TWO_WAVE */
 };
 
 /* Order in which the fused kernel's work items (triple x 16x16 block pair) run.
@@ -208,7 +214,7 @@ typedef struct {
     int mode;       /* CCSDT_MODE_* */
     int slices;     /* Ozaki slice count (3..7), ignored otherwise; 0 = 4 */
     int flags;      /* CCSDT_RUN_* */
-    int variant;    /* CCSDT_TILE_* (FP64 modes) */
+    int variant;    /* CCSDT_TILE_* */
     int wg_per_cu;  /* persistent grid: workgroups per CU (0 = as many as fit) */
     int order;      /* CCSDT_ORDER_* (fused modes; the separate-pass mode runs task order) */
     int occ_digits; /* CCSDT_MODE_OZAKI_INT8 with slices 3: 1 runs the K=nocc half as INT8
@@ -330,7 +336,8 @@ typedef struct {
     int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = 65536 */
     int mode;                /* CCSDT_MODE_FP64, CCSDT_MODE_OZAKI_INT8 or CCSDT_MODE_BF16X3 */
     int slices;              /* Ozaki slices (0 = 4) */
-    int variant;             /* CCSDT_TILE_* of the FP64 kernel */
+    int variant;             /* CCSDT_TILE_* (FP64: ONE_WAVE or TWO_WAVE; Ozaki INT8 with
+                                at most 4 slices may take ROW_PAIR) */
     int occ_digits;          /* CCSDT_MODE_OZAKI_INT8 with slices 3: 1 packs every job's
                                 K=nocc half as INT8 digit planes too (as
                                 ccsdt_run_opts.occ_digits); 0 otherwise */

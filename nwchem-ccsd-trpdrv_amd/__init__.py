@@ -49,7 +49,7 @@ MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3, "fp32": 4}
 # library); otherwise that half is an FP64 tail
 TAIL8_MAX_SLICES = 3
 MODE_NAMES = {v: k for k, v in MODES.items()}
-TILE_ONE_WAVE, TILE_TWO_WAVE = 0, 1
+TILE_ONE_WAVE, TILE_TWO_WAVE, TILE_ROW_PAIR = 0, 1, 2
 ORDER_TASK, ORDER_PAIR = 0, 1
 RUN_PERSISTENT, RUN_GRAPH = 1, 2
 E_CODES = {0: "OK", -1: "E_ARG", -2: "E_DEVICE", -3: "E_NOMEM", -4: "E_BUDGET",

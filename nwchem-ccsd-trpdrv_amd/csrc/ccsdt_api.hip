@@ -369,6 +369,8 @@ Geom make_geom(int ncor, int nocc, int nvir)
     g.kp = (int)round_up(g.kdim, 4);
     g.nks = g.kp / 4;
     g.npairs = g.nb * (g.nb + 1) / 2;
+    g.nrpairs = 0;
+    for (int ci = 0; ci < g.nb; ++ci) g.nrpairs += (g.nb - ci + 1) / 2;
     g.vecsz = (int64_t)g.nvp * g.kp;
     return g;
 }
@@ -515,6 +517,7 @@ struct ccsdt_job_s {
     PanelDesc *d_desc = nullptr;
     int npanels = 0;
     short2 *d_pairs = nullptr;
+    int4 *d_rpairs = nullptr;  // row-pair tile items (JobDev::rpairs)
     bool tensors = false;  // full-tensor job (ccsdt_job_create_tensors)
     int kind = JOB_HARNESS;
     Task *d_tasks = nullptr;
@@ -571,6 +574,7 @@ void job_free(ccsdt_job j)
     host_free(j->h_hdr);
     host_free(j->h_terms);
     if (j->d_pairs) dev_free(j->d_pairs);
+    if (j->d_rpairs) dev_free(j->d_rpairs);
     if (j->d_tasks) dev_free(j->d_tasks);
     host_free(j->h_tasks);
     if (j->d_order) dev_free(j->d_order);
@@ -807,6 +811,16 @@ int job_create(const ccsdt_job_desc *desc, ccsdt_job *out, bool staged, int kind
         for (int ci = 0; ci <= bi; ++ci) pairs.push_back(make_short2((short)bi, (short)ci));
     JOB_TRY(dev_alloc((void **)&j->d_pairs, pairs.size() * sizeof(short2)));
     JOB_HIP(hipMemcpy(j->d_pairs, pairs.data(), pairs.size() * sizeof(short2), hipMemcpyHostToDevice));
+    // row-pair tile: per column block ci, row blocks ci, ci+2, ... each with the
+    // next one (absent at the last block), and both blocks' lower-triangle pair
+    // indices (their partials keep the block-pair layout of the other tiles)
+    std::vector<int4> rpairs;
+    auto pidx = [](int bi, int ci) { return bi * (bi + 1) / 2 + ci; };
+    for (int ci = 0; ci < g.nb; ++ci)
+        for (int r0 = ci; r0 < g.nb; r0 += 2)
+            rpairs.push_back(make_int4(r0, ci, pidx(r0, ci), r0 + 1 < g.nb ? pidx(r0 + 1, ci) : -1));
+    JOB_TRY(dev_alloc((void **)&j->d_rpairs, rpairs.size() * sizeof(int4)));
+    JOB_HIP(hipMemcpy(j->d_rpairs, rpairs.data(), rpairs.size() * sizeof(int4), hipMemcpyHostToDevice));
 
     JOB_TRY(job_upload(j, desc->arrays, nullptr));
 #undef JOB_TRY
@@ -940,8 +954,12 @@ int persistent_grid(bool split, int slices, int variant, int wg_per_cu)
     return g < 8 ? 8 : g;
 }
 
+// Work items per task of a fused kernel: block pairs, or row pairs for the
+// row-pair tile of the split form.
+int tile_items(const Geom &g, bool rows) { return rows ? g.nrpairs : g.npairs; }
+
 // Fills the one-job tables of a fused run (job 0, one segment of n tasks).
-void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f, bool pmajor)
+void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f, bool pmajor, bool rows)
 {
     JobDev &jd = *j->h_jobdev;
     jd.g = j->g;
@@ -953,6 +971,7 @@ void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f, bool 
     jd.ev = j->d_ev;
     jd.vec6 = j->d_vec6;
     jd.pairs = j->d_pairs;
+    jd.rpairs = j->d_rpairs;
     Segment &sgm = *j->h_seg;
     sgm.tasks = j->d_tasks;
     sgm.partial = j->d_partial;
@@ -965,7 +984,7 @@ void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f, bool 
     sgm.pmajor = pmajor ? 1 : 0;
     WorkHdr &h = *j->h_hdr;
     memset(&h, 0, sizeof(h));  // also zeroes the persistent item counters
-    h.total = (int)(n * j->g.npairs);
+    h.total = (int)(n * tile_items(j->g, rows));
     h.nsegs = 1;
     h.ntasks = (int)n;
     h.pad = 0;
@@ -996,13 +1015,14 @@ int enqueue_fused(ccsdt_job j, int mode, const SplitGeom &sg, int64_t n, bool pe
     else
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, variant == CCSDT_TILE_TWO_WAVE, j->st));
     if (timed) HIP_TRY(hipEventRecord(E[1], j->st));
-    const int64_t items = n * g.npairs;
+    const bool rows = split && variant == kTileRowPair;
+    const int64_t items = n * tile_items(g, rows);
     const int grid = persistent ? persistent_grid(split || f32, code, variant, wg_per_cu) : (int)items;
     const int64_t vv = (int64_t)g.nvir * g.nvir;
     const bool persist = persistent && !with_f;
     if (split || f32)
         HIP_TRY(launch_fused_split(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
-                                   code, 8 * vv, true, with_f, j->st));
+                                   code, variant, 8 * vv, true, with_f, j->st));
     else
         HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, persist ? grid : (int)items, persist,
                              variant, 8 * vv, true, with_f, j->st));
@@ -1042,9 +1062,23 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     if (occ && slices > kTail8MaxSlices)
         return set_error(CCSDT_E_ARG, "occupied digit planes need slices <= %d (got %d)",
                          kTail8MaxSlices, slices);
-    const int variant = (is_pair_mode(mode) || !opts) ? 0 : opts->variant;
-    if (variant != CCSDT_TILE_ONE_WAVE && variant != CCSDT_TILE_TWO_WAVE)
+    // pair modes: 0 (or 1) the two-wave block-pair kernel, CCSDT_TILE_ROW_PAIR the
+    // row-pair tile (INT8 with at most kRowPairMaxSlices slices)
+    int variant = opts ? opts->variant : 0;
+    if (is_pair_mode(mode)) {
+        if (variant == CCSDT_TILE_TWO_WAVE) variant = 0;
+        if (variant != 0 && variant != CCSDT_TILE_ROW_PAIR)
+            return set_error(CCSDT_E_ARG, "unknown tile variant %d", variant);
+        if (variant == CCSDT_TILE_ROW_PAIR && (mode != CCSDT_MODE_OZAKI_INT8 || slices > kRowPairMaxSlices))
+            return set_error(CCSDT_E_ARG, "the row-pair tile runs Ozaki INT8 with at most %d slices",
+                             kRowPairMaxSlices);
+        if (variant == CCSDT_TILE_ROW_PAIR && (j->g.nvir > kRowPairMaxK || j->g.nocc > kRowPairMaxK))
+            return set_error(CCSDT_E_ARG, "the row-pair tile runs nvir and nocc up to %d (nvir=%d nocc=%d)",
+                             kRowPairMaxK, j->g.nvir, j->g.nocc);
+    } else if (variant != CCSDT_TILE_ONE_WAVE && variant != CCSDT_TILE_TWO_WAVE) {
         return set_error(CCSDT_E_ARG, "unknown tile variant %d", variant);
+    }
+    const bool rows = (variant == CCSDT_TILE_ROW_PAIR);
     const int wg_per_cu = opts ? opts->wg_per_cu : 0;
     if (wg_per_cu < 0 || wg_per_cu > 64)
         return set_error(CCSDT_E_ARG, "workgroups per CU %d outside 0..64", wg_per_cu);
@@ -1094,7 +1128,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     const bool use_graph = (flags & CCSDT_RUN_GRAPH) != 0 && fused && !fouts;
     INJECT_POINT("launch");
     if (fused) {
-        fill_tables(j, sg, n, fouts != nullptr, order == CCSDT_ORDER_PAIR);
+        fill_tables(j, sg, n, fouts != nullptr, order == CCSDT_ORDER_PAIR, rows);
         if (use_graph) {
             if (terms && n > j->cap_hterms) {
                 host_free(j->h_terms);
@@ -1152,7 +1186,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         HIP_TRY(launch_pack(j->d_desc, j->npanels, g, variant == CCSDT_TILE_TWO_WAVE, j->st));
         HIP_TRY(hipEventRecord(E[1], j->st));
         launches = 1;
-        fill_tables(j, sg, n, true, false);
+        fill_tables(j, sg, n, true, false, false);
         for (int64_t t0 = 0; t0 < n; t0 += chunk) {
             const int m = (int)std::min<int64_t>(chunk, n - t0);
             j->h_seg->tasks = j->d_tasks + t0;
@@ -1288,7 +1322,16 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     const int orders[2] = {CCSDT_ORDER_PAIR, CCSDT_ORDER_TASK};
     for (int oi = 0; oi < (fused ? 2 : 1); ++oi) {
         const int ord = fused ? orders[oi] : CCSDT_ORDER_TASK;
-        for (int v = 0; v < (split ? 1 : 2); ++v) {
+        // tiles: FP64 one-wave and two-wave; the split forms their two-wave kernel
+        // and, for INT8 with at most kRowPairMaxSlices slices, the row-pair tile
+        std::vector<int> tiles = {0, 1};
+        if (split) {
+            tiles = {0};
+            if (b0.mode == CCSDT_MODE_OZAKI_INT8 && (b0.slices ? b0.slices : 4) <= kRowPairMaxSlices &&
+                j->g.nvir <= kRowPairMaxK && j->g.nocc <= kRowPairMaxK)
+                tiles.push_back(CCSDT_TILE_ROW_PAIR);
+        }
+        for (int v : tiles) {
             ccsdt_run_opts o = b0;
             o.order = ord;
             o.variant = v;
@@ -1649,6 +1692,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
     jd.ev = j->d_ev;
     jd.vec6 = j->d_vec6;
     jd.pairs = j->d_pairs;
+    jd.rpairs = j->d_rpairs;
     HIP_TRY(hipMemcpyAsync(s->d_jobs + sj->slot, &jd, sizeof(JobDev), hipMemcpyHostToDevice, j->st));
     HIP_TRY(hipEventCreate(&sj->fin));
     HIP_TRY(hipStreamSynchronize(j->st));
@@ -1737,12 +1781,12 @@ int server_launch_tick(ccsdt_server s, bool *launched)
         SJob *sj = s->jobs[s->running[(s->rr + q) % nr]];
         const int64_t rem = sj->n - sj->next;
         if (rem <= 0) continue;
-        const int np = sj->job->g.npairs;
+        const int np = tile_items(sj->job->g, s->o.variant == CCSDT_TILE_ROW_PAIR);
         const int64_t share = std::max<int64_t>(1, budget / (int64_t)nr / np);
         const int64_t m = std::min(rem, share);
         Segment &sg = T.h_seg[nseg++];
         sg.tasks = sj->job->d_tasks + sj->next;
-        sg.partial = sj->job->d_partial + sj->next * np * 4;
+        sg.partial = sj->job->d_partial + sj->next * sj->job->g.npairs * 4;  // by block pair
         sg.fout = nullptr;
         sg.terms = sj->job->d_terms + sj->next * 4;
         sg.job = sj->slot;
@@ -1767,8 +1811,8 @@ int server_launch_tick(ccsdt_server s, bool *launched)
     if (is_split_mode(s->o.mode))
         HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false,
                                    s->o.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
-                                                                  : s->o.slices + s->o.occ_digits * kSlicesOccDigits, 0,
-                                   true, false, s->st));
+                                                                  : s->o.slices + s->o.occ_digits * kSlicesOccDigits,
+                                   s->o.variant, 0, true, false, s->st));
     else
         HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.variant, 0, true,
                              false, s->st));
@@ -1904,6 +1948,8 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
             (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
     }
+    if (s->o.variant == CCSDT_TILE_ROW_PAIR && (desc->nvir > kRowPairMaxK || desc->nocc > kRowPairMaxK))
+        return set_error(CCSDT_E_ARG, "the server's row-pair tile runs nvir and nocc up to %d", kRowPairMaxK);
     const int64_t fp = server_footprint(s->o, desc, n, kind);
     if (fp > s->o.budget_bytes)
         return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
@@ -2064,7 +2110,11 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
     if (o.mode == CCSDT_MODE_OZAKI_INT8) {
         if (o.slices == 0) o.slices = 4;
         if (o.slices < 3 || o.slices > 7) return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
-        o.variant = 0;
+        if (o.variant == CCSDT_TILE_TWO_WAVE) o.variant = 0;
+        if (o.variant != 0 && o.variant != CCSDT_TILE_ROW_PAIR)
+            return set_error(CCSDT_E_ARG, "unknown tile variant %d", o.variant);
+        if (o.variant == CCSDT_TILE_ROW_PAIR && o.slices > kRowPairMaxSlices)
+            return set_error(CCSDT_E_ARG, "the row-pair tile runs at most %d slices", kRowPairMaxSlices);
         if (o.occ_digits != 0 && o.occ_digits != 1)
             return set_error(CCSDT_E_ARG, "occ_digits %d is neither 0 nor 1", o.occ_digits);
         if (o.occ_digits && o.slices > kTail8MaxSlices)
@@ -2077,7 +2127,7 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
         o.slices = 3;
         o.variant = 0;
     }
-    if (o.variant != CCSDT_TILE_ONE_WAVE && o.variant != CCSDT_TILE_TWO_WAVE)
+    if (o.mode == CCSDT_MODE_FP64 && o.variant != CCSDT_TILE_ONE_WAVE && o.variant != CCSDT_TILE_TWO_WAVE)
         return set_error(CCSDT_E_ARG, "unknown tile variant %d", o.variant);
     int rc = ensure_device(-1);
     if (rc) return rc;

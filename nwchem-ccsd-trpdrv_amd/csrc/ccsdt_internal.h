@@ -40,6 +40,7 @@ struct Geom {
     int kp;      // kdim rounded up to 4
     int nks;     // kp / 4 MFMA contraction steps
     int npairs;  // nb*(nb+1)/2 lower-triangle 16x16 block pairs
+    int nrpairs; // items per task of the row-pair tile: sum over ci of ceil((nb - ci) / 2)
     int64_t vecsz;  // doubles in one packed panel: nvp * kp
 };
 
@@ -106,6 +107,8 @@ struct JobDev {
     const double *ev;       // virtual orbital energies, padded to nvp
     const double *vec6;     // vector pool: rows of nvp doubles (Task::vrow)
     const short2 *pairs;    // lower-triangle 16x16 block pairs
+    const int4 *rpairs;     // row-pair tile items: (r0, ci, pair index of (r0, ci), pair
+                            // index of (r0 + 1, ci) or -1 when r0 + 1 = nb)
 };
 
 // A run of consecutive tasks of one job inside a launch.  Work items are
@@ -158,9 +161,11 @@ int tengy_npairs(const Geom &g);
 hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
                              const SplitGeom &sg, int8_t *digits, double *scales, double *tail,
                              hipStream_t st);
+// variant kTileRowPair: the row-pair tile (k_split_rows; INT8 with slices 3, 4 or
+// 3 + kSlicesOccDigits); otherwise the two-wave block-pair kernels.
 hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
-                              int grid, bool persist, int slices, int64_t fstride, bool tengy,
-                              bool writef, hipStream_t st);
+                              int grid, bool persist, int slices, int variant, int64_t fstride,
+                              bool tengy, bool writef, hipStream_t st);
 // Per-task sums of the pair partials over all segments (grid may be persistent).
 hipError_t launch_reduce_segments(const JobDev *jobs, const Segment *segs, const WorkHdr *hdr,
                                   int grid, hipStream_t st);
@@ -172,6 +177,16 @@ constexpr int kSlicesFp32 = 200;
 // pan32 + p * vecsz).
 hipError_t launch_pack32(const PanelDesc *d_desc, int npanels, const Geom &g, float *pan32,
                          hipStream_t st);
+// Tile variant of the split kernels: two row blocks over one column block on four
+// waves (CCSDT_TILE_ROW_PAIR in ccsdt.h).
+constexpr int kTileRowPair = 2;
+// Largest INT8 slice count the row-pair tile runs (its accumulators for two
+// blocks must fit two waves per SIMD).
+constexpr int kRowPairMaxSlices = 4;
+// Largest K (nvir, and nocc) the row-pair tile runs: it combines adjacent digit
+// levels exactly in INT32, (acc_L << 7) + acc_(L+1), and a level sum of K terms is
+// at most 24448 K in magnitude (S <= 4), 129 x 24448 x 768 < 2^31.
+constexpr int kRowPairMaxK = 768;
 // Resident workgroups per CU of the fused kernels (for persistent grids).
 int fused_blocks_per_cu(bool split, int slices, int variant);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,

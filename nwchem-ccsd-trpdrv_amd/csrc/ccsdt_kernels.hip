@@ -1469,6 +1469,470 @@ __global__ __launch_bounds__(128) void k_fused2(const JobDev *__restrict__ jobs,
     pair_items<0, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
 }
 
+// ------------------------------------------------------------------ split form, row-pair tile
+// Work item of the row-pair tile: the 16x16 block pairs (r0, ci) and (r0+1, ci)
+// of one task -- two row blocks over one column block; the second is absent when
+// r0 + 1 = nb -- on four waves, one product group each: wave 0 the n products at
+// (b,c) (LJ, LK x R1, R3), wave 1 the t products (LJt, LKt x R1t, R3t), waves 2
+// and 3 the mirrored ones f(c,b) (R1, R3 x LJ, LK and R1t, R3t x LJt, LKt).  Per
+// 64-wide K step a wave loads the S digit planes of its two A panels at both row
+// blocks and of its two B panels at ci -- 6S 16-byte loads -- for 4 products x 2
+// blocks x S(S+1)/2 digit products: 18 loads per 48 MFMAs at S = 3, against 12 per
+// 24 for the two-wave tile, where the vector-memory path, not the matrix cores,
+// bounds the kernel.  Each wave holds its group's INT32 level accumulators for
+// both blocks (4 S (4 x 2) VGPRs).  The products of both blocks then go to LDS
+// and the energy of block m is evaluated by waves 2m and 2m+1 with the rows and
+// lanes of the two-wave tile.  The f matrices are bitwise those of the two-wave
+// tile (exact integer products, the same recombined values); the energy
+// summands are evaluated in other instruction sequences, so terms agree with
+// that tile's to rounding.
+
+// Block rows of the row-pair items: rpair (r0, ci, pair index of (r0, ci), pair
+// index of (r0+1, ci) or -1), read from the job's table.
+struct RowPair {
+    int r0, ci, p0, p1;
+};
+
+// A wave computes product A_a x B_b into slot a + 2b.  For the direct waves
+// (A = L panels, B = R panels) slot s is f_(s+1) of its group; for the mirrored
+// ones (A = R, B = L) slot a + 2b holds f_(b + 2a + 1) at (c,b), so matrix s
+// sits in slot kMirSlot[s].
+__device__ constexpr int kMirSlot[4] = {0, 2, 1, 3};
+
+__device__ __forceinline__ RowPair read_rowpair(const int4 *p)
+{
+    const int *q = reinterpret_cast<const int *>(p);
+    return RowPair{uniform(cread(q)), uniform(cread(q + 1)), uniform(cread(q + 2)), uniform(cread(q + 3))};
+}
+
+
+// One lane's row r of a digit-product tile: V = sum_L acc_L 2^(7 (S-1-L)), an
+// integer below 2^53 and so exact in FP64, equal to sum_L acc_L w(L) / w(S-1).
+// Adjacent levels are first combined in INT32, (acc_L << 7) + acc_(L+1), exact
+// for at most kIntPairMaxK terms per level sum (the tile's shape limit): one
+// conversion and one FMA fewer per pair than converting every level.
+template <int S>
+__device__ __forceinline__ double level_sum(const int4v (&acc)[S], int r)
+{
+    double u;
+    int q;
+    if constexpr (S % 2 == 1) {
+        u = (double)acc[0][r];
+        q = 1;
+    } else {
+        u = (double)((acc[0][r] << 7) + acc[1][r]);
+        q = 2;
+    }
+#pragma unroll
+    for (; q + 1 < S; q += 2) u = __builtin_fma(u, 16384.0, (double)((acc[q][r] << 7) + acc[q + 1][r]));
+    return u;
+}
+
+// The K = nvir digit products of one group over row blocks 0 and, with TWO,
+// block 1 (the caller picks the instantiation once per item: a branch around the
+// second block's products inside the loop made the compiler copy every
+// accumulator at each join).  The first product of each level starts from zero
+// (no separate clearing of the accumulators).
+template <int S, bool TWO>
+__device__ __forceinline__ void rowpair_nvir(int4v (&iacc)[2][4][S], const __amdgpu_buffer_rsrc_t rd,
+                                             const uint32_t (&oa)[2][2], const uint32_t (&ob)[2],
+                                             uint32_t plane, uint32_t voff8, int nkv)
+{
+    constexpr int NM = TWO ? 2 : 1;
+    const int4v z = {0, 0, 0, 0};
+    int4v a[2][2][S], b[2][S];
+#define RP_LOADS(kstep)                                                                          \
+    _Pragma("unroll") for (int q = 0; q < S; ++q) {                                             \
+        const uint32_t ko_ = (uint32_t)(kstep) * 1024 + q * plane;                               \
+        b[0][q] = buffer_load_i8x16(rd, voff8, ob[0] + ko_);                                     \
+        b[1][q] = buffer_load_i8x16(rd, voff8, ob[1] + ko_);                                     \
+        _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                        \
+            a[m][0][q] = buffer_load_i8x16(rd, voff8, oa[m][0] + ko_);                           \
+            a[m][1][q] = buffer_load_i8x16(rd, voff8, oa[m][1] + ko_);                           \
+        }                                                                                        \
+    }
+#define RP_MFMA(first)                                                                           \
+    _Pragma("unroll") for (int m = 0; m < NM; ++m)                                              \
+    _Pragma("unroll") for (int sa = 0; sa < S; ++sa) {                                          \
+        _Pragma("unroll") for (int sb = 0; sb < S - sa; ++sb) {                                 \
+            const int lv = sa + sb;                                                              \
+            const bool z_ = (first) && sa == 0;                                                  \
+            iacc[m][0][lv] = PlaneMath<false>::mfma(a[m][0][sa], b[0][sb], z_ ? z : iacc[m][0][lv]); \
+            iacc[m][1][lv] = PlaneMath<false>::mfma(a[m][1][sa], b[0][sb], z_ ? z : iacc[m][1][lv]); \
+            iacc[m][2][lv] = PlaneMath<false>::mfma(a[m][0][sa], b[1][sb], z_ ? z : iacc[m][2][lv]); \
+            iacc[m][3][lv] = PlaneMath<false>::mfma(a[m][1][sa], b[1][sb], z_ ? z : iacc[m][3][lv]); \
+        }                                                                                        \
+    }
+    RP_LOADS(0)
+    RP_MFMA(true)
+    for (int ks = 1; ks < nkv; ++ks) {
+        RP_LOADS(ks)
+        RP_MFMA(false)
+    }
+#undef RP_LOADS
+#undef RP_MFMA
+}
+
+// One row-pair item on four waves (see above).  part0/part1: the partials of
+// block pairs (r0, ci) and (r0+1, ci) (part1 unused without the second block).
+template <int S, bool TENGY, bool WRITEF>
+__device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
+                                             const RowPair &rp, int w, int lane,
+                                             double *__restrict__ part0, double *__restrict__ part1,
+                                             double *__restrict__ fo)
+{
+    constexpr bool T8 = (S == 3 + kSlicesOccDigits);
+    constexpr int NP = T8 ? 3 : S;
+    static_assert(NP <= kRowPairMaxSlices, "row-pair tile: at most 4 slices");
+    // LDS: every wave's FP64 products X[w][m][slot][row][lane] (64 KiB), the row
+    // scales of each wave (two sets of 96), the energy vectors of the three
+    // blocks [q][block r0, r0+1, ci][16], and the per-wave energy sums
+    __shared__ double X[4][2][4][4][64];
+    __shared__ __attribute__((aligned(16))) double lds_s[4][2 * 96];
+    __shared__ __attribute__((aligned(16))) double vst[7 * 3 * 16];
+    __shared__ double red[4][4];
+    const bool two = rp.p1 >= 0;
+    const int r0 = rp.r0, ci = rp.ci, r1 = two ? rp.r0 + 1 : rp.r0;
+    const Geom &g = jd.g;
+    const SplitGeom &sg = jd.sg;
+    const int nvp = g.nvp;
+    // wave roles: w & 1 = group (n, t), w >> 1 = mirrored
+    const int LJ = 2 * pp.li, LK = LJ + 1, LJT = 2 * pp.lk, LKT = LJT + 1;
+    const int R1 = 2 * pp.rkj, R3 = R1 + 1, R1T = 2 * pp.rij, R3T = R1T + 1;
+    const int gi = w & 1, mir = w >> 1;
+    const int L0 = gi ? LJT : LJ, L1 = gi ? LKT : LK, Q0 = gi ? R1T : R1, Q1 = gi ? R3T : R3;
+    const int A0 = mir ? Q0 : L0, A1 = mir ? Q1 : L1, B0 = mir ? L0 : Q0, B1 = mir ? L1 : Q1;
+    // row scales: lane l < 48 fetches elements 2l, 2l+1 of the 96: A side [a][m][16],
+    // then B side [b][16]
+    if (lane < 48) {
+        const int e = 2 * lane;
+        int pid, row;
+        if (e < 64) {
+            pid = (e >> 5) ? A1 : A0;
+            row = (((e >> 4) & 1) ? r1 : r0) * 16 + (e & 15);
+        } else {
+            pid = ((e - 64) >> 4) ? B1 : B0;
+            row = ci * 16 + (e & 15);
+        }
+        const int64_t so = (int64_t)pid * nvp + row;
+        *reinterpret_cast<dbl2 *>(&lds_s[w][e]) = gload2(jd.scales + so);
+        if (T8)
+            *reinterpret_cast<dbl2 *>(&lds_s[w][96 + e]) = gload2(jd.scales + (int64_t)sg.npanels * nvp + so);
+    }
+    // energy vectors: 7 x 3 blocks x 16 = 336 doubles, two per thread of waves 0..2
+    if (TENGY && threadIdx.x < 168) {
+        const int e = 2 * threadIdx.x;
+        const int q = e / 48, blk = (e / 16) % 3, i = e & 15;
+        int row = 0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) row = q == r + 1 ? uniform(tk.vrow[r]) : row;
+        const double *src = q == 0 ? jd.ev : jd.vec6 + (int64_t)row * nvp;
+        const int xb = blk == 0 ? r0 : blk == 1 ? r1 : ci;
+        *reinterpret_cast<dbl2 *>(&vst[e]) = gload2(src + xb * 16 + i);
+    }
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
+                                                  (uint32_t)(sg.npanels * NP * sg.slice_bytes));
+    const uint32_t plane = (uint32_t)sg.slice_bytes;
+    const uint32_t blk8 = (uint32_t)(sg.nkv + sg.nkx) * 1024;
+    const uint32_t voff8 = lane * 16;
+    uint32_t oa[2][2], ob[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        oa[m][0] = (uint32_t)A0 * NP * plane + (m ? r1 : r0) * blk8;
+        oa[m][1] = (uint32_t)A1 * NP * plane + (m ? r1 : r0) * blk8;
+    }
+    ob[0] = (uint32_t)B0 * NP * plane + ci * blk8;
+    ob[1] = (uint32_t)B1 * NP * plane + ci * blk8;
+    int4v iacc[2][4][NP];
+    if (two) rowpair_nvir<NP, true>(iacc, rd, oa, ob, plane, voff8, sg.nkv);
+    else rowpair_nvir<NP, false>(iacc, rd, oa, ob, plane, voff8, sg.nkv);
+
+    const double *sst = lds_s[w];
+    double *xw = &X[w][0][0][0][0];  // this wave's [m][slot][row][lane]
+    const int row0 = 4 * (lane >> 4);
+    const int nv = g.nvir;
+    const int cx = ci * 16 + (lane & 15);
+    // 2^(-7(S+1)): weight of the lowest level, applied with the column scale
+    const double wlow = __builtin_ldexp(1.0, -7 * (NP + 1));
+    // products f = V sA(b) (sB(c) wlow) of both blocks into fw (add: onto fw)
+    dbl4 fw[2][4];
+    auto recombine = [&](const int4v (&acc)[2][4][NP], const double *ss, bool add) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int ca = c & 1, cb = c >> 1;
+                const double sB = ss[64 + cb * 16 + (lane & 15)] * wlow;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double f = level_sum<NP>(acc[m][c], r) * ss[(ca * 2 + m) * 16 + row0 + r] * sB;
+                    fw[m][c][r] = add ? fw[m][c][r] + f : f;
+                }
+            }
+    };
+    recombine(iacc, sst, false);
+    if constexpr (T8) {
+        // the K = nocc half as digit planes with scales of their own (sst + 96); the
+        // K = nvir products wait in this wave's LDS slots meanwhile (xw is not
+        // restrict-qualified, and the fence keeps the compiler from forwarding the
+        // stores to the reloads, which would hold them in registers after all)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) xw[((m * 4 + c) * 4 + r) * 64 + lane] = fw[m][c][r];
+        asm volatile("" ::: "memory");
+        const int4v z = {0, 0, 0, 0};
+        if (sg.fold) {
+            // folded slots (k_split_pack, nocc <= 32): A [d1|d2], [d3|0]; B [d1|0],
+            // [d2|d1], [d3|d2]; levels as split_wave's
+            const uint32_t k0 = (uint32_t)sg.nkv * 1024, k1 = k0 + 1024;
+            int4v bx[2][3];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                bx[p][0] = buffer_load_i8x16(rd, voff8, ob[p] + 2 * plane + k0);
+                bx[p][1] = buffer_load_i8x16(rd, voff8, ob[p] + k1);
+                bx[p][2] = buffer_load_i8x16(rd, voff8, ob[p] + plane + k1);
+            }
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int4v ax[2][2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    ax[p][0] = buffer_load_i8x16(rd, voff8, oa[m][p] + k0);
+                    ax[p][1] = buffer_load_i8x16(rd, voff8, oa[m][p] + plane + k0);
+                }
+#pragma unroll
+                for (int lv = 0; lv < 3; ++lv) {
+                    iacc[m][0][lv] = PlaneMath<false>::mfma(ax[0][0], bx[0][lv], z);
+                    iacc[m][1][lv] = PlaneMath<false>::mfma(ax[1][0], bx[0][lv], z);
+                    iacc[m][2][lv] = PlaneMath<false>::mfma(ax[0][0], bx[1][lv], z);
+                    iacc[m][3][lv] = PlaneMath<false>::mfma(ax[1][0], bx[1][lv], z);
+                }
+                iacc[m][0][2] = PlaneMath<false>::mfma(ax[0][1], bx[0][0], iacc[m][0][2]);
+                iacc[m][1][2] = PlaneMath<false>::mfma(ax[1][1], bx[0][0], iacc[m][1][2]);
+                iacc[m][2][2] = PlaneMath<false>::mfma(ax[0][1], bx[1][0], iacc[m][2][2]);
+                iacc[m][3][2] = PlaneMath<false>::mfma(ax[1][1], bx[1][0], iacc[m][3][2]);
+            }
+        } else {
+            // one digit per 64-wide K step (nocc > 32): B planes, then each block's
+            // A panels in turn
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) iacc[m][c][q] = z;
+            for (int kx = 0; kx < sg.nkx; ++kx) {
+                const uint32_t ko = (uint32_t)(sg.nkv + kx) * 1024;
+                int4v bq[2][NP];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    bq[0][q] = buffer_load_i8x16(rd, voff8, ob[0] + ko + q * plane);
+                    bq[1][q] = buffer_load_i8x16(rd, voff8, ob[1] + ko + q * plane);
+                }
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        int4v ax[NP];
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) ax[q] = buffer_load_i8x16(rd, voff8, oa[m][p] + ko + q * plane);
+#pragma unroll
+                        for (int sa = 0; sa < NP; ++sa)
+#pragma unroll
+                            for (int sb = 0; sb < NP - sa; ++sb) {
+                                const int lv = sa + sb;
+                                iacc[m][p][lv] = PlaneMath<false>::mfma(ax[sa], bq[0][sb], iacc[m][p][lv]);
+                                iacc[m][p + 2][lv] = PlaneMath<false>::mfma(ax[sa], bq[1][sb], iacc[m][p + 2][lv]);
+                            }
+                    }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) fw[m][c][r] = xw[((m * 4 + c) * 4 + r) * 64 + lane];
+        recombine(iacc, sst + 96, true);
+    } else {
+        // FP64 tail (K = nocc) in the step-pair layout, A rows fed in the order
+        // 4*(i%4) + i/4 (as split_wave), accumulated onto the recombined products
+        const __amdgpu_buffer_rsrc_t rt = buffer_rsrc(jd.tail, (uint32_t)(sg.npanels * sg.tail_sz * 8));
+        const uint32_t tsb = (uint32_t)(sg.tail_sz * 8);
+        const uint32_t blkt = (uint32_t)sg.nkt * 512;
+        const int li = lane & 15;
+        const uint32_t voffA = (uint32_t)((4 * (li & 3) + (li >> 2)) + (lane & 48)) * 16;
+        const uint32_t voffB = lane * 16;
+        for (int sp = 0; sp < sg.nkt / 2; ++sp) {
+            const dbl2 b2[2] = {buffer_load_f64x2(rt, voffB, (uint32_t)B0 * tsb + ci * blkt + sp * 1024),
+                                buffer_load_f64x2(rt, voffB, (uint32_t)B1 * tsb + ci * blkt + sp * 1024)};
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int bm = m ? r1 : r0;
+                const dbl2 a2[2] = {buffer_load_f64x2(rt, voffA, (uint32_t)A0 * tsb + bm * blkt + sp * 1024),
+                                    buffer_load_f64x2(rt, voffA, (uint32_t)A1 * tsb + bm * blkt + sp * 1024)};
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        fw[m][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[c & 1][h], b2[c >> 1][h], fw[m][c], 0, 0, 0);
+            }
+        }
+    }
+    if (WRITEF) {
+        // body order f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t: slot s of group gi is matrix
+        // 2s + gi (mirrored: 2 kMirSlot[s] + gi at (c,b)); on a diagonal block the
+        // direct waves write every element
+        const int64_t vv = (int64_t)nv * nv;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int bm = m ? r1 : r0;
+            if ((m == 1 && !two) || (mir && bm == ci)) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int bx = bm * 16 + row0 + r;
+                if (bx < nv && cx < nv) {
+                    const int64_t o = mir ? cx + (int64_t)bx * nv : bx + (int64_t)cx * nv;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int sm = mir ? kMirSlot[s] : s;
+                        fo[(2 * sm + gi) * vv + o] = fw[m][s][r];
+                    }
+                }
+            }
+        }
+    }
+    if (TENGY) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[w][m][s][r][lane] = fw[m][s][r];
+        __syncthreads();
+        // block pair m = w >> 1 on waves 2m, 2m+1, rows r = 2 (w & 1) + h of each
+        // lane's four
+        const int m = w >> 1, wo = w & 1;
+        if (m == 0 || two) {
+            const int bm = m ? r1 : r0;
+            const bool diag = (bm == ci);
+            auto vb = [&](int q, int i) { return vst[q * 48 + m * 16 + i]; };
+            auto vc = [&](int q, int i) { return vst[q * 48 + 32 + i]; };
+            const int lc = lane & 15;
+            Sum4 s4 = {0, 0, 0, 0};
+            const double ec = vc(0, lc);
+            const double u1c = vc(1, lc), u2c = vc(2, lc);
+            const double x1c = vc(3, lc), x2c = vc(4, lc);
+            const double y1c = vc(5, lc), y2c = vc(6, lc);
+            const double eaijk = tk.eaijk;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = 2 * wo + h;
+                // dv: n1..n4 (wave 0), t1..t4 (wave 1) at (b,c); mv: the same at (c,b)
+                double dv[8], mv[8];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    dv[s] = X[0][m][s][r][lane];
+                    dv[4 + s] = X[1][m][s][r][lane];
+                    mv[s] = X[2][m][kMirSlot[s]][r][lane];
+                    mv[4 + s] = X[3][m][kMirSlot[s]][r][lane];
+                }
+                const int bx = bm * 16 + row0 + r;
+                if (bx < nv && cx < nv) {
+                    const int lb = row0 + r;
+                    const double dd = -1.0 / (vb(0, lb) + ec + eaijk);
+                    tengy_point<double>(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
+                                        mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
+                                        dd, vb(1, lb), vb(2, lb), x1c, x2c, y1c, y2c, s4);
+                    if (!diag)
+                        tengy_point<double>(mv[0], mv[1], mv[2], mv[3], mv[4], mv[5], mv[6], mv[7],
+                                            dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
+                                            dd, u1c, u2c, vb(3, lb), vb(4, lb), vb(5, lb), vb(6, lb), s4);
+                }
+            }
+            s4.e4i = wave_sum(s4.e4i);
+            s4.e5i = wave_sum(s4.e5i);
+            s4.e4k = wave_sum(s4.e4k);
+            s4.e5k = wave_sum(s4.e5k);
+            if (lane == 0) {
+                red[w][0] = s4.e4i; red[w][1] = s4.e5i; red[w][2] = s4.e4k; red[w][3] = s4.e5k;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) part0[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
+        else if (two && threadIdx.x < 8) part1[threadIdx.x - 4] = red[2][threadIdx.x - 4] + red[3][threadIdx.x - 4];
+    }
+    __syncthreads();  // the LDS buffers are reused by the workgroup's next item
+}
+
+template <int S, bool TENGY, bool WRITEF, bool PERSIST>
+__device__ __forceinline__ void rowpair_items(const JobDev *__restrict__ jobs,
+                                              const Segment *__restrict__ segs, WorkHdr *hdr,
+                                              int64_t fstride)
+{
+    const int nsegs = hdr->nsegs;
+    const ItemRange ir = item_range(hdr->total, blockIdx.x);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // rowpair_item ends with a workgroup barrier, which next_item relies on
+    for (int item = PERSIST ? next_item<4>(hdr, ir) : plain_item(ir); item >= 0;
+         item = PERSIST ? next_item<4>(hdr, ir) : -1) {
+        const int si = uniform(find_segment(segs, nsegs, item));
+        const Segment sg = segs[si];
+        const JobDev jd = jobs[uniform(sg.job)];
+        int t, p;
+        decode_item(sg, jd.g.nrpairs, item, t, p);
+        const Task tk = read_task(sg.tasks + t);
+        const RowPair rp = read_rowpair(jd.rpairs + p);
+        int lane_i = lane;
+        if (PERSIST) asm volatile("" : "+v"(lane_i));
+        double *pbase = sg.partial + (int64_t)t * jd.g.npairs * 4;
+        rowpair_item<S, TENGY, WRITEF>(jd, tk, panel_pairs(tk), rp, w, lane_i, pbase + (int64_t)rp.p0 * 4,
+                                       pbase + (int64_t)(rp.p1 >= 0 ? rp.p1 : rp.p0) * 4,
+                                       WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
+    }
+}
+
+// The Ozaki INT8 split form on the row-pair tile: S = 3 or 4, or 3 +
+// kSlicesOccDigits; three waves per SIMD for S = 3 (at most 168 VGPRs), two for
+// S = 4 (its 128 accumulator VGPRs and operands need more).
+#ifndef RP_WAVES_S3
+#define RP_WAVES_S3 2
+#endif
+template <int S, bool TENGY, bool WRITEF, bool PERSIST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 4 ? 2 : RP_WAVES_S3)))
+void k_split_rows(const JobDev *__restrict__ jobs, const Segment *__restrict__ segs, WorkHdr *hdr,
+                  int64_t fstride)
+{
+    rowpair_items<S, TENGY, WRITEF, PERSIST>(jobs, segs, hdr, fstride);
+}
+
+template <int S>
+static hipError_t launch_split_rows_s(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
+                                      int grid, bool persist, int64_t fstride, bool tengy,
+                                      bool writef, hipStream_t st)
+{
+    dim3 g3((unsigned)grid), block(256);
+    if (persist && tengy && !writef)
+        hipLaunchKernelGGL((k_split_rows<S, true, false, true>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (persist)
+        return hipErrorInvalidValue;
+    else if (tengy && writef)
+        hipLaunchKernelGGL((k_split_rows<S, true, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (tengy)
+        hipLaunchKernelGGL((k_split_rows<S, true, false, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else if (writef)
+        hipLaunchKernelGGL((k_split_rows<S, false, true, false>), g3, block, 0, st, jobs, segs, hdr, fstride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 static hipError_t launch_fused2(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
                                 int grid, bool persist, int64_t fstride, bool tengy, bool writef,
                                 hipStream_t st)
@@ -1511,10 +1975,19 @@ static hipError_t launch_split_s(const JobDev *jobs, const Segment *segs, WorkHd
 }
 
 hipError_t launch_fused_split(const JobDev *jobs, const Segment *segs, WorkHdr *hdr,
-                              int grid, bool persist, int slices, int64_t fstride, bool tengy,
-                              bool writef, hipStream_t st)
+                              int grid, bool persist, int slices, int variant, int64_t fstride,
+                              bool tengy, bool writef, hipStream_t st)
 {
     if (grid <= 0) return hipErrorInvalidValue;
+    if (variant == kTileRowPair) {
+        switch (slices) {
+        case 3 + kSlicesOccDigits:
+            return launch_split_rows_s<3 + kSlicesOccDigits>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+        case 3: return launch_split_rows_s<3>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+        case 4: return launch_split_rows_s<4>(jobs, segs, hdr, grid, persist, fstride, tengy, writef, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (slices == kSlicesFp32) {
         dim3 g3((unsigned)grid), block(128);
         if (persist && tengy && !writef)
@@ -1562,7 +2035,15 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
 {
     int n = 0;
     hipError_t e;
-    if (!split && variant == 1) {
+    if (split && variant == kTileRowPair) {
+        switch (slices) {
+        case 3 + kSlicesOccDigits:
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_split_rows<3 + kSlicesOccDigits, true, false, true>, 256, 0);
+            break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_split_rows<3, true, false, true>, 256, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_split_rows<4, true, false, true>, 256, 0); break;
+        }
+    } else if (!split && variant == 1) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused2<true, false, true>, 128, 0);
     } else if (!split) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<true, false, true>, 64, 0);

@@ -122,7 +122,9 @@ def test_autotune_picks_a_working_candidate(cc, mode, slices):
         ref = job.run(trip, mode=mode, slices=slices, terms=True)
         got = job.run(trip, terms=True, **best)
     variants = {row["variant"] for row in table}
-    assert variants == ({0, 1} if mode == "fp64" else {0})
+    # FP64: the one- and two-wave tiles; INT8 with 4 slices: the two-wave and the
+    # row-pair tile (CCSDT_TILE_ROW_PAIR)
+    assert variants == ({0, 1} if mode == "fp64" else {0, 2})
     assert any(row["persistent"] for row in table) and any(not row["persistent"] for row in table)
     assert all(row["status"] == 0 and row["ms"] > 0 for row in table)
     # the pick is within the 1% tie band of the fastest candidate
