@@ -112,6 +112,27 @@ void ccsd_tengy_omp_(const double *f1n, const double *f1t,
                      double *emp4i, double *emp5i, double *emp4k, double *emp5k,
                      const int *ncor, const int *nocc, const int *nvir);
 
+/* ccsd_tengy_omp's energy reduction on device-resident arrays, enqueued on
+ * `stream` (a hipStream_t; null = the default stream), allocating nothing: for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * serving loops that forbid hipMalloc.  No counterpart in the reference.
+ * f1n..f4t (nvir x nvir each, column major), the six vectors (nvir each) and
+ * eorb (ncor+nocc+nvir) are device pointers with ccsd_tengy_omp's meaning; out4
+ * (device, 4 doubles) receives emp4i, emp5i, emp4k, emp5k when the stream gets
+ * there, bitwise what ccsd_tengy_omp returns for the same values.  The per-tile
+ * partials go to a static device array of 66,560 bytes per GPU, held until the
+ * process ends, so nvir is at most 2048 (CCSDT_E_ARG above).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (other streams, other
+ * threads) race.  Returns CCSDT_OK, CCSDT_E_ARG or CCSDT_E_DEVICE. */
+int ccsdt_tengy_device_noalloc(const double *f1n, const double *f1t,
+                               const double *f2n, const double *f2t,
+                               const double *f3n, const double *f3t,
+                               const double *f4n, const double *f4t,
+                               const double *dintc1, const double *dintx1, const double *t1v1,
+                               const double *dintc2, const double *dintx2, const double *t1v2,
+                               const double *eorb, double eaijk, double *out4,
+                               int ncor, int nocc, int nvir, void *stream);
+
 /* ---------------------------------------------------------------- status */
 /* No counterpart in the reference, which has no error reporting
  * (omp/ccsd_trpdrv_omp_cbody.c returns void and validates nothing). */

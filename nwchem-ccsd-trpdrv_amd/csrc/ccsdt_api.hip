@@ -328,6 +328,33 @@ std::mutex g_init_mu;
 int g_device = -1;
 int g_num_cu = 256;
 
+// Device address of the allocation-free entry point's static scratch array,
+// per device (each GPU holds its own copy of the array): fetched once, with the
+// device current, at its initialisation or on that entry point's first call there.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+int scratch_for_current_device(double **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(CCSDT_E_DEVICE, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        double *p = nullptr;
+        const hipError_t e = scratch_address(&p);
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return set_error(CCSDT_E_DEVICE, "scratch address on device %d: %s", dev,
+                             e != hipSuccess ? hipGetErrorString(e) : "null");
+        }
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return CCSDT_OK;
+}
+
 int ensure_device(int want)
 {
     std::lock_guard<std::mutex> lk(g_init_mu);
@@ -352,7 +379,8 @@ int ensure_device(int want)
     HIP_TRY(hipSetDevice(dev));
     g_device = dev;
     g_num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    return CCSDT_OK;
+    double *sp = nullptr;
+    return scratch_for_current_device(&sp);
 }
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
@@ -2473,6 +2501,43 @@ void ccsd_tengy_omp_(const double *f1n, const double *f1t, const double *f2n, co
     }
     tengy_impl("ccsd_tengy_omp_", f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t, dintc1, dintx1, t1v1,
                dintc2, dintx2, t1v2, eorb, *eaijk, emp4i, emp5i, emp4k, emp5k, *ncor, *nocc, *nvir);
+}
+
+int ccsdt_tengy_device_noalloc(const double *f1n, const double *f1t, const double *f2n,
+                               const double *f2t, const double *f3n, const double *f3t,
+                               const double *f4n, const double *f4t, const double *dintc1,
+                               const double *dintx1, const double *t1v1, const double *dintc2,
+                               const double *dintx2, const double *t1v2, const double *eorb,
+                               double eaijk, double *out4, int ncor, int nocc, int nvir,
+                               void *stream)
+{
+    const double *fs[8] = {f1n, f1t, f2n, f2t, f3n, f3t, f4n, f4t};
+    for (int m = 0; m < 8; ++m)
+        if (!fs[m]) return set_error(CCSDT_E_ARG, "null f matrix %d", m);
+    if (!dintc1 || !dintx1 || !t1v1 || !dintc2 || !dintx2 || !t1v2 || !eorb || !out4)
+        return set_error(CCSDT_E_ARG, "null pointer argument");
+    if (ncor < 0 || nocc < 1 || nvir < 1)
+        return set_error(CCSDT_E_ARG, "invalid ncor=%d nocc=%d nvir=%d", ncor, nocc, nvir);
+    if (nvir > kNoAllocMaxNvir)
+        return set_error(CCSDT_E_ARG, "nvir=%d above the allocation-free limit of %d (its scratch "
+                         "holds %d partials)", nvir, kNoAllocMaxNvir, kNoAllocPartials / 4);
+    double *sp = nullptr;
+    int rc = scratch_for_current_device(&sp);
+    if (rc) return rc;
+    TengyDirect in;
+    for (int m = 0; m < 8; ++m) in.f[m] = fs[m];
+    // the energy's vector order: virtual energies, t1v1 t1v2 dintx1 dintx2 dintc1 dintc2
+    const double *v[7] = {eorb + ncor + nocc, t1v1, t1v2, dintx1, dintx2, dintc1, dintc2};
+    for (int q = 0; q < 7; ++q) in.v[q] = v[q];
+    in.eaijk = eaijk;
+    const Geom g = make_geom(ncor, nocc, nvir);
+    const hipError_t e = launch_tengy_direct(in, g, sp, out4, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(CCSDT_E_DEVICE, "allocation-free energy pass: %s", hipGetErrorString(e));
+    }
+    t_err = CCSDT_OK;
+    return CCSDT_OK;
 }
 
 }  // extern "C"

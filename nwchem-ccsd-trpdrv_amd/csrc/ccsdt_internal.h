@@ -157,6 +157,25 @@ hipError_t launch_fused(const JobDev *jobs, const Segment *segs, WorkHdr *hdr, i
 hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
                         int ntasks, const double *ev, const double *vec6, double *partial,
                         hipStream_t st);
+// One triple's energy-pass inputs in the caller's device arrays: f1n, f1t, ...,
+// f4t (nvir x nvir column major each); the virtual orbital energies, then t1v1,
+// t1v2, dintx1, dintx2, dintc1, dintc2 (nvir each, unpadded); eaijk.
+struct TengyDirect {
+    const double *f[8];
+    const double *v[7];
+    double eaijk;
+};
+// Allocation-free energy pass (ccsdt_tengy_device_noalloc): its per-tile
+// partials go to a static device array of kNoAllocPartials doubles, enough for
+// nvir up to kNoAllocMaxNvir (64 x 65 / 2 tile pairs of 32 x 32, four doubles each).
+constexpr int kNoAllocMaxNvir = 2048;
+constexpr int kNoAllocPartials = (kNoAllocMaxNvir / 32) * (kNoAllocMaxNvir / 32 + 1) / 2 * 4;
+// The current device's address of that array (hipGetSymbolAddress).
+hipError_t scratch_address(double **ptr);
+// The pass on `st` into the array at scratch_ptr, then the per-tile sums into
+// out4 (device, 4 doubles: emp4i, emp5i, emp4k, emp5k).
+hipError_t launch_tengy_direct(const TengyDirect &in, const Geom &g, double *scratch_ptr, double *out4,
+                               hipStream_t st);
 int tengy_npairs(const Geom &g);
 hipError_t launch_split_pack(const PanelDesc *d_desc, int npanels, const Geom &g,
                              const SplitGeom &sg, int8_t *digits, double *scales, double *tail,

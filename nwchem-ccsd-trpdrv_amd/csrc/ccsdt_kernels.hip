@@ -2087,11 +2087,18 @@ int tengy_npairs(const Geom &g)
     return nt * (nt + 1) / 2;
 }
 
+// DIRECT: one triple's inputs straight from the caller's device arrays (the
+// allocation-free entry point, TengyDirect): eight separate f matrices, the
+// virtual orbital energies and the six vectors unpadded (elements past nvir read
+// as 0, as the padding of the job's own arrays does), eaijk by value.  Otherwise
+// the job's f workspace, padded vector pool and task records.  Both forms run the
+// same arithmetic, so their results are bitwise the same.
+template <bool DIRECT>
 __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int64_t fstride,
                                                Geom g, const Task *__restrict__ tasks,
                                                int ntasks, const double *__restrict__ ev,
                                                const double *__restrict__ vec6,
-                                               double *__restrict__ partial)
+                                               double *__restrict__ partial, TengyDirect dir)
 {
     __shared__ double tile[2][32][33];
     __shared__ __attribute__((aligned(16))) double vst[7 * 2 * 32];
@@ -2105,7 +2112,18 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     int BI = 0;
     while ((BI + 1) * (BI + 2) / 2 <= p) ++BI;
     const int CI = p - BI * (BI + 1) / 2;
-    const Task tk = read_task(tasks + t);
+    double eaijk;
+    int vrow[6];
+    if constexpr (DIRECT) {
+        eaijk = dir.eaijk;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) vrow[r] = r;
+    } else {
+        const Task tk = read_task(tasks + t);
+        eaijk = tk.eaijk;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) vrow[r] = uniform(tk.vrow[r]);
+    }
     const double *ft = f + (int64_t)t * fstride;
     const int64_t vv = (int64_t)nv * nv;
     // thread (tx, ty): rows 2tx, 2tx+1 of the B tile, columns ty, ty+16 of the C tile
@@ -2122,9 +2140,18 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
             const int x = (side ? CI : BI) * 32 + i;
             int row = 0;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) row = q == r + 1 ? uniform(tk.vrow[r]) : row;
-            const double *src = q == 0 ? ev : vec6 + (int64_t)row * nvp;
-            if (x < nvp) vreg = gload2(src + x);  // nvp is a multiple of 16: x+1 < nvp too
+            for (int r = 0; r < 6; ++r) row = q == r + 1 ? vrow[r] : row;
+            if constexpr (DIRECT) {
+                // unpadded caller arrays (x is even): both, one or neither element
+                const double *src = dir.v[0];
+#pragma unroll
+                for (int r = 1; r < 7; ++r) src = q == r ? dir.v[r] : src;
+                if (x + 1 < nv) vreg = gload2(src + x);
+                else if (x < nv) vreg = dbl2{gld(src + x), 0.0};
+            } else {
+                const double *src = q == 0 ? ev : vec6 + (int64_t)row * nvp;
+                if (x < nvp) vreg = gload2(src + x);  // nvp is a multiple of 16: x+1 < nvp too
+            }
         }
     }
 
@@ -2133,7 +2160,14 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     // transposed below.  Byte offsets inside one matrix; the matrix is selected by
     // the uniform offset m*vv*8.  Out-of-range points load from offset 0 and are
     // masked; a pair whose second row is past nv (odd nv) loads one double.
-    const __amdgpu_buffer_rsrc_t rf = buffer_rsrc(ft, (uint32_t)(8 * vv * 8));
+    // one resource over the eight matrices (matrix m at byte offset m vv 8), or
+    // one per matrix for DIRECT (each at offset 0)
+    __amdgpu_buffer_rsrc_t rf = buffer_rsrc(ft, (uint32_t)(8 * vv * 8));
+    __amdgpu_buffer_rsrc_t rfm[8];
+    if constexpr (DIRECT) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) rfm[m] = buffer_rsrc(dir.f[m], (uint32_t)(vv * 8));
+    }
     uint32_t od[2], om[2];
     bool okd[2], okm[2];
     const int r0 = 2 * tx;
@@ -2147,9 +2181,11 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
         okm[q] = cc < nv && bb < nv;
         om[q] = okm[q] ? (uint32_t)(cc + bb * nv) * 8 : 0;
     }
-    auto load_pair = [&](uint32_t off, uint32_t moff, bool two) -> dbl2 {
-        if (two) return buffer_load_f64x2(rf, off, moff);
-        return dbl2{buffer_load_f64(rf, off, moff), 0.0};
+    auto load_pair = [&](int m, uint32_t off, bool two) -> dbl2 {
+        const __amdgpu_buffer_rsrc_t r = DIRECT ? rfm[m] : rf;
+        const uint32_t moff = DIRECT ? 0u : (uint32_t)(m * vv * 8);
+        if (two) return buffer_load_f64x2(r, off, moff);
+        return dbl2{buffer_load_f64(r, off, moff), 0.0};
     };
     // the mirrored tiles first: their transposes can start while the direct
     // tiles are still arriving
@@ -2158,12 +2194,12 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     for (int m = 0; m < 8; ++m)
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-            mv[m][q] = load_pair(om[q], (uint32_t)(m * vv * 8), okm[q] ? two_m : nv > 1);
+            mv[m][q] = load_pair(m, om[q], okm[q] ? two_m : nv > 1);
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-            dv[m][q] = load_pair(od[q], (uint32_t)(m * vv * 8), okd[q] ? two_d : nv > 1);
+            dv[m][q] = load_pair(m, od[q], okd[q] ? two_d : nv > 1);
     if (2 * (int)threadIdx.x < 7 * 64) *reinterpret_cast<dbl2 *>(vst + 2 * threadIdx.x) = vreg;
     // transpose: thread (tx, ty) loaded f(c = CI*32 + 2tx + h, b = BI*32 + ty + 16q)
     // and needs f(c = CI*32 + ty + 16q, b = BI*32 + 2tx + h)
@@ -2190,7 +2226,7 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
         for (int h = 0; h < 2; ++h) {
             const int lb = r0 + h, bx = BI * 32 + lb;
             if (bx < nv && cx < nv) {
-                const double d = -1.0 / (vb(0, lb) + vc(0, lc) + tk.eaijk);
+                const double d = -1.0 / (vb(0, lb) + vc(0, lc) + eaijk);
                 // body order: 0 f1n, 1 f1t, 2 f2n, 3 f2t, 4 f3n, 5 f3t, 6 f4n, 7 f4t
                 tengy_point(dv[0][q][h], dv[2][q][h], dv[4][q][h], dv[6][q][h], dv[1][q][h],
                             dv[3][q][h], dv[5][q][h], dv[7][q][h], mv[0][q][h], mv[2][q][h],
@@ -2228,9 +2264,34 @@ hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const T
     const int64_t total = (int64_t)ntasks * tengy_npairs(g);
     if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
     if ((int64_t)g.nvir * g.nvir * 64 >= (int64_t(1) << 32)) return hipErrorInvalidValue;  // 32-bit offsets
-    hipLaunchKernelGGL(k_tengy, dim3((unsigned)total), dim3(256), 0, st, f, fstride, g, tasks,
-                       ntasks, ev, vec6, partial);
+    hipLaunchKernelGGL(k_tengy<false>, dim3((unsigned)total), dim3(256), 0, st, f, fstride, g, tasks,
+                       ntasks, ev, vec6, partial, TengyDirect{});
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ allocation-free energy pass
+// The per-tile partials of ccsdt_tengy_device_noalloc live in this file-scope
+// array, one copy per device, held until the process ends.  Host code must not
+// pass `scratch` itself to a kernel (in host code the name is a stand-in, not a
+// device address): scratch_address() gives the current device's address, which
+// the library fetches once per device and passes to the kernels.
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+hipError_t scratch_address(double **ptr)
+{
+    return hipGetSymbolAddress(reinterpret_cast<void **>(ptr), HIP_SYMBOL(scratch));
+}
+
+hipError_t launch_tengy_direct(const TengyDirect &in, const Geom &g, double *scratch_ptr, double *out4,
+                               hipStream_t st)
+{
+    const int np = tengy_npairs(g);
+    if (np <= 0 || np * 4 > kNoAllocPartials) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tengy<true>, dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, g,
+                       nullptr, 1, nullptr, nullptr, scratch_ptr, in);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_reduce_terms(scratch_ptr, np, 1, out4, st);
 }
 
 // ------------------------------------------------------------------ reductions
