@@ -38,6 +38,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -156,7 +157,14 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
     occ_digits = int(occ_digits) if mode == "ozaki_int8" else 0
     res, lat, span = run_serving(cc, jobs, arrivals, mode, slices, budget=budget, occ_digits=occ_digits)
     flops = sum(len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) for jb in jobs)
-    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": mode if mode == "fp64" else "%s s=%d%s" % (mode, slices, " occ_digits" if occ_digits else ""),
+    if mode == "fp64":
+        label = mode
+    elif slices == cc.SLICES_PER_JOB:
+        label = "%s per-job slices (%d jobs at s=3 occ_digits, %d at s=4)" % (
+            mode, sum(r["slices"] == 3 for r in res), sum(r["slices"] == 4 for r in res))
+    else:
+        label = "%s s=%d%s" % (mode, slices, " occ_digits" if occ_digits else "")
+    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": label,
            "job_format": "tensors" if tensors else "harness",
            "jobs_per_s": njobs / span, "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
@@ -166,11 +174,16 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
     if check:
         import oracle
         worst, worst_o, nor = 0.0, 0.0, 0
+        per_job = []
         for jb, r in zip(jobs, res):
             with job_alone(cc, jb) as job:
                 a = job.run(jb["triples"], mode="fp64")
             E = a["emp4"] + a["emp5"]
-            worst = max(worst, abs(r["emp4"] + r["emp5"] - E) / abs(E))
+            dE = abs(r["emp4"] + r["emp5"] - E) / abs(E)
+            worst = max(worst, dE)
+            per_job.append({"nocc": jb["nocc"], "nvir": jb["nvir"], "triples": len(jb["triples"]),
+                            "slices": r["slices"], "occ_digits": r["occ_digits"],
+                            "latency_ms": (r["t_finish"] - r["t_submit"]) * 1e3, "rel_dE_vs_fp64": dE})
             if len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) <= oracle_gflop * 1e9:
                 no, nv = jb["nocc"], jb["nvir"]
                 if "tensors" in jb:
@@ -185,6 +198,7 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
                 worst_o = max(worst_o, abs(r["emp4"] + r["emp5"] - Eo) / abs(Eo))
                 nor += 1
         out["max_rel_dE_vs_fp64_alone"] = worst
+        out["per_job"] = per_job
         out["oracle_checked_jobs"] = nor
         out["max_rel_dE_vs_oracle"] = worst_o
     return out
@@ -378,13 +392,18 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--slices", type=int, default=0,
-                    help="Ozaki slices of the split form (0: chosen by the autotuner)")
+                    help="Ozaki slices of the split form (0: chosen by the autotuner; with "
+                         "--serving: 0 = 4 slices, -1 = each job's form from its size)")
     ap.add_argument("--fp64-opts", default="", metavar="JSON",
                     help="run the FP64 form with these run options instead of autotuning, "
                          "e.g. '{\"order\": 1, \"variant\": 0, \"persistent\": true, "
                          "\"wg_per_cu\": 0}'")
     ap.add_argument("--split-opts", default="", metavar="JSON",
                     help="the same for the split form (may hold \"slices\")")
+    ap.add_argument("--tune-cache", default=os.environ.get(
+                        "CCSDT_TUNE_CACHE", os.path.join(tempfile.gettempdir(), "ccsdt_tune_cache.txt")),
+                    help="file of the autotuner's kept picks (read, then appended to); a run "
+                         "after one that tuned the same lists takes no tuning time")
     ap.add_argument("--no-extras", action="store_true",
                     help="time the two forms only (no drop-in, energy-pass, serving, full-tensor "
                          "or CPU legs): for profiling runs")
@@ -429,9 +448,11 @@ def main():
 
     cc = load_pkg()
     cc.init(0)
+    kept = cc.tune_cache_file(args.tune_cache or None)
+    print("bench: tuning cache %s (%d picks kept)" % (args.tune_cache, kept), file=sys.stderr, flush=True)
     if args.serving:
         rates = [float(x) for x in args.serving.split(",")]
-        S = args.slices or 4
+        S = args.slices or 4  # -1: per-job slice counts (CCSDT_SLICES_PER_JOB)
         jobs = cc.serving_jobs(args.serving_jobs, 0xC0DE0004, tensors=args.tensors)
         kw = dict(tensors=args.tensors, budget=int(args.budget_gb * 1e9), jobs=jobs)
         occ = dict(occ_digits=int(args.occ_digits))
@@ -491,8 +512,12 @@ def main():
             best.setdefault("slices", slices or (4 if mode == "ozaki_int8" else 0))
             table = []
         else:
+            t0 = time.perf_counter()
             best, table = job.autotune(trip, mode=mode, slices=slices, graph=True, reps=2)
+            best = dict(best)
         r = time_form(job, trip, args.steps, args.warmup, **best)
+        r["tune_s"] = 0.0 if pinned else time.perf_counter() - t0
+        r["tune_cached"] = bool(not pinned and not table)
         eager = dict(best, graph=False)
         kern = [job.run(trip, **eager)["ms_main"] for _ in range(3)]
         r["kernel_ms"] = float(np.median(kern))
@@ -575,9 +600,11 @@ def main():
                       "traffic": traffic_from_profile("fp64", fp["opts"]),
                       "p50_ms": fp["p50_ms"], "p99_ms": fp["p99_ms"],
                       "emp4": fp["res"]["emp4"], "emp5": fp["res"]["emp5"],
-                      "autotuned": fp["opts"], "candidates": cand(fp["table"])},
+                      "autotuned": fp["opts"], "candidates": cand(fp["table"]),
+                      "tune_seconds": fp["tune_s"], "tune_cached": fp["tune_cached"]},
         "split_form": {"slices": S, "kernel": sp_kernel, "autotuned": sp["opts"],
                        "candidates": cand(sp["table"]), "value": tf(sp["ms_per_step"]),
+                       "tune_seconds": sp["tune_s"], "tune_cached": sp["tune_cached"],
                        "ms_per_step": sp["ms_per_step"], "kernel_ms": sp["kernel_ms"],
                        "p50_ms": sp["p50_ms"], "p99_ms": sp["p99_ms"],
                        "emp4": sp["res"]["emp4"], "emp5": sp["res"]["emp5"],

@@ -294,6 +294,21 @@ typedef struct {
 int ccsdt_job_autotune(ccsdt_job job, const int *triples, int64_t n, const ccsdt_run_opts *base,
                        int reps, ccsdt_run_opts *best, ccsdt_tune_entry *table, int cap,
                        int *count);
+/* Tuning cache.  ccsdt_job_autotune keeps every pick, keyed by the request (base's
+ * mode, slices -- 0 meaning "choose" -- occ_digits and graph flag), the job (kind,
+ * ncor, nocc, nvir, kchunk), the triple list (length and a hash of its quintuples)
+ * and the device (architecture, CU count).  For a key it has seen it returns the
+ * kept pick at once, without timing: *count = 0 and table untouched.  Picks are
+ * kept in memory and, when a file is set, appended to it as lines of text.
+ * ccsdt_tune_cache_file sets that file (NULL or "" for memory only) and reads
+ * the picks it holds, returning how many; the environment variable
+ * CCSDT_TUNE_CACHE sets it at the first lookup unless this was called.
+ * ccsdt_tune_cache_clear forgets the in-memory picks (the file is kept);
+ * ccsdt_tune_cache_size counts them.  No counterpart in the reference, which
+ * leaves blocking to its BLAS (omp/ccsd_trpdrv_omp_cbody.c:71-116). */
+int ccsdt_tune_cache_file(const char *path);
+void ccsdt_tune_cache_clear(void);
+int ccsdt_tune_cache_size(void);
 /* Device bytes the job holds (arrays + packed operands + workspace for n triples). */
 int64_t ccsdt_job_footprint(int ncor, int nocc, int nvir, int kchunk, int64_t ntriples);
 
@@ -352,11 +367,18 @@ int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriple
  * are bitwise the same alone and in any mix (fixed per-job reduction order). */
 typedef struct ccsdt_server_s *ccsdt_server;
 
+/* ccsdt_server_opts.slices: choose the split form per job (Ozaki INT8). */
+#define CCSDT_SLICES_PER_JOB (-1)
+
 typedef struct {
     int64_t budget_bytes;    /* HBM budget for admitted jobs (0 = 200 GB) */
     int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = 65536 */
     int mode;                /* CCSDT_MODE_FP64, CCSDT_MODE_OZAKI_INT8 or CCSDT_MODE_BF16X3 */
-    int slices;              /* Ozaki slices (0 = 4) */
+    int slices;              /* Ozaki slices (0 = 4), or CCSDT_SLICES_PER_JOB: each job's own
+                                form, chosen at submit from its size (three slices with
+                                occupied digit planes for a job of at least 1e8 energy
+                                summands, triples x nvir^2, else four; the result reports
+                                it); each tick then runs one launch per form */
     int variant;             /* CCSDT_TILE_* (FP64: ONE_WAVE or TWO_WAVE; Ozaki INT8 with
                                 at most 4 slices may take ROW_PAIR) */
     int occ_digits;          /* CCSDT_MODE_OZAKI_INT8 with slices 3: 1 packs every job's
@@ -370,6 +392,8 @@ typedef struct {
     int64_t ntriples;
     int64_t footprint;       /* device bytes the job held */
     int status;              /* CCSDT_OK or the error that failed the job */
+    int slices;              /* the job's Ozaki slice count (0 outside the INT8 form) */
+    int occ_digits;          /* 1 when its K=nocc half ran as INT8 digit planes */
 } ccsdt_job_result;
 
 int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out);

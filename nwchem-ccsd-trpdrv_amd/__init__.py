@@ -52,6 +52,7 @@ MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE, TILE_ROW_PAIR = 0, 1, 2
 ORDER_TASK, ORDER_PAIR = 0, 1
 RUN_PERSISTENT, RUN_GRAPH = 1, 2
+SLICES_PER_JOB = -1  # server option: each job's split form chosen from its size
 E_CODES = {0: "OK", -1: "E_ARG", -2: "E_DEVICE", -3: "E_NOMEM", -4: "E_BUDGET",
            -5: "E_STATE", -6: "E_INJECTED"}
 
@@ -133,7 +134,8 @@ class JobResult(ctypes.Structure):
     _fields_ = [("emp4", ctypes.c_double), ("emp5", ctypes.c_double),
                 ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
                 ("t_finish", ctypes.c_double), ("ntriples", ctypes.c_int64),
-                ("footprint", ctypes.c_int64), ("status", ctypes.c_int)]
+                ("footprint", ctypes.c_int64), ("status", ctypes.c_int), ("slices", ctypes.c_int),
+                ("occ_digits", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -172,6 +174,9 @@ def lib():
                                                   ctypes.POINTER(RunOpts), ctypes.POINTER(TuneEntry),
                                                   ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
             "ccsdt_job_footprint": (i64, [ctypes.c_int] * 4 + [i64]),
+            "ccsdt_tune_cache_file": (ctypes.c_int, [ctypes.c_char_p]),
+            "ccsdt_tune_cache_clear": (None, []),
+            "ccsdt_tune_cache_size": (ctypes.c_int, []),
             "ccsdt_job_create_tensors": (ctypes.c_int, [ctypes.POINTER(TensorDesc), ctypes.POINTER(vp)]),
             "ccsdt_tensor_job_footprint": (i64, [ctypes.c_int] * 3 + [i64, ctypes.c_int]),
             "ccsdt_server_submit_tensors": (ctypes.c_int, [vp, ctypes.POINTER(TensorDesc), vp, i64,
@@ -517,7 +522,8 @@ class Job:
         triples (ccsdt_job_autotune); with ozaki_int8 and slices 0 it first picks
         the precision (slices, occ_digits).  Returns (best, table): best is a dict
         of run() keyword arguments, table one dict per candidate with its median
-        ms (-1 when it failed) and status."""
+        ms (-1 when it failed) and status -- empty when the pick came from the
+        tuning cache (tune_cache_file)."""
         trip = np.ascontiguousarray(triples, dtype=np.int32).reshape(-1, 5)
         base = RunOpts(MODES[mode], slices, RUN_GRAPH if graph else 0, 0, 0, 0, int(occ_digits))
         best = RunOpts()
@@ -655,6 +661,20 @@ def poisson_arrivals(njobs, rate, seed):
     """Arrival times (seconds) of a Poisson stream of the given rate."""
     u = pcg_uniform(seed, 91, 0.0, 1.0, njobs)
     return np.cumsum(-np.log1p(-u) / rate)
+
+
+def tune_cache_file(path):
+    """Sets the autotuner's cache file (None: memory only) and returns the number
+    of picks read from it (ccsdt_tune_cache_file)."""
+    return lib().ccsdt_tune_cache_file(None if path is None else os.fsencode(path))
+
+
+def tune_cache_clear():
+    lib().ccsdt_tune_cache_clear()
+
+
+def tune_cache_size():
+    return lib().ccsdt_tune_cache_size()
 
 
 def footprint(nocc, nvir, kchunk, ntriples, ncor=0):

@@ -327,6 +327,7 @@ void stream_put(hipStream_t st)
 std::mutex g_init_mu;
 int g_device = -1;
 int g_num_cu = 256;
+char g_arch[64] = "";  // gcnArchName of the initialised device
 
 // Device address of the allocation-free entry point's static scratch array,
 // per device (each GPU holds its own copy of the array): fetched once, with the
@@ -379,6 +380,9 @@ int ensure_device(int want)
     HIP_TRY(hipSetDevice(dev));
     g_device = dev;
     g_num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    snprintf(g_arch, sizeof g_arch, "%s", prop.gcnArchName);
+    for (char *c = g_arch; *c; ++c)
+        if (*c == ' ') *c = '_';
     double *sp = nullptr;
     return scratch_for_current_device(&sp);
 }
@@ -1286,6 +1290,113 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
     return CCSDT_OK;
 }
 
+// ------------------------------------------------------------------ tuning cache
+// The autotuner's picks, keyed by what decides them: the request (mode, slice
+// count or 0 = choose, occupied digit planes, graph flag), the job (kind,
+// ncor, nocc, nvir, kchunk), the triple list (length and an FNV-1a hash of its
+// quintuples) and the device (architecture, CU count).  Kept in memory and,
+// when a file is set (ccsdt_tune_cache_file, or CCSDT_TUNE_CACHE at the first
+// lookup), in that file: one line per pick, "key -> options median_ms",
+// appended as picks are made and read when the file is set (later lines win).
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<ccsdt_run_opts, double>> g_tune;
+std::string g_tune_path;
+bool g_tune_env_read = false;
+
+uint64_t fnv1a(const int *v, int64_t n)
+{
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(v);
+    for (int64_t i = 0; i < n * (int64_t)sizeof(int); ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string tune_key(ccsdt_job j, const ccsdt_run_opts &b, const int *trip, int64_t n)
+{
+    char buf[320];
+    snprintf(buf, sizeof buf, "v1 mode=%d slices=%d occ=%d graph=%d kind=%d ncor=%d nocc=%d nvir=%d "
+             "kchunk=%d n=%lld hash=%016llx dev=%s/%d",
+             b.mode, b.slices, b.occ_digits, (b.flags & CCSDT_RUN_GRAPH) ? 1 : 0, j->kind, j->g.ncor,
+             j->g.nocc, j->g.nvir, j->kchunk, (long long)n, (unsigned long long)fnv1a(trip, 5 * n),
+             g_arch, g_num_cu);
+    return buf;
+}
+
+// Parses one cache line; false for a malformed one.
+bool tune_parse(const char *line, std::string *key, ccsdt_run_opts *o, double *ms)
+{
+    const char *arrow = strstr(line, " -> ");
+    if (!arrow || strncmp(line, "v1 ", 3) != 0) return false;
+    ccsdt_run_opts r{};
+    double m = 0.0;
+    if (sscanf(arrow + 4, "mode=%d slices=%d flags=%d variant=%d wg_per_cu=%d order=%d occ=%d ms=%lf",
+               &r.mode, &r.slices, &r.flags, &r.variant, &r.wg_per_cu, &r.order, &r.occ_digits, &m) != 8)
+        return false;
+    key->assign(line, arrow - line);
+    *o = r;
+    *ms = m;
+    return true;
+}
+
+// Reads the cache file into memory (caller holds g_tune_mu); returns the picks
+// read, or -1 when the file cannot be opened for reading (a new file is not an
+// error: it is created by the first pick).
+int tune_load(const std::string &path)
+{
+    FILE *fp = fopen(path.c_str(), "r");
+    if (!fp) return -1;
+    int n = 0;
+    char line[512];
+    while (fgets(line, sizeof line, fp)) {
+        std::string key;
+        ccsdt_run_opts o;
+        double ms;
+        if (tune_parse(line, &key, &o, &ms)) {
+            g_tune[key] = {o, ms};
+            ++n;
+        }
+    }
+    fclose(fp);
+    return n;
+}
+
+void tune_env_once()
+{
+    if (g_tune_env_read) return;
+    g_tune_env_read = true;
+    const char *e = getenv("CCSDT_TUNE_CACHE");
+    if (e && *e && g_tune_path.empty()) {
+        g_tune_path = e;
+        (void)tune_load(g_tune_path);
+    }
+}
+
+bool tune_lookup(const std::string &key, ccsdt_run_opts *o)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_env_once();
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    *o = it->second.first;
+    return true;
+}
+
+void tune_store(const std::string &key, const ccsdt_run_opts &o, double ms)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_env_once();
+    g_tune[key] = {o, ms};
+    if (g_tune_path.empty()) return;
+    FILE *fp = fopen(g_tune_path.c_str(), "a");
+    if (!fp) return;  // the in-memory pick stands; the file is best effort
+    fprintf(fp, "%s -> mode=%d slices=%d flags=%d variant=%d wg_per_cu=%d order=%d occ=%d ms=%.6f\n",
+            key.c_str(), o.mode, o.slices, o.flags, o.variant, o.wg_per_cu, o.order, o.occ_digits, ms);
+    fclose(fp);
+}
+
 // ------------------------------------------------------------------ autotuner
 // E[T] agreement with the FP64 form the slice-count choice asks for: ten times
 // under the split form's 1e-8 bound, since the tuning list is a sample.
@@ -1304,10 +1415,17 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
     if (!j) return set_error(CCSDT_E_STATE, "null job");
     if (!best) return set_error(CCSDT_E_ARG, "null output options");
     if (reps < 1 || reps > 100) return set_error(CCSDT_E_ARG, "reps %d outside 1..100", reps);
+    if (n < 1 || !trip) return set_error(CCSDT_E_ARG, "empty triple list");
     int rc = ensure_device(-1);
     if (rc) return rc;
     ccsdt_run_opts b0 = base ? *base : ccsdt_run_opts{};
-    const bool split = is_pair_mode(b0.mode);  // one tile: the two-wave kernel of the mode
+    // a pick made before for the same request, job, list and device
+    const std::string key = tune_key(j, b0, trip, n);
+    if (tune_lookup(key, best)) {
+        if (count) *count = 0;
+        return CCSDT_OK;
+    }
+    const bool split = is_pair_mode(b0.mode);
     const bool fused = (b0.mode != CCSDT_MODE_FP64_SPLIT);
     std::vector<ccsdt_run_opts> cand;
     if (b0.mode == CCSDT_MODE_OZAKI_INT8 && b0.slices == 0) {
@@ -1416,6 +1534,7 @@ int job_autotune(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *
             break;
         }
     *best = cand[nbest];
+    tune_store(key, *best, meds[nbest]);
     return CCSDT_OK;
 }
 
@@ -1629,13 +1748,18 @@ struct SJob {
     double t_submit = 0.0, t_admit = 0.0, t_finish = 0.0;
     int state = 0;  // 0 waiting, 4 admitting, 1 running, 2 reducing, 3 finished, -1 failed
     int status = CCSDT_OK;
+    int slices = 0, occ = 0;  // the job's split form (Ozaki INT8: slice count, occupied digit planes)
     hipEvent_t fin = nullptr;
     double emp4 = 0.0, emp5 = 0.0;
 };
 
+// Split forms one tick can run (one launch each): the jobs of a server with
+// per-job slice counts take one of two.
+constexpr int kMaxTickForms = 4;
+
 struct TickBuf {
     Segment *h_seg = nullptr, *d_seg = nullptr;
-    WorkHdr *h_hdr = nullptr, *d_hdr = nullptr;
+    WorkHdr *h_hdr = nullptr, *d_hdr = nullptr;  // one header per form of the tick
     hipEvent_t done = nullptr;
     bool inflight = false;
     roctx_range_id_t trace = 0;  // roctx range from launch to retirement
@@ -1675,13 +1799,34 @@ double server_now(ccsdt_server s)
     return std::chrono::duration<double>(Clock::now() - s->t0).count();
 }
 
-int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n, int kind)
+// A job's split form under a per-job slice policy (CCSDT_SLICES_PER_JOB): three
+// INT8 slices with occupied digit planes when the job sums at least
+// kPerJobS3Points energy summands (triples x nvir^2), else four.  The E[T] error
+// of three slices falls about as one over the square root of that count (3.6e-10
+// at 30/300 with 4.5e8 summands, about 2e-8 on jobs of a few 1e5); from 1e8 on
+// it stays near 1e-9, a tenth of the split form's 1e-8 bound.
+constexpr double kPerJobS3Points = 1e8;
+
+void job_form(const ccsdt_server_opts &o, int nvir, int64_t n, int *slices, int *occ)
+{
+    if (o.mode == CCSDT_MODE_OZAKI_INT8 && o.slices == CCSDT_SLICES_PER_JOB) {
+        const bool big = (double)n * nvir * nvir >= kPerJobS3Points;
+        *slices = big ? 3 : 4;
+        *occ = big ? 1 : 0;
+    } else {
+        *slices = o.slices;
+        *occ = o.occ_digits;
+    }
+}
+
+int64_t server_footprint(const ccsdt_server_opts &o, const ccsdt_job_desc *d, int64_t n, int kind,
+                         int slices, int occ)
 {
     int64_t b = job_footprint(d->ncor, d->nocc, d->nvir, d->kchunk, n, kind);
     if (is_split_mode(o.mode)) {
         const Geom g = make_geom(d->ncor, d->nocc, d->nvir);
         const int npanels = (int)job_npanels(d->nocc, d->kchunk, kind);
-        const SplitGeom sg = make_split_geom(g, o.slices, npanels, o.mode == CCSDT_MODE_BF16X3, o.occ_digits != 0);
+        const SplitGeom sg = make_split_geom(g, slices, npanels, o.mode == CCSDT_MODE_BF16X3, occ != 0);
         b += npanels * (sg.slices * sg.slice_bytes + sg.tail_sz * 8 + split_scale_sets(sg) * (int64_t)g.nvp * 8);
     }
     return b;
@@ -1702,7 +1847,7 @@ int server_admit(ccsdt_server s, SJob *sj, const double *const *arrays)
     if ((rc = grow(&j->d_terms, &j->cap_terms, 4 * sj->n))) return rc;
     SplitGeom sg{};
     if (is_split_mode(s->o.mode)) {
-        sg = make_split_geom(g, s->o.slices, j->npanels, s->o.mode == CCSDT_MODE_BF16X3, s->o.occ_digits != 0);
+        sg = make_split_geom(g, sj->slices, j->npanels, s->o.mode == CCSDT_MODE_BF16X3, sj->occ != 0);
         if ((rc = ensure_split(j, sg))) return rc;
     }
     HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, (size_t)sj->n * sizeof(Task), hipMemcpyHostToDevice, j->st));
@@ -1791,7 +1936,10 @@ void server_admission_loop(ccsdt_server s)
 }
 
 // Builds and launches one tick from a free tick buffer; false when there is
-// nothing to run or no free buffer.
+// nothing to run or no free buffer.  Each running job contributes a round-robin
+// share of its remaining triples; the shares are grouped by the jobs' split form
+// and each form runs as one launch of its kernel (segments of one form
+// contiguous, with a work header of their own).
 int server_launch_tick(ccsdt_server s, bool *launched)
 {
     *launched = false;
@@ -1801,50 +1949,89 @@ int server_launch_tick(ccsdt_server s, bool *launched)
     if (tb < 0 || s->running.empty()) return CCSDT_OK;
     TickBuf &T = s->tick[tb];
     T.spans.clear();
+    const bool rows = s->o.variant == CCSDT_TILE_ROW_PAIR;
     const int64_t budget = s->o.items_per_tick;
     const size_t nr = s->running.size();
-    int64_t items = 0, tasks = 0;
-    int nseg = 0;
-    for (size_t q = 0; q < nr && items < budget && nseg < kMaxSlots; ++q) {
+    struct Share {
+        SJob *sj;
+        int64_t m;
+    };
+    std::vector<Share> shares;
+    int64_t items = 0;
+    for (size_t q = 0; q < nr && items < budget && (int)shares.size() < kMaxSlots; ++q) {
         SJob *sj = s->jobs[s->running[(s->rr + q) % nr]];
         const int64_t rem = sj->n - sj->next;
         if (rem <= 0) continue;
-        const int np = tile_items(sj->job->g, s->o.variant == CCSDT_TILE_ROW_PAIR);
+        const int np = tile_items(sj->job->g, rows);
         const int64_t share = std::max<int64_t>(1, budget / (int64_t)nr / np);
         const int64_t m = std::min(rem, share);
-        Segment &sg = T.h_seg[nseg++];
-        sg.tasks = sj->job->d_tasks + sj->next;
-        sg.partial = sj->job->d_partial + sj->next * sj->job->g.npairs * 4;  // by block pair
-        sg.fout = nullptr;
-        sg.terms = sj->job->d_terms + sj->next * 4;
-        sg.job = sj->slot;
-        sg.item0 = (int)items;
-        sg.ntasks = (int)m;
-        sg.order = nullptr;
-        sg.pmajor = 1;  // block-pair-major over the share: workgroups in flight share panels
+        shares.push_back({sj, m});
         items += m * np;
-        tasks += m;
-        sj->next += m;
-        T.spans.emplace_back(sj->id, m);
     }
     s->rr = (s->rr + 1) % nr;
-    if (nseg == 0) return CCSDT_OK;
-    memset(T.h_hdr, 0, sizeof(WorkHdr));
-    T.h_hdr->total = (int)items;
-    T.h_hdr->nsegs = nseg;
-    T.h_hdr->ntasks = (int)tasks;
-    T.h_hdr->pad = 0;
+    if (shares.empty()) return CCSDT_OK;
+    // the kernel code of each share's job, and the distinct codes in first-seen order
+    auto code_of = [&](const SJob *sj) {
+        if (s->o.mode == CCSDT_MODE_BF16X3) return kSlicesBf16x3;
+        if (s->o.mode == CCSDT_MODE_OZAKI_INT8) return sj->slices + sj->occ * kSlicesOccDigits;
+        return 0;
+    };
+    int codes[kMaxTickForms];
+    int nforms = 0;
+    for (const Share &sh : shares) {
+        const int c = code_of(sh.sj);
+        bool seen = false;
+        for (int f = 0; f < nforms; ++f) seen = seen || codes[f] == c;
+        if (!seen) {
+            if (nforms == kMaxTickForms) return set_error(CCSDT_E_STATE, "more than %d split forms in one tick", kMaxTickForms);
+            codes[nforms++] = c;
+        }
+    }
+    int nseg = 0;
+    int seg0[kMaxTickForms + 1];
+    for (int f = 0; f < nforms; ++f) {
+        seg0[f] = nseg;
+        int64_t fitems = 0, ftasks = 0;
+        for (const Share &sh : shares) {
+            if (code_of(sh.sj) != codes[f]) continue;
+            SJob *sj = sh.sj;
+            const int np = tile_items(sj->job->g, rows);
+            Segment &sg = T.h_seg[nseg++];
+            sg.tasks = sj->job->d_tasks + sj->next;
+            sg.partial = sj->job->d_partial + sj->next * sj->job->g.npairs * 4;  // by block pair
+            sg.fout = nullptr;
+            sg.terms = sj->job->d_terms + sj->next * 4;
+            sg.job = sj->slot;
+            sg.item0 = (int)fitems;
+            sg.ntasks = (int)sh.m;
+            sg.order = nullptr;
+            sg.pmajor = 1;  // block-pair-major over the share: workgroups in flight share panels
+            fitems += sh.m * np;
+            ftasks += sh.m;
+            sj->next += sh.m;
+            T.spans.emplace_back(sj->id, sh.m);
+        }
+        WorkHdr &h = T.h_hdr[f];
+        memset(&h, 0, sizeof(WorkHdr));
+        h.total = (int)fitems;
+        h.nsegs = nseg - seg0[f];
+        h.ntasks = (int)ftasks;
+        h.pad = 0;
+    }
+    seg0[nforms] = nseg;
     HIP_TRY(hipMemcpyAsync(T.d_seg, T.h_seg, (size_t)nseg * sizeof(Segment), hipMemcpyHostToDevice, s->st));
-    HIP_TRY(hipMemcpyAsync(T.d_hdr, T.h_hdr, sizeof(WorkHdr), hipMemcpyHostToDevice, s->st));
-    if (is_split_mode(s->o.mode))
-        HIP_TRY(launch_fused_split(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false,
-                                   s->o.mode == CCSDT_MODE_BF16X3 ? kSlicesBf16x3
-                                                                  : s->o.slices + s->o.occ_digits * kSlicesOccDigits,
-                                   s->o.variant, 0, true, false, s->st));
-    else
-        HIP_TRY(launch_fused(s->d_jobs, T.d_seg, T.d_hdr, (int)items, false, s->o.variant, 0, true,
-                             false, s->st));
-    HIP_TRY(launch_reduce_segments(s->d_jobs, T.d_seg, T.d_hdr, (int)tasks, s->st));
+    HIP_TRY(hipMemcpyAsync(T.d_hdr, T.h_hdr, (size_t)nforms * sizeof(WorkHdr), hipMemcpyHostToDevice, s->st));
+    for (int f = 0; f < nforms; ++f) {
+        const WorkHdr &h = T.h_hdr[f];
+        const Segment *dseg = T.d_seg + seg0[f];
+        WorkHdr *dhdr = T.d_hdr + f;
+        if (is_split_mode(s->o.mode))
+            HIP_TRY(launch_fused_split(s->d_jobs, dseg, dhdr, h.total, false, codes[f], s->o.variant, 0, true,
+                                       false, s->st));
+        else
+            HIP_TRY(launch_fused(s->d_jobs, dseg, dhdr, h.total, false, s->o.variant, 0, true, false, s->st));
+        HIP_TRY(launch_reduce_segments(s->d_jobs, dseg, dhdr, h.ntasks, s->st));
+    }
     HIP_TRY(hipEventRecord(T.done, s->st));
     T.trace = roctxRangeStartA("ccsdt_server_tick");
     T.inflight = true;
@@ -1969,16 +2156,18 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
             q[3] < 1 || q[3] > desc->nocc || q[4] < 1 || q[4] > q[3] || q[3] - q[4] >= desc->kchunk)
             return set_error(CCSDT_E_ARG, "triple %lld outside the job's shape", (long long)t);
     }
+    int slices = 0, occ = 0;
+    job_form(s->o, desc->nvir, n, &slices, &occ);
     if (is_split_mode(s->o.mode)) {
-        const SplitGeom sg = make_split_geom(g, s->o.slices, (int)job_npanels(desc->nocc, desc->kchunk, kind),
-                                             s->o.mode == CCSDT_MODE_BF16X3, s->o.occ_digits != 0);
+        const SplitGeom sg = make_split_geom(g, slices, (int)job_npanels(desc->nocc, desc->kchunk, kind),
+                                             s->o.mode == CCSDT_MODE_BF16X3, occ != 0);
         if ((int64_t)sg.npanels * sg.slices * sg.slice_bytes >= (int64_t(1) << 32) ||
             (int64_t)sg.npanels * sg.tail_sz * 8 >= (int64_t(1) << 32))
             return set_error(CCSDT_E_ARG, "split operands of the job exceed 4 GiB");
     }
     if (s->o.variant == CCSDT_TILE_ROW_PAIR && (desc->nvir > kRowPairMaxK || desc->nocc > kRowPairMaxK))
         return set_error(CCSDT_E_ARG, "the server's row-pair tile runs nvir and nocc up to %d", kRowPairMaxK);
-    const int64_t fp = server_footprint(s->o, desc, n, kind);
+    const int64_t fp = server_footprint(s->o, desc, n, kind, slices, occ);
     if (fp > s->o.budget_bytes)
         return set_error(CCSDT_E_BUDGET, "job footprint %lld B exceeds the budget %lld B",
                          (long long)fp, (long long)s->o.budget_bytes);
@@ -1988,6 +2177,8 @@ static int server_submit(ccsdt_server s, const ccsdt_job_desc *desc, const int *
     sj->kind = kind;
     sj->n = n;
     sj->footprint = fp;
+    sj->slices = slices;
+    sj->occ = occ;
     sj->trip.assign(triples, triples + 5 * n);
     // the arrays are copied before return (outside the server lock); the
     // admission thread uploads them
@@ -2106,6 +2297,28 @@ int ccsdt_job_autotune(ccsdt_job job, const int *triples, int64_t n, const ccsdt
     return rc;
 }
 
+int ccsdt_tune_cache_file(const char *path)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_env_read = true;  // an explicit setting overrides CCSDT_TUNE_CACHE
+    g_tune_path = path ? path : "";
+    if (g_tune_path.empty()) return 0;
+    const int n = tune_load(g_tune_path);
+    return n < 0 ? 0 : n;
+}
+
+void ccsdt_tune_cache_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+int ccsdt_tune_cache_size(void)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return (int)g_tune.size();
+}
+
 int ccsdt_job_destroy(ccsdt_job job)
 {
     if (!job) return set_error(CCSDT_E_STATE, "null job");
@@ -2137,11 +2350,15 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
         return set_error(CCSDT_E_ARG, "server mode %d must be FP64, OZAKI_INT8 or BF16X3", o.mode);
     if (o.mode == CCSDT_MODE_OZAKI_INT8) {
         if (o.slices == 0) o.slices = 4;
-        if (o.slices < 3 || o.slices > 7) return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
+        const bool per_job = (o.slices == CCSDT_SLICES_PER_JOB);
+        if (per_job && o.occ_digits != 0)
+            return set_error(CCSDT_E_ARG, "per-job slice counts choose occupied digit planes themselves");
+        if (!per_job && (o.slices < 3 || o.slices > 7))
+            return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
         if (o.variant == CCSDT_TILE_TWO_WAVE) o.variant = 0;
         if (o.variant != 0 && o.variant != CCSDT_TILE_ROW_PAIR)
             return set_error(CCSDT_E_ARG, "unknown tile variant %d", o.variant);
-        if (o.variant == CCSDT_TILE_ROW_PAIR && o.slices > kRowPairMaxSlices)
+        if (o.variant == CCSDT_TILE_ROW_PAIR && !per_job && o.slices > kRowPairMaxSlices)
             return set_error(CCSDT_E_ARG, "the row-pair tile runs at most %d slices", kRowPairMaxSlices);
         if (o.occ_digits != 0 && o.occ_digits != 1)
             return set_error(CCSDT_E_ARG, "occ_digits %d is neither 0 nor 1", o.occ_digits);
@@ -2173,9 +2390,9 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
     for (int q = 0; q < 2; ++q) {
         TickBuf &T = s->tick[q];
         if ((rc = dev_alloc((void **)&T.d_seg, kMaxSlots * sizeof(Segment)))) return fail(rc);
-        if ((rc = dev_alloc((void **)&T.d_hdr, sizeof(WorkHdr)))) return fail(rc);
+        if ((rc = dev_alloc((void **)&T.d_hdr, kMaxTickForms * sizeof(WorkHdr)))) return fail(rc);
         if (hipHostMalloc((void **)&T.h_seg, kMaxSlots * sizeof(Segment), 0) != hipSuccess ||
-            hipHostMalloc((void **)&T.h_hdr, sizeof(WorkHdr), 0) != hipSuccess ||
+            hipHostMalloc((void **)&T.h_hdr, kMaxTickForms * sizeof(WorkHdr), 0) != hipSuccess ||
             hipEventCreateWithFlags(&T.done, hipEventDisableTiming) != hipSuccess)
             return fail(set_error(CCSDT_E_NOMEM, "tick buffers failed"));
     }
@@ -2245,6 +2462,8 @@ int ccsdt_server_result(ccsdt_server s, int64_t id, ccsdt_job_result *out)
         out->ntriples = sj->n;
         out->footprint = sj->footprint;
         out->status = sj->status;
+        out->slices = is_split_mode(s->o.mode) ? sj->slices : 0;
+        out->occ_digits = sj->occ;
     }
     return 1;
 }

@@ -31,6 +31,15 @@ def cc():
     return load_package()
 
 
+@pytest.fixture(autouse=True)
+def fresh_tuning(cc):
+    """Every test tunes afresh: the autotuner keeps its picks per list and shape
+    for the life of the process (ccsdt_tune_cache_*), so a test that tunes a list
+    an earlier test tuned would otherwise get the kept pick and no timings."""
+    cc.tune_cache_clear()
+    yield
+
+
 @pytest.fixture(scope="session")
 def orc():
     import oracle

@@ -267,3 +267,26 @@ def test_server_options_validated_before_device(cc):
         for mode, slices, occ in (("ozaki_int8", 3, 1), ("fp64", 0, 1)):
             o = cc.ServerOpts(0, 0, cc.MODES[mode], slices, 0, occ)
             assert L.ccsdt_server_create(ctypes.byref(o), ctypes.byref(h)) == -2, (mode, slices, occ)
+
+
+def test_tune_cache_file_round_trip(cc, tmp_path):
+    """The tuning cache's file: well-formed lines are read (later lines win),
+    malformed ones skipped; clearing forgets the in-memory picks only.  Needs no
+    device."""
+    path = tmp_path / "tune.txt"
+    good = ("v1 mode=2 slices=0 occ=0 graph=1 kind=0 ncor=0 nocc=30 nvir=300 kchunk=16 n=4960 "
+            "hash=0123456789abcdef dev=gfx950/256 -> mode=2 slices=3 flags=2 variant=0 "
+            "wg_per_cu=0 order=1 occ=1 ms=10.900000\n")
+    other = good.replace("nvir=300", "nvir=400")
+    path.write_text(good + "this is not a pick\n" + other + good.replace("ms=10.9", "ms=10.7"))
+    try:
+        cc.tune_cache_clear()
+        assert cc.tune_cache_file(str(path)) == 3
+        assert cc.tune_cache_size() == 2  # the repeated key counts once
+        cc.tune_cache_clear()
+        assert cc.tune_cache_size() == 0
+        assert path.read_text().count("\n") == 4  # the file is kept
+        assert cc.tune_cache_file(str(tmp_path / "absent.txt")) == 0
+    finally:
+        cc.tune_cache_file(None)
+        cc.tune_cache_clear()

@@ -196,3 +196,33 @@ def test_batched_production_shape_sampled_oracle(cc, orc):
     assert np.max(np.abs(f["terms"][pick] - ref["terms"]) / scale) < 1e-11
     E = f["emp4"] + f["emp5"]
     assert abs(s["emp4"] + s["emp5"] - E) <= 1e-8 * abs(E)
+
+
+def test_tuning_cache_reuses_picks(cc, tmp_path):
+    """A second tuning of the same request, job and list returns the kept pick
+    at once (no candidates timed), also after the in-memory picks are dropped and
+    read back from the cache file, and that pick runs bitwise as the fresh one.
+    Another list is tuned afresh."""
+    path = tmp_path / "tune.txt"
+    cc.tune_cache_file(str(path))
+    try:
+        arrays, blocks, trip, kchunk = cc.config_case(0)
+        with cc.Job(arrays, 8, 40, kchunk) as job:
+            fresh, table = job.autotune(trip, mode="ozaki_int8", slices=0, reps=1)
+            assert len(table) > 0
+            ref = job.run(trip, terms=True, **fresh)
+            again, table2 = job.autotune(trip, mode="ozaki_int8", slices=0, reps=1)
+            assert table2 == [] and again == fresh
+        assert path.read_text().count(" -> ") == 1
+        cc.tune_cache_clear()
+        assert cc.tune_cache_file(str(path)) == 1
+        with cc.Job(arrays, 8, 40, kchunk) as job:
+            kept, table3 = job.autotune(trip, mode="ozaki_int8", slices=0, reps=1)
+            assert table3 == [] and kept == fresh
+            got = job.run(trip, terms=True, **kept)
+            assert np.array_equal(got["terms"], ref["terms"])
+            assert (got["emp4"], got["emp5"]) == (ref["emp4"], ref["emp5"])
+            _, table4 = job.autotune(trip[:50], mode="ozaki_int8", slices=0, reps=1)
+            assert len(table4) > 0
+    finally:
+        cc.tune_cache_file(None)

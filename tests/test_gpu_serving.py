@@ -176,3 +176,55 @@ def test_serving_workload_poisson(cc, orc, mode, slices, tol):
         ref = orc.run_list(jb["arrays"], 0, jb["nocc"], jb["nvir"], jb["triples"])
         E = ref["emp4"] + ref["emp5"]
         assert abs(m["emp4"] + m["emp5"] - E) <= tol * abs(E), (jb["nocc"], jb["nvir"])
+
+
+def test_per_job_forms_mixed(cc, orc):
+    """CCSDT_SLICES_PER_JOB: a job of at least 1e8 energy summands (triples x
+    nvir^2) runs at three slices with occupied digit planes, smaller ones at four,
+    in the same ticks (one launch per form).  Each job reports its form, equals its
+    plain run in that form bitwise, and is within 1e-8 in E[T] of the oracle
+    replaying the reference loop over its triples (omp/test.c:160-176)."""
+    big = dict(nocc=22, nvir=230, kchunk=22)
+    big["triples"] = cc.triple_list(22, cc.blocks_uniform(22, 22))
+    big["arrays"] = cc.make_inputs(22, 230, 22, 5151)
+    assert len(big["triples"]) * 230 ** 2 >= 1e8
+    jobs = [big] + small_jobs(cc, 4)
+    with cc.Server(mode="ozaki_int8", slices=cc.SLICES_PER_JOB, items_per_tick=4096) as srv:
+        ids = [srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+               for jb in jobs]
+        srv.drain()
+        mixed = [srv.result(i) for i in ids]
+    forms = [(m["slices"], m["occ_digits"]) for m in mixed]
+    assert forms[0] == (3, 1) and all(f == (4, 0) for f in forms[1:])
+    for jb, m in zip(jobs, mixed):
+        assert m["status"] == 0
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            a = job.run(jb["triples"], mode="ozaki_int8", slices=m["slices"], occ_digits=m["occ_digits"])
+        assert (a["emp4"], a["emp5"]) == (m["emp4"], m["emp5"])
+        ref = orc.run_list(jb["arrays"], 0, jb["nocc"], jb["nvir"], jb["triples"])
+        E = ref["emp4"] + ref["emp5"]
+        assert abs(m["emp4"] + m["emp5"] - E) <= 1e-8 * abs(E), (jb["nocc"], jb["nvir"], m["slices"])
+
+
+def test_per_job_forms_serving_workload(cc):
+    """The serving workload's 24 jobs (nocc log-U[5,40], nvir log-U[50,400]) under
+    per-job forms: every job within 1e-8 in E[T] of its FP64 run alone (itself
+    pinned to the oracle by the parity tests), and the three-slice jobs are those
+    the size rule picks."""
+    jobs = cc.serving_jobs(24, 0xC0DE0004)
+    with cc.Server(mode="ozaki_int8", slices=cc.SLICES_PER_JOB) as srv:
+        ids = [srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+               for jb in jobs]
+        srv.drain()
+        res = [srv.result(i) for i in ids]
+    n3 = 0
+    for jb, r in zip(jobs, res):
+        assert r["status"] == 0
+        big = len(jb["triples"]) * jb["nvir"] ** 2 >= 1e8
+        assert (r["slices"], r["occ_digits"]) == ((3, 1) if big else (4, 0))
+        n3 += big
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            f = job.run(jb["triples"], mode="fp64")
+        E = f["emp4"] + f["emp5"]
+        assert abs(r["emp4"] + r["emp5"] - E) <= 1e-8 * abs(E), (jb["nocc"], jb["nvir"], r["slices"])
+    assert 0 < n3 < len(jobs)
