@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libccsdt.so")
+# CCSDT_LIB: another build of the library (the sanitizer builds of sanitize.py)
+LIB_PATH = os.environ.get("CCSDT_LIB") or os.path.join(HERE, "libccsdt.so")
 
 # Job array order = the body's argument order (omp/ccsd_trpdrv_omp_cbody.c:31-44).
 ARRAY_NAMES = ["eorb", "tij", "tkj", "tia", "tka", "xia", "xka", "jia", "jka", "kia",

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# CCSDT_ORACLE_LIB: another build of the oracle (the sanitizer build of
+# nwchem-ccsd-trpdrv_amd/sanitize.py)
+LIB_PATH = os.environ.get("CCSDT_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 ARRAY_NAMES = ["eorb", "tij", "tkj", "tia", "tka", "xia", "xka", "jia", "jka", "kia",
