@@ -50,6 +50,7 @@ METRIC = "(T) FP64-eq TFLOP/s nocc=30 nvir=300, split vs FP64 @rel dE<1e-8; job 
 FP64_MFMA_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 INT8_MFMA_PEAK = 5000.0  # TOP/s, MI355X dense INT8 matrix
 HBM_PEAK = 8000.0        # GB/s
+READ_REF_BYTES = 4 << 30  # size of the streaming-read and copy reference passes
 ET_TOL = 1e-8
 
 
@@ -433,6 +434,10 @@ def main():
                          "inputs (nocc=30, nvir=300)")
     ap.add_argument("--batched-slices", type=int, default=0,
                     help="slices of the batched split form (0: chosen by the autotuner)")
+    ap.add_argument("--tengy-pass", action="store_true",
+                    help="only run the separate energy pass (f written by the FP64 GEMM, then "
+                         "k_tengy) over the nocc=30/nvir=300 list, with a plain device copy of "
+                         "the same bytes as the HBM reference point")
     ap.add_argument("--load-rate", action="store_true",
                     help="only measure the per-instruction cost of wave-wide loads of 4, 8 "
                          "and 16 bytes per lane (and 16-byte global->LDS DMA) on L1-resident "
@@ -480,6 +485,28 @@ def main():
     if args.batched:
         print(json.dumps(batched_report(cc, args.steps, args.warmup, args.batched_slices)),
               flush=True)
+        return
+    if args.tengy_pass:
+        nocc, nvir = 30, 300
+        arrays, blocks, trip, kchunk = cc.config_case(1)
+        nb = cc.tengy_bytes_per_triple(nvir)
+        with cc.Job(arrays, nocc, nvir, kchunk) as job:
+            job.run(trip, mode="fp64_split")
+            runs = [job.run(trip, mode="fp64_split") for _ in range(max(1, args.steps))]
+        ms = float(np.median([r["ms_tengy"] for r in runs]))
+        copy_bytes = READ_REF_BYTES
+        copy_ms = cc.lib().ccsdt_debug_copy(copy_bytes, 5)
+        read_ms = cc.lib().ccsdt_debug_read(copy_bytes, 5)
+        line = {"metric": "separate energy pass (k_tengy) HBM GB/s, nocc=30 nvir=300, 4960 triples",
+                "unit": "GB/s", "kernel": "k_tengy", "ms": ms, "bytes": len(trip) * nb,
+                "GB_per_s": len(trip) * nb / (ms * 1e-3) / 1e9,
+                "hbm_frac": len(trip) * nb / (ms * 1e-3) / 1e9 / HBM_PEAK,
+                "copy_reference": {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "ms": copy_ms,
+                                   "GB_per_s": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9},
+                "read_reference": {"bytes_read": copy_bytes, "ms": read_ms,
+                                   "GB_per_s": copy_bytes / (read_ms * 1e-3) / 1e9},
+                "emp4": runs[-1]["emp4"], "emp5": runs[-1]["emp5"]}
+        print(json.dumps(line), flush=True)
         return
     if args.load_rate:
         clock_ghz = 2.4  # MI355X peak engine clock
@@ -533,6 +560,8 @@ def main():
     else:
         sep = job.run(trip, mode="fp64_split")  # GEMM writes f to HBM, separate energy pass
         dropin = dropin_report(cc, arrays, trip, nocc, nvir, flops)
+        read_ms = cc.lib().ccsdt_debug_read(READ_REF_BYTES, 5)
+        sep["read_GB_per_s"] = READ_REF_BYTES / (read_ms * 1e-3) / 1e9 if read_ms > 0 else None
     job.close()
 
     e_fp = fp["res"]["emp4"] + fp["res"]["emp5"]
@@ -614,7 +643,12 @@ def main():
         "tengy_pass": {"kernel": "k_tengy", "ms": sep["ms_tengy"], "GB_per_s": tengy_gbs,
                        "hbm_frac": tengy_gbs / HBM_PEAK,
                        "bytes_per_triple": cc.tengy_bytes_per_triple(nvir),
-                       "gemm_only_ms": sep["ms_main"]},
+                       "gemm_only_ms": sep["ms_main"],
+                       # a plain streaming read of 4 GiB on the same device: the
+                       # practical ceiling for a kernel that only reads
+                       "read_ceiling_GB_per_s": sep.get("read_GB_per_s"),
+                       "frac_of_read_ceiling": (tengy_gbs / sep["read_GB_per_s"]
+                                                if sep.get("read_GB_per_s") else None)},
     }
     if args.no_extras:
         line["cpu_baseline"] = None

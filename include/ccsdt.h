@@ -436,6 +436,11 @@ int ccsdt_selftest_mfma_i8(const int32_t *a256, const int32_t *b256, int32_t *d,
  * average milliseconds per copy (or -1): the reference point for HBM counters.
  * No counterpart in the reference. */
 double ccsdt_debug_copy(int64_t bytes, int reps);
+/* Streams `bytes` through a plain device read-and-sum kernel (one warm-up launch,
+ * then `reps` timed) and returns the average milliseconds per pass (or -1): the
+ * HBM reference point for kernels that only read.  No counterpart in the
+ * reference. */
+double ccsdt_debug_read(int64_t bytes, int reps);
 /* Nanoseconds per wave-wide load per CU when every CU streams L1-resident data
  * with loads of `bytes` (4, 8 or 16) per lane into VGPRs, or (to_lds = 1, 16
  * bytes) as global->LDS DMA; multiply by the shader clock for cycles.  No

@@ -2641,6 +2641,39 @@ double ccsdt_debug_copy(int64_t bytes, int reps)
     return ms / reps;
 }
 
+double ccsdt_debug_read(int64_t bytes, int reps)
+{
+    if (ensure_device(-1)) return -1.0;
+    const int64_t n = bytes / 16 * 2;
+    if (n <= 0 || reps < 1) {
+        set_error(CCSDT_E_ARG, "read probe: bytes %lld, reps %d", (long long)bytes, reps);
+        return -1.0;
+    }
+    double *src = nullptr, *out = nullptr;
+    if (dev_alloc((void **)&src, n * 8) || dev_alloc((void **)&out, kReadSumBlocks * 8)) {
+        if (src) dev_free(src);
+        return -1.0;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    bool ok = hipMemset(src, 0, n * 8) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess &&
+              launch_readsum(src, out, n, kReadSumBlocks, nullptr) == hipSuccess &&
+              hipEventRecord(e0, nullptr) == hipSuccess;
+    for (int r = 0; r < reps && ok; ++r) ok = launch_readsum(src, out, n, kReadSumBlocks, nullptr) == hipSuccess;
+    ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(src);
+    dev_free(out);
+    if (!ok) {
+        set_error(CCSDT_E_DEVICE, "read probe failed");
+        return -1.0;
+    }
+    return ms / reps;
+}
+
 void ccsd_trpdrv_omp_cbody_(double *f1n, double *f1t, double *f2n, double *f2t, double *f3n,
                             double *f3t, double *f4n, double *f4t, double *eorb, int *ncor,
                             int *nocc, int *nvir, double *emp4, double *emp5, int *a, int *i,

@@ -215,6 +215,9 @@ hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks
 hipError_t launch_mfma_probe(const double *a, const double *b, double *d, hipStream_t st);
 hipError_t launch_mfma_i8_probe(const int *a, const int *b, int *d, int shape, hipStream_t st);
 hipError_t launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
+// Streaming read-and-sum of n doubles into out[blocks] (blocks <= kReadSumBlocks).
+constexpr int kReadSumBlocks = 16384;
+hipError_t launch_readsum(const double *src, double *out, int64_t n, int blocks, hipStream_t st);
 // Load-rate probe (ccsdt_debug_load_rate): grid workgroups of 256 threads, each
 // thread issuing iters*8 loads of `bytes` (4, 8, 16) per lane, to LDS when lds.
 hipError_t launch_load_rate(const uint32_t *buf, uint32_t *out, int bytes, bool lds, int grid,

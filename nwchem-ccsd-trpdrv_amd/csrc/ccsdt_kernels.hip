@@ -2493,6 +2493,47 @@ hipError_t launch_copy(const double *src, double *dst, int64_t n, hipStream_t st
     return hipGetLastError();
 }
 
+// Plain streaming read of n doubles (the read-only HBM reference): block b sums
+// its own contiguous chunk of the array, every thread keeping eight independent
+// 16-byte loads in flight (the fastest of the read shapes tried on MI355X:
+// contiguous chunks beat grid-stride loops, and the rate levels off from about
+// 8192 blocks); the block writes one double, so the writes are negligible.
+__global__ __launch_bounds__(256) void k_readsum(const double2 *__restrict__ src, double *__restrict__ out, int64_t n2)
+{
+    constexpr int U = 8;
+    const int64_t per = (n2 + gridDim.x - 1) / gridDim.x;
+    const int64_t b = (int64_t)blockIdx.x * per;
+    const int64_t e = b + per < n2 ? b + per : n2;
+    double acc[U] = {};
+    int64_t i = b + threadIdx.x;
+    for (; i + (U - 1) * 256 < e; i += U * 256) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * 256];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] += v[u].x + v[u].y;
+    }
+    for (; i < e; i += 256) acc[0] += src[i].x + src[i].y;
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) t += acc[u];
+    __shared__ double red[256];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+hipError_t launch_readsum(const double *src, double *out, int64_t n, int blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_readsum, dim3(blocks), dim3(256), 0, st, reinterpret_cast<const double2 *>(src), out,
+                       n / 2);
+    return hipGetLastError();
+}
+
 // Load-rate probe: every wave streams `iters` x 8 wave-wide loads of W dwords
 // per lane from its workgroup's 16 KiB region of `buf` (64 regions, 1 MiB; the
 // offsets stay below 10 KiB into the region), to VGPRs (LDS = false) or, W = 4
