@@ -656,11 +656,14 @@ def main():
         return
     # serving sample: one unmeasured pass of the same jobs first (kernels loaded,
     # pinned and device pools populated), then the measured pass, every job
-    # checked against its FP64 run alone and the small ones against the oracle
+    # checked against its FP64 run alone and the small ones against the oracle.
+    # Per-job split forms: three slices with occupied digit planes for jobs of at
+    # least 1e8 energy summands, four below.
     sjobs = cc.serving_jobs(args.serving_jobs_sample, 0xC0DE0004)
-    serving_report(cc, 40.0, len(sjobs), "ozaki_int8", 4, check=False, jobs=sjobs)
-    srv = serving_report(cc, 40.0, len(sjobs), "ozaki_int8", 4, check=True, jobs=sjobs,
+    serving_report(cc, 40.0, len(sjobs), "ozaki_int8", cc.SLICES_PER_JOB, check=False, jobs=sjobs)
+    srv = serving_report(cc, 40.0, len(sjobs), "ozaki_int8", cc.SLICES_PER_JOB, check=True, jobs=sjobs,
                          oracle_gflop=0.0 if args.no_cpu else 20.0)
+    srv["within_tol"] = srv["max_rel_dE_vs_fp64_alone"] <= ET_TOL
     srv["warm"] = True
     line["serving"] = srv
     line["serving_job_p50_ms"] = srv["p50_ms"]

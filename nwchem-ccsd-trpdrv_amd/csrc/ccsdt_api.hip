@@ -1801,10 +1801,13 @@ double server_now(ccsdt_server s)
 
 // A job's split form under a per-job slice policy (CCSDT_SLICES_PER_JOB): three
 // INT8 slices with occupied digit planes when the job sums at least
-// kPerJobS3Points energy summands (triples x nvir^2), else four.  The E[T] error
-// of three slices falls about as one over the square root of that count (3.6e-10
-// at 30/300 with 4.5e8 summands, about 2e-8 on jobs of a few 1e5); from 1e8 on
-// it stays near 1e-9, a tenth of the split form's 1e-8 bound.
+// kPerJobS3Points energy summands (triples x nvir^2), else four.  Three slices
+// drop the digit products of the lowest levels, an error that does not average
+// away over summands: on Poisson job mixes (nocc 5..40, nvir 50..400) jobs from
+// 1.2e8 to 1.3e9 summands measured 2.5e-10 to 5.5e-9 in E[T] against the FP64
+// form, while jobs of a few 1e5 summands reached 2.1e-8 at three slices, over
+// the split form's 1e-8 bound.  Four slices
+// measured at most 1.8e-10 on every job of those mixes.
 constexpr double kPerJobS3Points = 1e8;
 
 void job_form(const ccsdt_server_opts &o, int nvir, int64_t n, int *slices, int *occ)
