@@ -108,46 +108,25 @@ def time_form(job, trip, steps, warmup, **kw):
             "p99_ms": float(np.percentile(step_ms, 99)), "res": res}
 
 
-def submit_job(srv, jb):
-    if "tensors" in jb:
-        return srv.submit_tensors(jb["tensors"], jb["nocc"], jb["nvir"], jb["triples"])
-    return srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
-
-
 def job_alone(cc, jb):
     if "tensors" in jb:
         return cc.Job.from_tensors(jb["tensors"], jb["nocc"], jb["nvir"])
     return cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"])
 
 
-def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=65536, budget=0, occ_digits=0):
+def run_serving(cc, jobs, arrivals, mode, slices, items_per_tick=0, budget=0, occ_digits=0,
+                variant=-1):
     """Feeds the jobs to a server at their arrival times (seconds after start)
-    and returns per-job latency and results."""
-    srv = cc.Server(budget_bytes=budget, mode=mode, slices=slices, items_per_tick=items_per_tick,
-                    occ_digits=occ_digits)
-    ids = [None] * len(jobs)
-    nxt = 0
-    start = srv.clock()
-    while True:
-        now = srv.clock() - start
-        while nxt < len(jobs) and arrivals[nxt] <= now:
-            ids[nxt] = submit_job(srv, jobs[nxt])
-            nxt += 1
-        srv.progress()
-        c = srv.counts()
-        if nxt == len(jobs) and c["waiting"] == 0 and c["running"] == 0:
-            break
-        if c["running"] == 0 and nxt < len(jobs):
-            time.sleep(max(0.0, min(arrivals[nxt] - now, 0.002)))
-    res = [srv.result(i) for i in ids]
-    srv.close()
-    lat = np.array([(r["t_finish"] - r["t_submit"]) * 1e3 for r in res])
-    span = max(r["t_finish"] for r in res) - min(r["t_submit"] for r in res)
-    return res, lat, span
+    and returns per-job results, latency (ms), the span (s) and the server's
+    options (tile and tick size as resolved: -1 / 0 take the tuning cache's
+    server pick, else tile 0 and 131072 items)."""
+    return cc.serve(jobs, arrivals, budget_bytes=budget, mode=mode, slices=slices,
+                    items_per_tick=items_per_tick, occ_digits=occ_digits, variant=variant)
 
 
 def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, tensors=False,
-                   budget=0, oracle_gflop=20.0, jobs=None, occ_digits=0):
+                   budget=0, oracle_gflop=20.0, jobs=None, occ_digits=0, variant=-1,
+                   items_per_tick=0):
     """One Poisson serving run.  check: every job's E[T] against the same job
     run alone in the FP64 form on the GPU, and the jobs of at most
     oracle_gflop GEMM work against the CPU oracle (on gathered arrays for
@@ -156,7 +135,9 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
         jobs = cc.serving_jobs(njobs, seed, tensors=tensors)
     arrivals = cc.poisson_arrivals(njobs, rate, seed)
     occ_digits = int(occ_digits) if mode == "ozaki_int8" else 0
-    res, lat, span = run_serving(cc, jobs, arrivals, mode, slices, budget=budget, occ_digits=occ_digits)
+    res, lat, span, sopts = run_serving(cc, jobs, arrivals, mode, slices, budget=budget,
+                                        occ_digits=occ_digits, variant=variant,
+                                        items_per_tick=items_per_tick)
     flops = sum(len(jb["triples"]) * cc.gemm_flops_per_triple(jb["nocc"], jb["nvir"]) for jb in jobs)
     if mode == "fp64":
         label = mode
@@ -165,7 +146,8 @@ def serving_report(cc, rate, njobs, mode, slices, seed=0xC0DE0004, check=True, t
             mode, sum(r["slices"] == 3 for r in res), sum(r["slices"] == 4 for r in res))
     else:
         label = "%s s=%d%s" % (mode, slices, " occ_digits" if occ_digits else "")
-    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": label,
+    out = {"rate_jobs_per_s": rate, "jobs": njobs, "mode": label, "variant": sopts["variant"],
+           "items_per_tick": sopts["items_per_tick"],
            "job_format": "tensors" if tensors else "harness",
            "jobs_per_s": njobs / span, "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
@@ -416,6 +398,18 @@ def main():
     ap.add_argument("--serving", default="", metavar="RATES",
                     help="only run the serving loop at these Poisson rates (jobs/s, comma list)")
     ap.add_argument("--serving-jobs", type=int, default=24)
+    ap.add_argument("--serving-variant", type=int, default=-1,
+                    help="server tile (CCSDT_TILE_*) for --serving; -1: the tuning cache's "
+                         "server pick, else tile 0")
+    ap.add_argument("--items-per-tick", type=int, default=0,
+                    help="work items per serving tick for --serving; 0: the tuning cache's "
+                         "server pick, else 131072")
+    ap.add_argument("--serving-tune", action="store_true",
+                    help="with --serving: tune the server (tile x items per tick, lowest p99 "
+                         "at the first rate) for each mode first and keep the picks in the "
+                         "tuning cache")
+    ap.add_argument("--serving-no-check", action="store_true",
+                    help="--serving without the per-job accuracy checks (timing sweeps)")
     ap.add_argument("--occ-digits", action="store_true",
                     help="serving in the split form with the K=nocc half as INT8 digit planes "
                          "too (needs --slices 3)")
@@ -459,21 +453,40 @@ def main():
         rates = [float(x) for x in args.serving.split(",")]
         S = args.slices or 4  # -1: per-job slice counts (CCSDT_SLICES_PER_JOB)
         jobs = cc.serving_jobs(args.serving_jobs, 0xC0DE0004, tensors=args.tensors)
-        kw = dict(tensors=args.tensors, budget=int(args.budget_gb * 1e9), jobs=jobs)
+        kw = dict(tensors=args.tensors, budget=int(args.budget_gb * 1e9), jobs=jobs,
+                  items_per_tick=args.items_per_tick)
+        tuning = []
+        if args.serving_tune:
+            arr = cc.poisson_arrivals(len(jobs), rates[0], 0xC0DE0004)
+            for mode, sl, oc in (("ozaki_int8", S, int(args.occ_digits)), ("fp64", 0, 0)):
+                best, table = cc.tune_server(jobs, arr, mode=mode, slices=sl, occ_digits=oc,
+                                             budget_bytes=kw["budget"])
+                tuning.append({"mode": mode, "slices": sl, "rate_jobs_per_s": rates[0],
+                               "pick": best, "table": table})
+                print("bench: server tuned for %s slices %d: tile %d, %d items per tick (p99 %.1f ms)" % (
+                    mode, sl, best["variant"], best["items_per_tick"], best["p99_ms"]),
+                    file=sys.stderr, flush=True)
+
+        def var(mode):
+            return args.serving_variant
         occ = dict(occ_digits=int(args.occ_digits))
         # warm-up: one unmeasured pass of the same jobs, so the measured runs see
         # a server in steady state (kernels loaded, pinned staging and device
         # memory pools populated)
-        serving_report(cc, rates[0], args.serving_jobs, "ozaki_int8", S, check=False, **kw, **occ)
+        serving_report(cc, rates[0], args.serving_jobs, "ozaki_int8", S, check=False,
+                       variant=var("ozaki_int8"), **kw, **occ)
         reps = []
         for rate, mode, sl in [(r, "ozaki_int8", S) for r in rates] + [(rates[-1], "fp64", 0)]:
-            r = serving_report(cc, rate, args.serving_jobs, mode, sl, **kw, **occ)
+            r = serving_report(cc, rate, args.serving_jobs, mode, sl, variant=var(mode),
+                               check=not args.serving_no_check, **kw, **occ)
             reps.append(r)
-            print("bench: serving %s %s rate %.0f: %.1f jobs/s p50 %.1f ms p99 %.1f ms" % (
-                r["job_format"], r["mode"], r["rate_jobs_per_s"], r["jobs_per_s"], r["p50_ms"],
-                r["p99_ms"]), file=sys.stderr, flush=True)
+            print("bench: serving %s %s variant %d items/tick %d rate %.0f: %.1f jobs/s p50 %.1f ms "
+                  "p99 %.1f ms" % (r["job_format"], r["mode"], r["variant"], r["items_per_tick"],
+                                   r["rate_jobs_per_s"], r["jobs_per_s"], r["p50_ms"], r["p99_ms"]),
+                  file=sys.stderr, flush=True)
         print(json.dumps({"metric": "serving: Poisson stream of triples jobs, nocc log-U[5,40], "
-                                    "nvir log-U[50,400], one GPU", "unit": "jobs/s", "runs": reps}),
+                                    "nvir log-U[50,400], one GPU", "unit": "jobs/s", "runs": reps,
+                          "server_tuning": tuning}),
               flush=True)
         return
     if args.tensor_job:

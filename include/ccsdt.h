@@ -217,10 +217,13 @@ enum {
 enum {
     CCSDT_TILE_ONE_WAVE = 0,  /* one wave computes all 16 products of a 16x16 block pair */
     CCSDT_TILE_TWO_WAVE = 1,  /* two waves share the pair, eight products each */
-    CCSDT_TILE_ROW_PAIR = 2   /* four waves share two block pairs stacked in one column
+    CCSDT_TILE_ROW_PAIR = 2,  /* four waves share two block pairs stacked in one column
                                  block (16x16 blocks (r, c) and (r+1, c)), one product
                                  group of four each: fewer operand loads per matrix
-                                 operation.  Results are bitwise those of TWO_WAVE */
+                                 operation.  The f matrices are bitwise those of
+                                 TWO_WAVE; the energy epilogue sums in another order, so
+                                 terms agree to about 1e-13 of a triple's largest term */
+    CCSDT_TILE_AUTO = -1      /* servers only: the tuning cache's pick (ccsdt_server_opts) */
 };
 
 /* Order in which the fused kernel's work items (triple x 16x16 block pair) run.
@@ -372,7 +375,9 @@ typedef struct ccsdt_server_s *ccsdt_server;
 
 typedef struct {
     int64_t budget_bytes;    /* HBM budget for admitted jobs (0 = 200 GB) */
-    int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = 65536 */
+    int items_per_tick;      /* work items per tick (triple x 16x16 block pair); 0 = the
+                                tuning cache's server pick for this mode and slice policy
+                                on this device (ccsdt_server_tune_store), else 131072 */
     int mode;                /* CCSDT_MODE_FP64, CCSDT_MODE_OZAKI_INT8 or CCSDT_MODE_BF16X3 */
     int slices;              /* Ozaki slices (0 = 4), or CCSDT_SLICES_PER_JOB: each job's own
                                 form, chosen at submit from its size (three slices with
@@ -380,7 +385,9 @@ typedef struct {
                                 summands, triples x nvir^2, else four; the result reports
                                 it); each tick then runs one launch per form */
     int variant;             /* CCSDT_TILE_* (FP64: ONE_WAVE or TWO_WAVE; Ozaki INT8 with
-                                at most 4 slices may take ROW_PAIR) */
+                                at most 4 slices may take ROW_PAIR), or CCSDT_TILE_AUTO: the
+                                tuning cache's server pick, else 0 (as the job API's
+                                default) */
     int occ_digits;          /* CCSDT_MODE_OZAKI_INT8 with slices 3: 1 packs every job's
                                 K=nocc half as INT8 digit planes too (as
                                 ccsdt_run_opts.occ_digits); 0 otherwise */
@@ -416,6 +423,16 @@ int ccsdt_server_result(ccsdt_server s, int64_t id, ccsdt_job_result *out);
 int ccsdt_server_counts(ccsdt_server s, int64_t *waiting, int64_t *running, int64_t *finished,
                         int64_t *bytes_in_use);
 double ccsdt_server_clock(ccsdt_server s);  /* seconds since creation */
+/* The options the server runs with: the caller's, with every default and cached
+ * pick filled in. */
+int ccsdt_server_options(ccsdt_server s, ccsdt_server_opts *out);
+/* Records opts' tile (variant) and items_per_tick as the server pick for opts'
+ * mode, slice policy and occupied digit planes on the current device, in the
+ * tuning cache (memory, and the cache file when one is set); score_ms is kept
+ * with it (the tuner's measure, e.g. p99 job latency).  Servers created later
+ * with CCSDT_TILE_AUTO or items_per_tick = 0 use it.  No counterpart in the
+ * reference: the serving loop has none. */
+int ccsdt_server_tune_store(const ccsdt_server_opts *opts, double score_ms);
 int ccsdt_server_destroy(ccsdt_server s);
 
 /* ---------------------------------------------------------------- diagnostics */

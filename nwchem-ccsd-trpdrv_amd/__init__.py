@@ -51,6 +51,7 @@ MODES = {"fp64": 0, "fp64_split": 1, "ozaki_int8": 2, "bf16x3": 3, "fp32": 4}
 TAIL8_MAX_SLICES = 3
 MODE_NAMES = {v: k for k, v in MODES.items()}
 TILE_ONE_WAVE, TILE_TWO_WAVE, TILE_ROW_PAIR = 0, 1, 2
+TILE_AUTO = -1  # servers: the tuning cache's pick, else 0
 ORDER_TASK, ORDER_PAIR = 0, 1
 RUN_PERSISTENT, RUN_GRAPH = 1, 2
 SLICES_PER_JOB = -1  # server option: each job's split form chosen from its size
@@ -196,6 +197,8 @@ def lib():
             "ccsdt_server_result": (ctypes.c_int, [vp, i64, ctypes.POINTER(JobResult)]),
             "ccsdt_server_counts": (ctypes.c_int, [vp] + [ctypes.POINTER(i64)] * 4),
             "ccsdt_server_clock": (ctypes.c_double, [vp]),
+            "ccsdt_server_options": (ctypes.c_int, [vp, ctypes.POINTER(ServerOpts)]),
+            "ccsdt_server_tune_store": (ctypes.c_int, [ctypes.POINTER(ServerOpts), ctypes.c_double]),
             "ccsdt_server_destroy": (ctypes.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -564,9 +567,11 @@ class Job:
 
 
 class Server:
-    """Continuous-batching server over the device (see ccsdt_server_* in ccsdt.h)."""
+    """Continuous-batching server over the device (see ccsdt_server_* in ccsdt.h).
+    variant=TILE_AUTO and items_per_tick=0 take the tuning cache's server pick
+    for this mode and slice policy (tune_server), else tile 0 and 131072."""
 
-    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0, variant=0,
+    def __init__(self, budget_bytes=0, items_per_tick=0, mode="fp64", slices=0, variant=TILE_AUTO,
                  occ_digits=0):
         o = ServerOpts(int(budget_bytes), int(items_per_tick), MODES[mode], int(slices),
                        int(variant), int(occ_digits))
@@ -618,6 +623,12 @@ class Server:
     def clock(self):
         return lib().ccsdt_server_clock(self.handle)
 
+    def options(self):
+        """The options the server runs with (defaults and cached picks filled in)."""
+        o = ServerOpts()
+        _check(lib().ccsdt_server_options(self.handle, ctypes.byref(o)))
+        return {k: getattr(o, k) for k, _ in ServerOpts._fields_}
+
     def drain(self):
         """Progresses until every submitted job has finished."""
         while True:
@@ -663,6 +674,73 @@ def poisson_arrivals(njobs, rate, seed):
     """Arrival times (seconds) of a Poisson stream of the given rate."""
     u = pcg_uniform(seed, 91, 0.0, 1.0, njobs)
     return np.cumsum(-np.log1p(-u) / rate)
+
+
+def submit_job(srv, jb):
+    """Submits one job dict of serving_jobs() (harness arrays or tensors)."""
+    if "tensors" in jb:
+        return srv.submit_tensors(jb["tensors"], jb["nocc"], jb["nvir"], jb["triples"])
+    return srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+
+
+def serve(jobs, arrivals, **server_opts):
+    """Feeds the jobs to a new Server (server_opts as for Server) at their
+    arrival times (seconds after the start) and returns (results, latencies in
+    ms, span in s from the first submit to the last finish, the server's
+    options)."""
+    import time
+    with Server(**server_opts) as srv:
+        opts = srv.options()
+        ids = [None] * len(jobs)
+        nxt = 0
+        start = srv.clock()
+        while True:
+            now = srv.clock() - start
+            while nxt < len(jobs) and arrivals[nxt] <= now:
+                ids[nxt] = submit_job(srv, jobs[nxt])
+                nxt += 1
+            srv.progress()
+            c = srv.counts()
+            if nxt == len(jobs) and c["waiting"] == 0 and c["running"] == 0:
+                break
+            if c["running"] == 0 and nxt < len(jobs):
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.002)))
+        res = [srv.result(i) for i in ids]
+    for r in res:
+        if r["status"] != 0:
+            raise CcsdtError(r["status"], "a served job failed")
+    lat = np.array([(r["t_finish"] - r["t_submit"]) * 1e3 for r in res])
+    span = max(r["t_finish"] for r in res) - min(r["t_submit"] for r in res)
+    return res, lat, span, opts
+
+
+def tune_server(jobs, arrivals, mode="fp64", slices=0, occ_digits=0, budget_bytes=0,
+                candidates=None, warm=True):
+    """Serves the same jobs and arrivals once per candidate (tile variant, items
+    per tick) and keeps the one with the lowest p99 job latency as the server
+    pick for this mode and slice policy on this device
+    (ccsdt_server_tune_store: memory and the cache file).  Servers created
+    afterwards with variant=TILE_AUTO or items_per_tick=0 use it.  Returns
+    (pick, table).  Default candidates: tile 0 at 32k, 64k, 128k and 256k items
+    per tick, and the other tiles of the mode at 128k."""
+    if candidates is None:
+        rows = slices in (0, 3, 4, SLICES_PER_JOB)  # the row-pair tile runs at most 4 slices
+        other = {"fp64": [TILE_TWO_WAVE], "ozaki_int8": [TILE_ROW_PAIR] if rows else []}
+        candidates = [(0, n) for n in (32768, 65536, 131072, 262144)]
+        candidates += [(v, 131072) for v in other.get(mode, [])]
+    base = dict(mode=mode, slices=slices, occ_digits=occ_digits, budget_bytes=budget_bytes)
+    if warm:  # kernels loaded, pinned and device pools populated before the first timing
+        serve(jobs, arrivals, variant=candidates[0][0], items_per_tick=candidates[0][1], **base)
+    table = []
+    for v, n in candidates:
+        _, lat, span, _ = serve(jobs, arrivals, variant=v, items_per_tick=n, **base)
+        table.append({"variant": v, "items_per_tick": n, "p50_ms": float(np.percentile(lat, 50)),
+                      "p99_ms": float(np.percentile(lat, 99)), "jobs_per_s": len(jobs) / span})
+    best = min(table, key=lambda r: r["p99_ms"])
+    o = ServerOpts(int(budget_bytes), best["items_per_tick"], MODES[mode], int(slices),
+                   best["variant"], int(occ_digits))
+    _check(lib().ccsdt_server_tune_store(ctypes.byref(o), best["p99_ms"]))
+    return best, table
 
 
 def tune_cache_file(path):

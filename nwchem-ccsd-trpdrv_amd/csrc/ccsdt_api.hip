@@ -1298,8 +1298,16 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
 // when a file is set (ccsdt_tune_cache_file, or CCSDT_TUNE_CACHE at the first
 // lookup), in that file: one line per pick, "key -> options median_ms",
 // appended as picks are made and read when the file is set (later lines win).
+// Server picks (tile and tick size for a mode and slice policy on a device,
+// ccsdt_server_tune_store) share the map and the file under "v1 srv ..." keys,
+// with the tick size as a ninth field.
+struct TunePick {
+    ccsdt_run_opts o;
+    int items;  // work items per tick (server picks), 0 for job picks
+    double ms;
+};
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<ccsdt_run_opts, double>> g_tune;
+std::map<std::string, TunePick> g_tune;
 std::string g_tune_path;
 bool g_tune_env_read = false;
 
@@ -1325,8 +1333,16 @@ std::string tune_key(ccsdt_job j, const ccsdt_run_opts &b, const int *trip, int6
     return buf;
 }
 
+std::string server_tune_key(const ccsdt_server_opts &o)
+{
+    char buf[160];
+    snprintf(buf, sizeof buf, "v1 srv mode=%d slices=%d occ=%d dev=%s/%d", o.mode, o.slices, o.occ_digits,
+             g_arch, g_num_cu);
+    return buf;
+}
+
 // Parses one cache line; false for a malformed one.
-bool tune_parse(const char *line, std::string *key, ccsdt_run_opts *o, double *ms)
+bool tune_parse(const char *line, std::string *key, TunePick *pick)
 {
     const char *arrow = strstr(line, " -> ");
     if (!arrow || strncmp(line, "v1 ", 3) != 0) return false;
@@ -1335,9 +1351,13 @@ bool tune_parse(const char *line, std::string *key, ccsdt_run_opts *o, double *m
     if (sscanf(arrow + 4, "mode=%d slices=%d flags=%d variant=%d wg_per_cu=%d order=%d occ=%d ms=%lf",
                &r.mode, &r.slices, &r.flags, &r.variant, &r.wg_per_cu, &r.order, &r.occ_digits, &m) != 8)
         return false;
+    int items = 0;
+    const char *it = strstr(arrow, " items=");
+    if (it && (sscanf(it, " items=%d", &items) != 1 || items <= 0)) return false;
+    const bool srv = strncmp(line, "v1 srv ", 7) == 0;
+    if (srv != (items > 0)) return false;  // server picks carry a tick size, job picks none
     key->assign(line, arrow - line);
-    *o = r;
-    *ms = m;
+    *pick = {r, items, m};
     return true;
 }
 
@@ -1352,10 +1372,9 @@ int tune_load(const std::string &path)
     char line[512];
     while (fgets(line, sizeof line, fp)) {
         std::string key;
-        ccsdt_run_opts o;
-        double ms;
-        if (tune_parse(line, &key, &o, &ms)) {
-            g_tune[key] = {o, ms};
+        TunePick pick;
+        if (tune_parse(line, &key, &pick)) {
+            g_tune[key] = pick;
             ++n;
         }
     }
@@ -1374,26 +1393,29 @@ void tune_env_once()
     }
 }
 
-bool tune_lookup(const std::string &key, ccsdt_run_opts *o)
+bool tune_lookup(const std::string &key, ccsdt_run_opts *o, int *items = nullptr)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tune_env_once();
     auto it = g_tune.find(key);
     if (it == g_tune.end()) return false;
-    *o = it->second.first;
+    *o = it->second.o;
+    if (items) *items = it->second.items;
     return true;
 }
 
-void tune_store(const std::string &key, const ccsdt_run_opts &o, double ms)
+void tune_store(const std::string &key, const ccsdt_run_opts &o, double ms, int items = 0)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tune_env_once();
-    g_tune[key] = {o, ms};
+    g_tune[key] = {o, items, ms};
     if (g_tune_path.empty()) return;
     FILE *fp = fopen(g_tune_path.c_str(), "a");
     if (!fp) return;  // the in-memory pick stands; the file is best effort
-    fprintf(fp, "%s -> mode=%d slices=%d flags=%d variant=%d wg_per_cu=%d order=%d occ=%d ms=%.6f\n",
+    fprintf(fp, "%s -> mode=%d slices=%d flags=%d variant=%d wg_per_cu=%d order=%d occ=%d ms=%.6f",
             key.c_str(), o.mode, o.slices, o.flags, o.variant, o.wg_per_cu, o.order, o.occ_digits, ms);
+    if (items > 0) fprintf(fp, " items=%d", items);
+    fputc('\n', fp);
     fclose(fp);
 }
 
@@ -2342,15 +2364,24 @@ int64_t ccsdt_tensor_job_footprint(int ncor, int nocc, int nvir, int64_t ntriple
 }
 
 
-int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
+// Server defaults when neither the caller nor the tuning cache sets them: the
+// tick size measured best on Poisson mixes of the serving workload (nocc 5..40,
+// nvir 50..400) among 16k..256k items, and tile 0 (the job API's default, so a
+// job's totals are bitwise those of the same job run alone with default
+// options; the FP64 tiles served those mixes alike).
+constexpr int kDefaultItemsPerTick = 131072;
+
+// Validates server options and fills every default but the tile and tick size
+// (auto = 1 lets them stay CCSDT_TILE_AUTO / 0 for the tuning cache).
+int server_opts_check(ccsdt_server_opts *po, bool allow_auto)
 {
-    if (!out) return set_error(CCSDT_E_ARG, "null output handle");
-    *out = nullptr;
-    ccsdt_server_opts o = opts ? *opts : ccsdt_server_opts{};
+    ccsdt_server_opts &o = *po;
     if (o.budget_bytes <= 0) o.budget_bytes = int64_t(200) * 1000 * 1000 * 1000;
-    if (o.items_per_tick <= 0) o.items_per_tick = 65536;
+    if (o.items_per_tick < 0 || (!allow_auto && o.items_per_tick == 0))
+        return set_error(CCSDT_E_ARG, "items per tick %d", o.items_per_tick);
     if (o.mode != CCSDT_MODE_FP64 && !is_split_mode(o.mode))
         return set_error(CCSDT_E_ARG, "server mode %d must be FP64, OZAKI_INT8 or BF16X3", o.mode);
+    if (!allow_auto && o.variant == CCSDT_TILE_AUTO) return set_error(CCSDT_E_ARG, "tile variant must be set");
     if (o.mode == CCSDT_MODE_OZAKI_INT8) {
         if (o.slices == 0) o.slices = 4;
         const bool per_job = (o.slices == CCSDT_SLICES_PER_JOB);
@@ -2359,7 +2390,7 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
         if (!per_job && (o.slices < 3 || o.slices > 7))
             return set_error(CCSDT_E_ARG, "slices %d outside 3..7", o.slices);
         if (o.variant == CCSDT_TILE_TWO_WAVE) o.variant = 0;
-        if (o.variant != 0 && o.variant != CCSDT_TILE_ROW_PAIR)
+        if (o.variant != 0 && o.variant != CCSDT_TILE_ROW_PAIR && o.variant != CCSDT_TILE_AUTO)
             return set_error(CCSDT_E_ARG, "unknown tile variant %d", o.variant);
         if (o.variant == CCSDT_TILE_ROW_PAIR && !per_job && o.slices > kRowPairMaxSlices)
             return set_error(CCSDT_E_ARG, "the row-pair tile runs at most %d slices", kRowPairMaxSlices);
@@ -2370,15 +2401,52 @@ int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
                              kTail8MaxSlices, o.slices);
     } else {
         o.occ_digits = 0;
+        o.slices = 0;
     }
     if (o.mode == CCSDT_MODE_BF16X3) {
         o.slices = 3;
         o.variant = 0;
     }
-    if (o.mode == CCSDT_MODE_FP64 && o.variant != CCSDT_TILE_ONE_WAVE && o.variant != CCSDT_TILE_TWO_WAVE)
+    if (o.mode == CCSDT_MODE_FP64 && o.variant != CCSDT_TILE_ONE_WAVE && o.variant != CCSDT_TILE_TWO_WAVE &&
+        o.variant != CCSDT_TILE_AUTO)
         return set_error(CCSDT_E_ARG, "unknown tile variant %d", o.variant);
-    int rc = ensure_device(-1);
+    return CCSDT_OK;
+}
+
+int ccsdt_server_tune_store(const ccsdt_server_opts *opts, double score_ms)
+{
+    if (!opts) return set_error(CCSDT_E_ARG, "null options");
+    ccsdt_server_opts o = *opts;
+    int rc = server_opts_check(&o, false);
     if (rc) return rc;
+    if ((rc = ensure_device(-1))) return rc;
+    ccsdt_run_opts r{};
+    r.mode = o.mode;
+    r.slices = o.slices;
+    r.occ_digits = o.occ_digits;
+    r.variant = o.variant;
+    tune_store(server_tune_key(o), r, score_ms, o.items_per_tick);
+    return CCSDT_OK;
+}
+
+int ccsdt_server_create(const ccsdt_server_opts *opts, ccsdt_server *out)
+{
+    if (!out) return set_error(CCSDT_E_ARG, "null output handle");
+    *out = nullptr;
+    ccsdt_server_opts o = opts ? *opts : ccsdt_server_opts{};
+    int rc = server_opts_check(&o, true);
+    if (rc) return rc;
+    if ((rc = ensure_device(-1))) return rc;
+    if (o.variant == CCSDT_TILE_AUTO || o.items_per_tick == 0) {
+        ccsdt_run_opts r{};
+        int items = 0;
+        if (tune_lookup(server_tune_key(o), &r, &items)) {
+            if (o.variant == CCSDT_TILE_AUTO) o.variant = r.variant;
+            if (o.items_per_tick == 0) o.items_per_tick = items;
+        }
+        if (o.variant == CCSDT_TILE_AUTO) o.variant = 0;
+        if (o.items_per_tick == 0) o.items_per_tick = kDefaultItemsPerTick;
+    }
     ccsdt_server s = new ccsdt_server_s();
     s->o = o;
     s->t0 = Clock::now();
@@ -2483,6 +2551,14 @@ int ccsdt_server_counts(ccsdt_server s, int64_t *waiting, int64_t *running, int6
     if (running) *running = run;
     if (finished) *finished = s->nfinished;
     if (bytes_in_use) *bytes_in_use = s->used;
+    return CCSDT_OK;
+}
+
+int ccsdt_server_options(ccsdt_server s, ccsdt_server_opts *out)
+{
+    if (!s) return set_error(CCSDT_E_STATE, "null server");
+    if (!out) return set_error(CCSDT_E_ARG, "null output options");
+    *out = s->o;
     return CCSDT_OK;
 }
 

@@ -290,3 +290,31 @@ def test_tune_cache_file_round_trip(cc, tmp_path):
     finally:
         cc.tune_cache_file(None)
         cc.tune_cache_clear()
+
+
+def test_tune_cache_server_picks(cc, tmp_path):
+    """Server picks (tile and tick size per mode and slice policy) share the
+    cache file: a "v1 srv" line needs its items field and a job line may not
+    carry one.  Storing a pick checks the options on the host first: the tile
+    and tick size must be set.  Needs no device."""
+    path = tmp_path / "tune.txt"
+    srv = "v1 srv mode=2 slices=-1 occ=0 dev=gfx950/256 -> mode=2 slices=-1 flags=0 variant=0 " \
+          "wg_per_cu=0 order=0 occ=0 ms=47.100000 items=131072\n"
+    job = ("v1 mode=0 slices=0 occ=0 graph=0 kind=0 ncor=0 nocc=8 nvir=40 kchunk=8 n=120 "
+           "hash=0123456789abcdef dev=gfx950/256 -> mode=0 slices=0 flags=0 variant=1 "
+           "wg_per_cu=0 order=1 occ=0 ms=1.000000\n")
+    bad = [srv.replace(" items=131072", ""), job.replace("ms=1.000000", "ms=1.000000 items=4096"),
+           srv.replace("items=131072", "items=0")]
+    path.write_text(srv + job + "".join(bad))
+    try:
+        cc.tune_cache_clear()
+        assert cc.tune_cache_file(str(path)) == 2
+        assert cc.tune_cache_size() == 2
+    finally:
+        cc.tune_cache_file(None)
+        cc.tune_cache_clear()
+    L = cc.lib()
+    for variant, items in ((cc.TILE_AUTO, 65536), (0, 0), (7, 65536)):
+        o = cc.ServerOpts(0, items, cc.MODES["fp64"], 0, variant, 0)
+        assert L.ccsdt_server_tune_store(ctypes.byref(o), 1.0) == -1, (variant, items)
+    assert cc.tune_cache_size() == 0

@@ -228,3 +228,41 @@ def test_per_job_forms_serving_workload(cc):
         E = f["emp4"] + f["emp5"]
         assert abs(r["emp4"] + r["emp5"] - E) <= 1e-8 * abs(E), (jb["nocc"], jb["nvir"], r["slices"])
     assert 0 < n3 < len(jobs)
+
+
+def test_server_tuning_pick_is_used(cc, tmp_path):
+    """tune_server keeps the lowest-p99 (tile, items per tick) candidate as the
+    server pick for its mode and slice policy; a server created with the tile
+    and tick size left to it runs with the pick (and serves bitwise what the
+    same options given explicitly serve), explicit options win, another slice
+    policy gets the defaults, and the pick survives in the cache file."""
+    path = tmp_path / "tune.txt"
+    cc.tune_cache_file(str(path))
+    try:
+        jobs = small_jobs(cc, 4)
+        arr = cc.poisson_arrivals(len(jobs), 200.0, 0xC0DE0004)
+        cands = [(0, 2048), (cc.TILE_ROW_PAIR, 8192)]
+        best, table = cc.tune_server(jobs, arr, mode="ozaki_int8", slices=4, candidates=cands,
+                                     warm=False)
+        assert [(r["variant"], r["items_per_tick"]) for r in table] == cands
+        assert best["p99_ms"] == min(r["p99_ms"] for r in table)
+        pick = (best["variant"], best["items_per_tick"])
+        res, _, _, opts = cc.serve(jobs, arr, mode="ozaki_int8", slices=4)
+        assert (opts["variant"], opts["items_per_tick"]) == pick
+        ref, _, _, _ = cc.serve(jobs, arr, mode="ozaki_int8", slices=4, variant=pick[0],
+                                items_per_tick=pick[1])
+        assert [(r["emp4"], r["emp5"]) for r in res] == [(r["emp4"], r["emp5"]) for r in ref]
+        with cc.Server(mode="ozaki_int8", slices=4, variant=0, items_per_tick=1024) as srv:
+            o = srv.options()
+            assert (o["variant"], o["items_per_tick"]) == (0, 1024)
+        with cc.Server(mode="ozaki_int8", slices=3) as srv:
+            o = srv.options()
+            assert (o["variant"], o["items_per_tick"]) == (0, 131072)
+        cc.tune_cache_clear()
+        assert cc.tune_cache_file(str(path)) == 1
+        with cc.Server(mode="ozaki_int8", slices=4) as srv:
+            o = srv.options()
+            assert (o["variant"], o["items_per_tick"]) == pick
+    finally:
+        cc.tune_cache_file(None)
+        cc.tune_cache_clear()
