@@ -296,12 +296,13 @@ def dropin_report(cc, arrays, trip, nocc, nvir, flops, ncalls=24):
     return out
 
 
-def tensor_job_report(cc, steps, warmup, slices):
+def tensor_job_report(cc, steps, warmup, slices, pinned=None):
     """The nocc=30/nvir=300 list (4960 triples) as a full-tensor job: every
     (i,j) and (k,j) slice distinct, as real amplitudes and integrals are, so
     operand panels are shared between triples only through common occupied
     indices (the harness-shaped job shares its i-indexed panels across the whole
-    list).  Both forms autotuned and timed as graph replays."""
+    list).  Both forms autotuned (or run with pinned[mode], JSON run options)
+    and timed as graph replays."""
     nocc, nvir = 30, 300
     t = cc.make_tensor_inputs(nocc, nvir, cc.config_seed(1))
     trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, 16))
@@ -311,7 +312,11 @@ def tensor_job_report(cc, steps, warmup, slices):
            "footprint_gb": cc.tensor_footprint(nocc, nvir, len(trip)) / 1e9, "forms": {}}
     with cc.Job.from_tensors(t, nocc, nvir) as job:
         for mode, S in (("fp64", 0), ("ozaki_int8", slices)):
-            best, table = job.autotune(trip, mode=mode, slices=S, graph=True, reps=2)
+            if pinned and pinned.get(mode):
+                best = dict(json.loads(pinned[mode]), mode=mode, graph=True)
+                best.setdefault("slices", S or (4 if mode == "ozaki_int8" else 0))
+            else:
+                best, table = job.autotune(trip, mode=mode, slices=S, graph=True, reps=2)
             r = time_form(job, trip, steps, warmup, **best)
             kern = [job.run(trip, **dict(best, graph=False))["ms_main"] for _ in range(3)]
             out["forms"][mode] = {"value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
@@ -490,7 +495,13 @@ def main():
               flush=True)
         return
     if args.tensor_job:
-        print(json.dumps(tensor_job_report(cc, args.steps, args.warmup, args.slices)), flush=True)
+        rep = tensor_job_report(cc, args.steps, args.warmup, args.slices,
+                                pinned={"fp64": args.fp64_opts, "ozaki_int8": args.split_opts})
+        if args.copy_calibration > 0:
+            nbytes = int(args.copy_calibration * 2 ** 20)
+            ms = cc.lib().ccsdt_debug_copy(nbytes, 3)
+            rep["copy_calibration"] = {"bytes": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}
+        print(json.dumps(rep), flush=True)
         return
     if args.split_sweep:
         print(json.dumps(split_sweep_report(cc, args.steps, args.warmup)), flush=True)
