@@ -623,6 +623,18 @@ def main():
                 "traffic": traffic_from_profile("fp64", fp["opts"]), "kernel_ms": fp["kernel_ms"],
                 "flops_per_launch": total_flops}
 
+    if split_ok and not args.no_extras:
+        # the rate the INT8 matrix cores sustain on register operands on this
+        # device: the datasheet figure assumes operands whose bits do not matter
+        rates = {}
+        for shape, name in ((0, "16x16x64"), (1, "32x32x32")):
+            for rnd in (1, 0):
+                rates["%s_%s" % (name, "random" if rnd else "constant")] = \
+                    cc.lib().ccsdt_debug_mfma_rate(shape, rnd)
+        roof["measured_int8_mfma_TOPs"] = rates
+        if rates["16x16x64_random"] > 0:
+            roof["frac_of_measured_16x16x64_random"] = roof["achieved"] / rates["16x16x64_random"]
+
     def cand(table):
         return [{k: v for k, v in row.items() if k not in ("mode", "graph", "status")}
                 for row in table]
@@ -659,6 +671,7 @@ def main():
                        "candidates": cand(sp["table"]), "value": tf(sp["ms_per_step"]),
                        "tune_seconds": sp["tune_s"], "tune_cached": sp["tune_cached"],
                        "ms_per_step": sp["ms_per_step"], "kernel_ms": sp["kernel_ms"],
+                       "kernel_tflops": tf(sp["kernel_ms"]),
                        "p50_ms": sp["p50_ms"], "p99_ms": sp["p99_ms"],
                        "emp4": sp["res"]["emp4"], "emp5": sp["res"]["emp5"],
                        "rel_dE_vs_fp64": dev, "within_tol": split_ok,

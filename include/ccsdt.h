@@ -458,6 +458,13 @@ double ccsdt_debug_copy(int64_t bytes, int reps);
  * HBM reference point for kernels that only read.  No counterpart in the
  * reference. */
 double ccsdt_debug_read(int64_t bytes, int reps);
+/* INT8 matrix-core rate on register operands, in TOP/s (or -1): shape 0 runs
+ * v_mfma_i32_16x16x64_i8 (the split kernels' instruction), 1
+ * v_mfma_i32_32x32x32_i8, at four waves per SIMD; random_operands = 1 feeds
+ * per-lane pseudo-random bytes (consecutive MFMAs see different bits, as digit
+ * planes do), 0 all ones.  The sustained rate depends on the operand bits.  No
+ * counterpart in the reference. */
+double ccsdt_debug_mfma_rate(int shape, int random_operands);
 /* Nanoseconds per wave-wide load per CU when every CU streams L1-resident data
  * with loads of `bytes` (4, 8 or 16) per lane into VGPRs, or (to_lds = 1, 16
  * bytes) as global->LDS DMA; multiply by the shader clock for cycles.  No

@@ -187,6 +187,7 @@ def lib():
             "ccsdt_selftest_mfma_i8": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
             "ccsdt_debug_copy": (ctypes.c_double, [i64, ctypes.c_int]),
             "ccsdt_debug_read": (ctypes.c_double, [i64, ctypes.c_int]),
+            "ccsdt_debug_mfma_rate": (ctypes.c_double, [ctypes.c_int, ctypes.c_int]),
             "ccsdt_debug_load_rate": (ctypes.c_double, [ctypes.c_int, ctypes.c_int]),
             "ccsdt_debug_host_copy": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
             "ccsdt_debug_occ_fold": (None, [ctypes.c_int]),

@@ -2720,6 +2720,37 @@ double ccsdt_debug_copy(int64_t bytes, int reps)
     return ms / reps;
 }
 
+double ccsdt_debug_mfma_rate(int shape, int random_operands)
+{
+    if (ensure_device(-1)) return -1.0;
+    if (shape != 0 && shape != 1) {
+        set_error(CCSDT_E_ARG, "mfma rate probe: shape %d", shape);
+        return -1.0;
+    }
+    const int grid = g_num_cu * 4;  // four waves per SIMD
+    const int iters = 2000;
+    int *out = nullptr;
+    if (dev_alloc((void **)&out, (size_t)grid * 256 * sizeof(int))) return -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    const bool rnd = random_operands != 0;
+    bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+              launch_mfma_rate(out, shape, rnd, grid, iters, nullptr) == hipSuccess &&
+              hipEventRecord(e0, nullptr) == hipSuccess;
+    for (int r = 0; r < 5 && ok; ++r) ok = launch_mfma_rate(out, shape, rnd, grid, iters, nullptr) == hipSuccess;
+    ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(out);
+    if (!ok || ms <= 0.f) {
+        set_error(CCSDT_E_DEVICE, "mfma rate probe failed");
+        return -1.0;
+    }
+    const double ops = (double)grid * 4 * iters * 8 * (shape == 1 ? 65536.0 : 32768.0) * 5;
+    return ops / (ms * 1e-3) / 1e12;
+}
+
 double ccsdt_debug_read(int64_t bytes, int reps)
 {
     if (ensure_device(-1)) return -1.0;

@@ -214,6 +214,8 @@ hipError_t launch_reduce_list(const double *terms, const Task *tasks, int ntasks
                               hipStream_t st);
 hipError_t launch_mfma_probe(const double *a, const double *b, double *d, hipStream_t st);
 hipError_t launch_mfma_i8_probe(const int *a, const int *b, int *d, int shape, hipStream_t st);
+// INT8 MFMA rate probe: grid workgroups of 256 threads into out[grid * 256].
+hipError_t launch_mfma_rate(int *out, int shape, bool rnd, int grid, int iters, hipStream_t st);
 hipError_t launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
 // Streaming read-and-sum of n doubles into out[blocks] (blocks <= kReadSumBlocks).
 constexpr int kReadSumBlocks = 16384;

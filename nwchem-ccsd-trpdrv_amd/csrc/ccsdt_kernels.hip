@@ -2478,6 +2478,67 @@ hipError_t launch_mfma_i8_probe(const int *a, const int *b, int *d, int shape, h
     return hipGetLastError();
 }
 
+// INT8 MFMA rate probe on register operands: every wave runs iters x 8 MFMAs
+// (v_mfma_i32_16x16x64_i8, or with BIG v_mfma_i32_32x32x32_i8) over eight
+// operand register pairs, all ones or (RND) per-lane xorshift values, so that
+// consecutive MFMAs see the same or different bits.  Measures how much of the
+// datasheet INT8 rate the instruction sustains for the data it is given.
+__device__ __forceinline__ int4v probe_rnd4(uint32_t s)
+{
+    int4v v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        s ^= s << 13;
+        s ^= s >> 17;
+        s ^= s << 5;
+        v[i] = (int)s;
+    }
+    return v;
+}
+
+template <bool BIG, bool RND>
+__global__ __launch_bounds__(256) void k_mfma_rate(int *__restrict__ out, int iters)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    int4v a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        a[u] = RND ? probe_rnd4(0x9E3779B9u * (lane + 1) + 977u * u + blockIdx.x) : int4v{1, 1, 1, 1};
+        b[u] = RND ? probe_rnd4(0x85EBCA6Bu * (lane + 7) + 131u * u + blockIdx.x) : int4v{1, 1, 1, 1};
+    }
+    int s = 0;
+    if constexpr (BIG) {
+        int16v acc[4] = {};
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u & 3] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[u], acc[u & 3], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += acc[u][lane & 15];
+    } else {
+        int4v acc[8] = {};
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u], b[(u + 3) & 7], acc[u], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += acc[u][lane & 3];
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+hipError_t launch_mfma_rate(int *out, int shape, bool rnd, int grid, int iters, hipStream_t st)
+{
+    if (shape == 1) {
+        if (rnd) hipLaunchKernelGGL((k_mfma_rate<true, true>), dim3(grid), dim3(256), 0, st, out, iters);
+        else hipLaunchKernelGGL((k_mfma_rate<true, false>), dim3(grid), dim3(256), 0, st, out, iters);
+    } else if (shape == 0) {
+        if (rnd) hipLaunchKernelGGL((k_mfma_rate<false, true>), dim3(grid), dim3(256), 0, st, out, iters);
+        else hipLaunchKernelGGL((k_mfma_rate<false, false>), dim3(grid), dim3(256), 0, st, out, iters);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // Plain streaming copy of n doubles (HBM counter calibration: reads and writes
 // exactly 8*n bytes each).
 __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n2)

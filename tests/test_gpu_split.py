@@ -225,6 +225,29 @@ def test_int8_and_bf16_mfma_layouts(cc):
     ref = A.astype(np.float64) @ B.astype(np.float64)
     got = np.array([[D[16 * (r // 4) + c, r % 4] for c in range(16)] for r in range(16)])
     assert np.array_equal(got, ref)
+    # INT8 32x32x32: lane l holds row l%32 (A) / column l%32 (B) at K bytes
+    # 16*(l/32); result element e of lane l is D[8*(e/4) + 4*(l/32) + e%4][l%32]
+    A = rng.integers(-100, 101, (32, 32)).astype(np.int8)
+    B = rng.integers(-100, 101, (32, 32)).astype(np.int8)
+    a = np.array([A[l % 32, 16 * (l // 32):16 * (l // 32) + 16] for l in range(64)])
+    b = np.array([B[16 * (l // 32):16 * (l // 32) + 16, l % 32] for l in range(64)])
+    d = cc.selftest_mfma_i8(a, b, 1)
+    got = np.zeros((32, 32), np.int64)
+    for l in range(64):
+        for e in range(16):
+            got[8 * (e // 4) + 4 * (l // 32) + e % 4, l % 32] = d[l, e]
+    assert np.array_equal(got, A.astype(np.int64) @ B.astype(np.int64))
+
+
+def test_mfma_rate_probe(cc):
+    """ccsdt_debug_mfma_rate runs both INT8 instructions on constant and random
+    register operands and reports a rate in TOP/s below the 5.0 POP/s datasheet
+    figure (the values themselves are measurements, reported by bench.py)."""
+    for shape in (0, 1):
+        for rnd in (0, 1):
+            r = cc.lib().ccsdt_debug_mfma_rate(shape, rnd)
+            assert 100.0 < r < 5200.0, (shape, rnd, r)
+    assert cc.lib().ccsdt_debug_mfma_rate(2, 0) == -1.0
 
 
 @pytest.mark.parametrize("illcond", [False, True])
