@@ -9,6 +9,9 @@
   tiny shapes, and between the two DGEMM providers.
 * Invariants the mathematics fixes: n<->t swap symmetry, exact power-of-two
   homogeneity, f = 0 => 0, and the i == k rule.
+* tengy against its constant-denominator closed form (Frobenius products and
+  bilinear forms of the f matrices, numpy matrix algebra), which also rejects a
+  variant with one operand left untransposed.
 """
 import ctypes
 import ctypes.util
@@ -302,3 +305,64 @@ def test_fp32_reference_body_tracks_fp64(orc, cc):
         E4, S4 = t[0] + ik * t[2], s[0] + ik * s[2]
         E5, S5 = t[1] + ik * t[3], s[1] + ik * s[3]
         assert abs(e4 - E4) <= 1e-5 * S4 and abs(e5 - E5) <= 1e-5 * S5
+
+
+def _closed_form_tengy(F, v, delta, swap_one=False):
+    """The four sums of omp/ccsd_tengy_c99.c:91-111 when every denominator is
+    the same delta, as matrix expressions: a summand term X_bc Y_bc summed over
+    all (b, c) is the Frobenius product <X, Y>, X_cb Y_bc is <X^T, Y>, and
+    t1v[b] d[c] X_bc sums to t1v^T X d.  F[name] is the nvir x nvir matrix with
+    F[b, c] = f(b + c*nvir).  swap_one uses F1n where F1n^T belongs (a plausible
+    transcription error the check must catch)."""
+    def ip(x, y):
+        return float(np.sum(x * y))
+
+    def e4(n, t):
+        # n, t: dicts "1".."4" -> the n-side and t-side matrices of this sum
+        T = (lambda m: m) if swap_one else (lambda m: m.T)
+        a1 = t["1"] + T(n["1"]) + t["2"].T + n["3"] + n["4"].T
+        b1 = t["1"] - 2 * t["2"] - 2 * t["3"] + t["4"]
+        a2 = n["1"] + t["1"].T + n["2"].T + n["3"].T
+        b2 = 2 * t["1"] - t["2"] - t["3"] + 2 * t["4"]
+        c3 = ip(n["1"], n["1"] + n["3"].T + 2 * t["4"].T) + ip(n["2"], t["2"].T) + ip(n["3"], t["4"])
+        return delta * (ip(a1, b1) - ip(a2, b2) + 3 * c3)
+
+    def e5(n, t, t1v, dx, dc):
+        mx = (t["1"] + n["2"] + n["4"].T - 2 * (t["3"] + n["4"] + n["2"].T + n["1"] + t["2"] + n["3"].T)
+              + 4 * (n["3"] + t["4"] + n["1"].T))
+        mc = n["1"] + n["4"] + t["1"].T - 2 * (n["2"] + n["3"] + t["2"].T)
+        return delta * (t1v @ mx @ dx + t1v @ mc @ dc)
+
+    fn = {q: F[q + "n"] for q in "1234"}
+    ft = {q: F[q + "t"] for q in "1234"}
+    return (e4(fn, ft), e5(fn, ft, v["t1v1"], v["dintx1"], v["dintc1"]),
+            e4(ft, fn), e5(ft, fn, v["t1v2"], v["dintx2"], v["dintc2"]))
+
+
+@pytest.mark.parametrize("nvir", [1, 7, 23])
+def test_tengy_constant_denominator_closed_form(orc, cc, one_thread, nvir):
+    """With every virtual orbital energy equal, denom = -1/(2 e_v + eaijk) is
+    one constant, and emp4i/emp5i/emp4k/emp5k are Frobenius products, traces
+    and bilinear forms of the f matrices (computed here with numpy matrix
+    algebra, not the summation loop).  The oracle's reduction must match them;
+    the same closed form with one operand left untransposed must not."""
+    nocc = 3
+    arrays = dict(rand_case(cc, nocc, nvir, nocc, seed=31))
+    eorb = np.array(arrays["eorb"], dtype=np.float64)
+    eorb[nocc:] = 0.75
+    arrays["eorb"] = eorb
+    ea = -1.9
+    delta = -1.0 / (0.75 + 0.75 + ea)
+    rng = np.random.default_rng(nvir)
+    names = ["1n", "1t", "2n", "2t", "3n", "3t", "4n", "4t"]  # the tengy argument order
+    F = {nm: rng.uniform(-1.0, 1.0, (nvir, nvir)) for nm in names}
+    got = orc.tengy([F[nm].T for nm in names], arrays, ea, 0, nocc, nvir)  # [c][b] = column-major
+    v = {nm: np.asarray(arrays[nm][:nvir], dtype=np.float64) for nm in
+         ("t1v1", "t1v2", "dintx1", "dintx2", "dintc1", "dintc2")}
+    want = _closed_form_tengy(F, v, delta)
+    scale = delta * 64 * nvir * nvir  # the sums' size: entries of order one, a few dozen terms each
+    for g, w in zip(got, want):
+        assert abs(g - w) <= 1e-13 * abs(scale), (got, want)
+    if nvir > 1:
+        wrong = _closed_form_tengy(F, v, delta, swap_one=True)
+        assert max(abs(g - w) for g, w in zip(got, wrong)) > 1e-3 * abs(scale)
