@@ -204,6 +204,26 @@ def test_config1_head_golden(cc, orc):
     assert rel_terms(got["terms"], g["terms"], ref["abs4"]) < TOL
 
 
+def test_config1_strided_sample_vs_oracle(cc, orc):
+    """Every 10th triple of the 30/300 list (496 triples: every k block, i, j and
+    k pattern of the list) against the oracle replaying the reference body: the
+    FP64 form per term within 1e-11 of max(|term|, sum of |summands|) and in
+    E[T] within 1e-11; the split form the bench reports (INT8, 3 slices, the
+    K = nocc half as digit planes) within 1e-8 in E[T] and 1e-6 of the summand
+    scale per term."""
+    arrays, blocks, trip, kchunk = cc.config_case(1)
+    sample = trip[::10]
+    ref = orc.run_list(arrays, 0, 30, 300, sample, with_abs=True)
+    E = ref["emp4"] + ref["emp5"]
+    with cc.Job(arrays, 30, 300, kchunk) as job:
+        got = job.run(sample, terms=True)
+        spl = job.run(sample, mode="ozaki_int8", slices=3, occ_digits=1, terms=True)
+    assert rel_terms(got["terms"], ref["terms"], ref["abs4"]) < TOL
+    assert abs(got["emp4"] + got["emp5"] - E) <= TOL * abs(E)
+    assert abs(spl["emp4"] + spl["emp5"] - E) <= 1e-8 * abs(E)
+    assert np.max(np.abs(spl["terms"] - ref["terms"]) / ref["abs4"]) < 1e-6
+
+
 def test_config1_full_list_properties(cc):
     """All 4960 triples at 30/300: totals equal the compensated sum of the
     per-triple terms with the i != k rule, the run is bitwise repeatable, and
