@@ -472,17 +472,15 @@ def main():
                     mode, sl, best["variant"], best["items_per_tick"], best["p99_ms"]),
                     file=sys.stderr, flush=True)
 
-        def var(mode):
-            return args.serving_variant
         occ = dict(occ_digits=int(args.occ_digits))
         # warm-up: one unmeasured pass of the same jobs, so the measured runs see
         # a server in steady state (kernels loaded, pinned staging and device
         # memory pools populated)
         serving_report(cc, rates[0], args.serving_jobs, "ozaki_int8", S, check=False,
-                       variant=var("ozaki_int8"), **kw, **occ)
+                       variant=args.serving_variant, **kw, **occ)
         reps = []
         for rate, mode, sl in [(r, "ozaki_int8", S) for r in rates] + [(rates[-1], "fp64", 0)]:
-            r = serving_report(cc, rate, args.serving_jobs, mode, sl, variant=var(mode),
+            r = serving_report(cc, rate, args.serving_jobs, mode, sl, variant=args.serving_variant,
                                check=not args.serving_no_check, **kw, **occ)
             reps.append(r)
             print("bench: serving %s %s variant %d items/tick %d rate %.0f: %.1f jobs/s p50 %.1f ms "
