@@ -299,8 +299,9 @@ int ccsdt_job_autotune(ccsdt_job job, const int *triples, int64_t n, const ccsdt
                        int *count);
 /* Tuning cache.  ccsdt_job_autotune keeps every pick, keyed by the request (base's
  * mode, slices -- 0 meaning "choose" -- occ_digits and graph flag), the job (kind,
- * ncor, nocc, nvir, kchunk), the triple list (length and a hash of its quintuples)
- * and the device (architecture, CU count).  For a key it has seen it returns the
+ * ncor, nocc, nvir, kchunk), the triple list (length and a hash of its quintuples),
+ * the device (architecture, CU count) and the library build (its compile time:
+ * another build's picks are not reused).  For a key it has seen it returns the
  * kept pick at once, without timing: *count = 0 and table untouched.  Picks are
  * kept in memory and, when a file is set, appended to it as lines of text.
  * ccsdt_tune_cache_file sets that file (NULL or "" for memory only) and reads

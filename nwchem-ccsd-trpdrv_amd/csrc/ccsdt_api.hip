@@ -1294,7 +1294,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
 // The autotuner's picks, keyed by what decides them: the request (mode, slice
 // count or 0 = choose, occupied digit planes, graph flag), the job (kind,
 // ncor, nocc, nvir, kchunk), the triple list (length and an FNV-1a hash of its
-// quintuples) and the device (architecture, CU count).  Kept in memory and,
+// quintuples), the device (architecture, CU count) and the library build.  Kept in memory and,
 // when a file is set (ccsdt_tune_cache_file, or CCSDT_TUNE_CACHE at the first
 // lookup), in that file: one line per pick, "key -> options median_ms",
 // appended as picks are made and read when the file is set (later lines win).
@@ -1322,22 +1322,26 @@ uint64_t fnv1a(const int *v, int64_t n)
     return h;
 }
 
+// The build a pick was made by (the library's compile time): a rebuilt library,
+// whose kernels may differ, does not reuse the picks of another build.
+const char *const kTuneBuild = __DATE__ " " __TIME__;
+
 std::string tune_key(ccsdt_job j, const ccsdt_run_opts &b, const int *trip, int64_t n)
 {
-    char buf[320];
+    char buf[384];
     snprintf(buf, sizeof buf, "v1 mode=%d slices=%d occ=%d graph=%d kind=%d ncor=%d nocc=%d nvir=%d "
-             "kchunk=%d n=%lld hash=%016llx dev=%s/%d",
+             "kchunk=%d n=%lld hash=%016llx dev=%s/%d build=%s",
              b.mode, b.slices, b.occ_digits, (b.flags & CCSDT_RUN_GRAPH) ? 1 : 0, j->kind, j->g.ncor,
              j->g.nocc, j->g.nvir, j->kchunk, (long long)n, (unsigned long long)fnv1a(trip, 5 * n),
-             g_arch, g_num_cu);
+             g_arch, g_num_cu, kTuneBuild);
     return buf;
 }
 
 std::string server_tune_key(const ccsdt_server_opts &o)
 {
-    char buf[160];
-    snprintf(buf, sizeof buf, "v1 srv mode=%d slices=%d occ=%d dev=%s/%d", o.mode, o.slices, o.occ_digits,
-             g_arch, g_num_cu);
+    char buf[224];
+    snprintf(buf, sizeof buf, "v1 srv mode=%d slices=%d occ=%d dev=%s/%d build=%s", o.mode, o.slices,
+             o.occ_digits, g_arch, g_num_cu, kTuneBuild);
     return buf;
 }
 
