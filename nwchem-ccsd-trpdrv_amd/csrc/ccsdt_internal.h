@@ -202,10 +202,15 @@ constexpr int kTileRowPair = 2;
 // Largest INT8 slice count the row-pair tile runs (its accumulators for two
 // blocks must fit two waves per SIMD).
 constexpr int kRowPairMaxSlices = 4;
-// Largest K (nvir, and nocc) the row-pair tile runs: it combines adjacent digit
-// levels exactly in INT32, (acc_L << 7) + acc_(L+1), and a level sum of K terms is
-// at most 24448 K in magnitude (S <= 4), 129 x 24448 x 768 < 2^31.
-constexpr int kRowPairMaxK = 768;
+// Largest K for which adjacent digit levels combine exactly in INT32,
+// (acc_L << 7) + acc_(L+1) (S <= 4).  Per K term a level sum is at most 16129,
+// 16256, 20352 and 24448 in magnitude (levels 0..3: |d_1| <= 127, later digits
+// <= 64), so the largest combined value, levels 2 and 3 at S = 4, is at most
+// (20352 x 128 + 24448) K = 2629504 K < 2^31 for K <= 816.
+constexpr int kIntPairMaxK = 768;
+// Largest K (nvir, and nocc) the row-pair tile runs: it always combines levels
+// in INT32.
+constexpr int kRowPairMaxK = kIntPairMaxK;
 // Resident workgroups per CU of the fused kernels (for persistent grids).
 int fused_blocks_per_cu(bool split, int slices, int variant);
 hipError_t launch_reduce_terms(const double *partial, int npairs, int ntasks, double *terms,

@@ -817,6 +817,39 @@ struct PlaneMath<true> {
     static __device__ __forceinline__ double weight(int) { return 1.0; }
 };
 
+// One lane's row r of a digit-product tile: V = sum_L acc_L 2^(7 (S-1-L)), an
+// integer below 2^53 and so exact in FP64, equal to sum_L acc_L w(L) / w(S-1).
+// Adjacent levels are first combined in INT32, (acc_L << 7) + acc_(L+1), exact
+// when every level sum has at most kIntPairMaxK terms (S <= 4): one conversion
+// and one FMA fewer per pair than converting every level.
+template <int S>
+__device__ __forceinline__ double level_sum(const int4v (&acc)[S], int r)
+{
+    double u;
+    int q;
+    if constexpr (S % 2 == 1) {
+        u = (double)acc[0][r];
+        q = 1;
+    } else {
+        u = (double)((acc[0][r] << 7) + acc[1][r]);
+        q = 2;
+    }
+#pragma unroll
+    for (; q + 1 < S; q += 2) u = __builtin_fma(u, 16384.0, (double)((acc[q][r] << 7) + acc[q + 1][r]));
+    return u;
+}
+
+// The same integer V for any K: every level converted, then Horner steps by 2^7
+// (each partial value an integer below 2^53, so every step is exact).
+template <int S>
+__device__ __forceinline__ double level_horner(const int4v (&acc)[S], int r)
+{
+    double u = (double)acc[0][r];
+#pragma unroll
+    for (int q = 1; q < S; ++q) u = __builtin_fma(u, 128.0, (double)acc[q][r]);
+    return u;
+}
+
 // stash: this wave's 8 KiB of LDS, holding group 0's four products while group
 // 1's digit products run, so their 32 VGPRs are free during the INT8 loops.
 // sst: this wave's 2 KiB of LDS for the row scales of its eight panels (p = A0,
@@ -858,6 +891,12 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
     const uint32_t blk8 = (uint32_t)(sg.nkv + sg.nkx) * 1024;  // bytes of a 16-row block of one plane
     const uint32_t voff8 = lane * 16;
     const int row0 = 4 * (lane >> 4), cx = ci * 16 + (lane & 15);
+    // INT8: weight of the lowest level, 2^(-7(S+1)), applied with the column
+    // scale; adjacent levels pair in INT32 when no level sum exceeds
+    // kIntPairMaxK terms (S <= 4)
+    const double wlow = BF ? 1.0 : __builtin_ldexp(1.0, -7 * (S + 1));
+    const bool pairable = !BF && S <= 4 && sg.nkv * 64 <= kIntPairMaxK &&
+                          (!T8 || sg.fold || sg.nkx * 64 <= kIntPairMaxK);
 #pragma unroll
     for (int gi = 0; gi < 2; ++gi) {
         // panels {A0, A1, B0, B1}: W=0 groups n, t; W=1 groups n', t'
@@ -870,11 +909,14 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             B0 = gi == 0 ? LJ : LJT;  B1 = gi == 0 ? LK : LKT;
         }
         typedef typename PlaneMath<BF>::acc_t acc_t;
+        const acc_t zacc = {0, 0, 0, 0};
         acc_t iacc[4][S];
+        // (cleared here: peeling the first K step to start each level from a
+        // zero operand instead raised the kernel above 168 VGPRs)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int q = 0; q < S; ++q) iacc[c][q] = acc_t{0, 0, 0, 0};
+            for (int q = 0; q < S; ++q) iacc[c][q] = zacc;
         const uint32_t oa0 = (uint32_t)A0 * S * plane + bi * blk8;
         const uint32_t oa1 = (uint32_t)A1 * S * plane + bi * blk8;
         const uint32_t ob0 = (uint32_t)B0 * S * plane + ci * blk8;
@@ -891,18 +933,22 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             B0_[q] = buffer_load_i8x16(rd, voff8, ob0 + q * plane + ko_);                        \
             B1_[q] = buffer_load_i8x16(rd, voff8, ob1 + q * plane + ko_);                        \
         }
-#define SPLIT_MFMA(A0_, A1_, B0_, B1_)                                                           \
+#define SPLIT_MFMA_Z(A0_, A1_, B0_, B1_, first)                                                  \
         _Pragma("unroll") for (int sa = 0; sa < S; ++sa) {                                      \
             _Pragma("unroll") for (int sb = 0; sb < S - sa; ++sb) {                             \
                 const int lv = sa + sb;                                                          \
-                iacc[0][lv] = PlaneMath<BF>::mfma(A0_[sa], B0_[sb], iacc[0][lv]);                 \
-                iacc[1][lv] = PlaneMath<BF>::mfma(A1_[sa], B0_[sb], iacc[1][lv]);                 \
-                iacc[2][lv] = PlaneMath<BF>::mfma(A0_[sa], B1_[sb], iacc[2][lv]);                 \
-                iacc[3][lv] = PlaneMath<BF>::mfma(A1_[sa], B1_[sb], iacc[3][lv]);                 \
+                const bool z_ = (first) && sa == 0;                                              \
+                iacc[0][lv] = PlaneMath<BF>::mfma(A0_[sa], B0_[sb], z_ ? zacc : iacc[0][lv]);     \
+                iacc[1][lv] = PlaneMath<BF>::mfma(A1_[sa], B0_[sb], z_ ? zacc : iacc[1][lv]);     \
+                iacc[2][lv] = PlaneMath<BF>::mfma(A0_[sa], B1_[sb], z_ ? zacc : iacc[2][lv]);     \
+                iacc[3][lv] = PlaneMath<BF>::mfma(A1_[sa], B1_[sb], z_ ? zacc : iacc[3][lv]);     \
             }                                                                                    \
         }
+#define SPLIT_MFMA(A0_, A1_, B0_, B1_) SPLIT_MFMA_Z(A0_, A1_, B0_, B1_, false)
         const int nkv = sg.nkv;
         if constexpr (S <= 4 && !BF) {
+            // step 0 is peeled: its level-opening products take a zero
+            // accumulator, so the accumulators need no clearing
             SPLIT_LOAD(xa0, xa1, xb0, xb1, 0)
             int ks = 0;
             for (; ks + 1 < nkv; ks += 2) {
@@ -924,18 +970,30 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
         // recombine: f = 2^(eA(b) + eB(c)) * sum_L acc_L * w(L) (INT8 w(L) =
         // 2^(-7(L+2)), BF16 w = 1); the combination (a, b) is product
         // 4*gi + a + 2*b (W=0) or 4*gi + 2*a + b (W=1).  With T8 a second pass
-        // adds the K = nocc steps with their own scales (sst + 128).
-        auto recombine = [&](const double *ss, bool add) {
+        // adds the K = nocc steps with their own scales (sst + 128).  INT8: the
+        // levels are summed as the exact integer V = sum_L acc_L 2^(7(S-1-L))
+        // (level_sum when the levels pair in INT32, else level_horner) and
+        // f = V sA (sB 2^(-7(S+1))): every step is exact (powers of two and
+        // integers below 2^53), so f is bitwise the weighted sum's.
+        auto recombine = [&](const double *ss, bool add, auto pairc) {
+            constexpr bool PAIR = decltype(pairc)::value;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int ca = c & 1, cb = c >> 1;
                 const int lp = 4 * gi + (W == 0 ? ca + 2 * cb : 2 * ca + cb);
-                const double sB = ss[(4 + 2 * gi + cb) * 16 + (lane & 15)];
+                const double sB = ss[(4 + 2 * gi + cb) * 16 + (lane & 15)] * wlow;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    double v = 0.0;
+                    double v;
+                    if constexpr (BF) {
+                        v = 0.0;
 #pragma unroll
-                    for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r] * PlaneMath<BF>::weight(q);
+                        for (int q = S - 1; q >= 0; --q) v += (double)iacc[c][q][r];
+                    } else if constexpr (PAIR) {
+                        v = level_sum<S>(iacc[c], r);
+                    } else {
+                        v = level_horner<S>(iacc[c], r);
+                    }
                     const double sA = ss[(2 * gi + ca) * 16 + row0 + r];
                     const double f = v * sA * sB;
                     if (gi == 0) {
@@ -948,17 +1006,14 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
             }
         };
         if (gi == 0) *reinterpret_cast<dbl2 *>(sst + 2 * lane) = sreg;
-        recombine(sst, false);
+        if (pairable) recombine(sst, false, std::true_type{});
+        else recombine(sst, false, std::false_type{});
         // a compiler fence (stash is not restrict-qualified, so it covers the
         // stash): without it the stores are forwarded to the reloads below and
         // group 0's values stay in registers through group 1 after all
         if (gi == 0) asm volatile("" ::: "memory");
         if constexpr (T8) {
             if (gi == 0) *reinterpret_cast<dbl2 *>(sst + 128 + 2 * lane) = sreg_t;
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int q = 0; q < S; ++q) iacc[c][q] = acc_t{0, 0, 0, 0};
             // (the scheduling barrier keeps these loads from being hoisted above the
             // recombination, where their registers would add to its own)
             __builtin_amdgcn_sched_barrier(0);
@@ -979,28 +1034,32 @@ __device__ __forceinline__ void split_wave(dbl4 (&facc)[8], const __amdgpu_buffe
                 xb1[2] = buffer_load_i8x16(rd, voff8, ob1 + plane + k1);
 #pragma unroll
                 for (int lv = 0; lv < 3; ++lv) {
-                    iacc[0][lv] = PlaneMath<BF>::mfma(xa0[0], xb0[lv], iacc[0][lv]);
-                    iacc[1][lv] = PlaneMath<BF>::mfma(xa1[0], xb0[lv], iacc[1][lv]);
-                    iacc[2][lv] = PlaneMath<BF>::mfma(xa0[0], xb1[lv], iacc[2][lv]);
-                    iacc[3][lv] = PlaneMath<BF>::mfma(xa1[0], xb1[lv], iacc[3][lv]);
+                    iacc[0][lv] = PlaneMath<BF>::mfma(xa0[0], xb0[lv], zacc);
+                    iacc[1][lv] = PlaneMath<BF>::mfma(xa1[0], xb0[lv], zacc);
+                    iacc[2][lv] = PlaneMath<BF>::mfma(xa0[0], xb1[lv], zacc);
+                    iacc[3][lv] = PlaneMath<BF>::mfma(xa1[0], xb1[lv], zacc);
                 }
                 iacc[0][2] = PlaneMath<BF>::mfma(xa0[1], xb0[0], iacc[0][2]);
                 iacc[1][2] = PlaneMath<BF>::mfma(xa1[1], xb0[0], iacc[1][2]);
                 iacc[2][2] = PlaneMath<BF>::mfma(xa0[1], xb1[0], iacc[2][2]);
                 iacc[3][2] = PlaneMath<BF>::mfma(xa1[1], xb1[0], iacc[3][2]);
             } else {
-                for (int kx = 0; kx < sg.nkx; ++kx) {
+                SPLIT_LOAD(xa0, xa1, xb0, xb1, nkv)
+                SPLIT_MFMA_Z(xa0, xa1, xb0, xb1, true)
+                for (int kx = 1; kx < sg.nkx; ++kx) {
                     SPLIT_LOAD(xa0, xa1, xb0, xb1, nkv + kx)
                     SPLIT_MFMA(xa0, xa1, xb0, xb1)
                 }
             }
-            recombine(sst + 128, true);
+            if (pairable) recombine(sst + 128, true, std::true_type{});
+            else recombine(sst + 128, true, std::false_type{});
             if (gi == 0) asm volatile("" ::: "memory");
             // group 1's first loads stay below this group's last recombination
             __builtin_amdgcn_sched_barrier(0);
         }
 #undef SPLIT_LOAD
 #undef SPLIT_MFMA
+#undef SPLIT_MFMA_Z
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -1506,70 +1565,123 @@ __device__ __forceinline__ RowPair read_rowpair(const int4 *p)
 }
 
 
-// One lane's row r of a digit-product tile: V = sum_L acc_L 2^(7 (S-1-L)), an
-// integer below 2^53 and so exact in FP64, equal to sum_L acc_L w(L) / w(S-1).
-// Adjacent levels are first combined in INT32, (acc_L << 7) + acc_(L+1), exact
-// for at most kIntPairMaxK terms per level sum (the tile's shape limit): one
-// conversion and one FMA fewer per pair than converting every level.
-template <int S>
-__device__ __forceinline__ double level_sum(const int4v (&acc)[S], int r)
-{
-    double u;
-    int q;
-    if constexpr (S % 2 == 1) {
-        u = (double)acc[0][r];
-        q = 1;
-    } else {
-        u = (double)((acc[0][r] << 7) + acc[1][r]);
-        q = 2;
-    }
-#pragma unroll
-    for (; q + 1 < S; q += 2) u = __builtin_fma(u, 16384.0, (double)((acc[q][r] << 7) + acc[q + 1][r]));
-    return u;
-}
-
 // The K = nvir digit products of one group over row blocks 0 and, with TWO,
 // block 1 (the caller picks the instantiation once per item: a branch around the
 // second block's products inside the loop made the compiler copy every
 // accumulator at each join).  The first product of each level starts from zero
 // (no separate clearing of the accumulators).
+//
+// Software pipeline at two waves per SIMD (one other wave to cover a load's
+// latency): the B planes, used by both blocks, are double-buffered and loaded a
+// whole K step ahead; each block's A planes have one buffer, reloaded for the
+// next step as soon as that block's products of this step are issued, so they
+// arrive while the other block's 24 products run.  With one block the A planes
+// are double-buffered too.  No step is loaded twice.  The INT32 sums are exact,
+// so the order of the products does not change a bit of the result.
 template <int S, bool TWO>
 __device__ __forceinline__ void rowpair_nvir(int4v (&iacc)[2][4][S], const __amdgpu_buffer_rsrc_t rd,
                                              const uint32_t (&oa)[2][2], const uint32_t (&ob)[2],
                                              uint32_t plane, uint32_t voff8, int nkv)
 {
-    constexpr int NM = TWO ? 2 : 1;
     const int4v z = {0, 0, 0, 0};
-    int4v a[2][2][S], b[2][S];
-#define RP_LOADS(kstep)                                                                          \
+    int4v bx[2][S], by[2][S], a0[2][S], a1[2][S];
+#define RP_LOAD_B(B_, kstep)                                                                     \
     _Pragma("unroll") for (int q = 0; q < S; ++q) {                                             \
         const uint32_t ko_ = (uint32_t)(kstep) * 1024 + q * plane;                               \
-        b[0][q] = buffer_load_i8x16(rd, voff8, ob[0] + ko_);                                     \
-        b[1][q] = buffer_load_i8x16(rd, voff8, ob[1] + ko_);                                     \
-        _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                        \
-            a[m][0][q] = buffer_load_i8x16(rd, voff8, oa[m][0] + ko_);                           \
-            a[m][1][q] = buffer_load_i8x16(rd, voff8, oa[m][1] + ko_);                           \
-        }                                                                                        \
+        B_[0][q] = buffer_load_i8x16(rd, voff8, ob[0] + ko_);                                    \
+        B_[1][q] = buffer_load_i8x16(rd, voff8, ob[1] + ko_);                                    \
     }
-#define RP_MFMA(first)                                                                           \
-    _Pragma("unroll") for (int m = 0; m < NM; ++m)                                              \
+#define RP_LOAD_A(A_, m, kstep)                                                                  \
+    _Pragma("unroll") for (int q = 0; q < S; ++q) {                                             \
+        const uint32_t ko_ = (uint32_t)(kstep) * 1024 + q * plane;                               \
+        A_[0][q] = buffer_load_i8x16(rd, voff8, oa[m][0] + ko_);                                 \
+        A_[1][q] = buffer_load_i8x16(rd, voff8, oa[m][1] + ko_);                                 \
+    }
+#define RP_MFMA(m, A_, B_, first)                                                                \
     _Pragma("unroll") for (int sa = 0; sa < S; ++sa) {                                          \
         _Pragma("unroll") for (int sb = 0; sb < S - sa; ++sb) {                                 \
             const int lv = sa + sb;                                                              \
             const bool z_ = (first) && sa == 0;                                                  \
-            iacc[m][0][lv] = PlaneMath<false>::mfma(a[m][0][sa], b[0][sb], z_ ? z : iacc[m][0][lv]); \
-            iacc[m][1][lv] = PlaneMath<false>::mfma(a[m][1][sa], b[0][sb], z_ ? z : iacc[m][1][lv]); \
-            iacc[m][2][lv] = PlaneMath<false>::mfma(a[m][0][sa], b[1][sb], z_ ? z : iacc[m][2][lv]); \
-            iacc[m][3][lv] = PlaneMath<false>::mfma(a[m][1][sa], b[1][sb], z_ ? z : iacc[m][3][lv]); \
+            iacc[m][0][lv] = PlaneMath<false>::mfma(A_[0][sa], B_[0][sb], z_ ? z : iacc[m][0][lv]); \
+            iacc[m][1][lv] = PlaneMath<false>::mfma(A_[1][sa], B_[0][sb], z_ ? z : iacc[m][1][lv]); \
+            iacc[m][2][lv] = PlaneMath<false>::mfma(A_[0][sa], B_[1][sb], z_ ? z : iacc[m][2][lv]); \
+            iacc[m][3][lv] = PlaneMath<false>::mfma(A_[1][sa], B_[1][sb], z_ ? z : iacc[m][3][lv]); \
         }                                                                                        \
     }
-    RP_LOADS(0)
-    RP_MFMA(true)
-    for (int ks = 1; ks < nkv; ++ks) {
-        RP_LOADS(ks)
-        RP_MFMA(false)
-    }
+    if constexpr (S > 3) {
+        // four planes: a second B buffer would spill (255 VGPRs); one buffer,
+        // each step's planes loaded at its start, B and A interleaved per plane
+        constexpr int NM = TWO ? 2 : 1;
+#define RP_LOADS(kstep)                                                                          \
+        _Pragma("unroll") for (int q = 0; q < S; ++q) {                                         \
+            const uint32_t ko_ = (uint32_t)(kstep) * 1024 + q * plane;                           \
+            bx[0][q] = buffer_load_i8x16(rd, voff8, ob[0] + ko_);                                \
+            bx[1][q] = buffer_load_i8x16(rd, voff8, ob[1] + ko_);                                \
+            _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                    \
+                (m ? a1 : a0)[0][q] = buffer_load_i8x16(rd, voff8, oa[m][0] + ko_);              \
+                (m ? a1 : a0)[1][q] = buffer_load_i8x16(rd, voff8, oa[m][1] + ko_);              \
+            }                                                                                    \
+        }
+        RP_LOADS(0)
+        RP_MFMA(0, a0, bx, true)
+        if constexpr (TWO) RP_MFMA(1, a1, bx, true)
+        for (int ks = 1; ks < nkv; ++ks) {
+            RP_LOADS(ks)
+            RP_MFMA(0, a0, bx, false)
+            if constexpr (TWO) RP_MFMA(1, a1, bx, false)
+        }
 #undef RP_LOADS
+    } else if constexpr (TWO) {
+        RP_LOAD_B(bx, 0)
+        RP_LOAD_A(a0, 0, 0)
+        RP_LOAD_A(a1, 1, 0)
+        if (nkv > 1) RP_LOAD_B(by, 1)
+        RP_MFMA(0, a0, bx, true)
+        if (nkv > 1) RP_LOAD_A(a0, 0, 1)
+        RP_MFMA(1, a1, bx, true)
+        if (nkv > 1) RP_LOAD_A(a1, 1, 1)
+        int ks = 1;
+        for (; ks + 1 < nkv; ks += 2) {
+            RP_LOAD_B(bx, ks + 1)
+            RP_MFMA(0, a0, by, false)
+            RP_LOAD_A(a0, 0, ks + 1)
+            RP_MFMA(1, a1, by, false)
+            RP_LOAD_A(a1, 1, ks + 1)
+            const bool more = ks + 2 < nkv;
+            if (more) RP_LOAD_B(by, ks + 2)
+            RP_MFMA(0, a0, bx, false)
+            if (more) RP_LOAD_A(a0, 0, ks + 2)
+            RP_MFMA(1, a1, bx, false)
+            if (more) RP_LOAD_A(a1, 1, ks + 2)
+        }
+        if (ks < nkv) {
+            RP_MFMA(0, a0, by, false)
+            RP_MFMA(1, a1, by, false)
+        }
+    } else {
+        // one block: ping-pong of both operands (a0/bx and a1/by), two steps per trip
+        RP_LOAD_B(bx, 0)
+        RP_LOAD_A(a0, 0, 0)
+        if (nkv > 1) {
+            RP_LOAD_B(by, 1)
+            RP_LOAD_A(a1, 0, 1)
+        }
+        RP_MFMA(0, a0, bx, true)
+        int ks = 1;
+        for (; ks + 1 < nkv; ks += 2) {
+            RP_LOAD_B(bx, ks + 1)
+            RP_LOAD_A(a0, 0, ks + 1)
+            RP_MFMA(0, a1, by, false)
+            if (ks + 2 < nkv) {
+                RP_LOAD_B(by, ks + 2)
+                RP_LOAD_A(a1, 0, ks + 2)
+            }
+            RP_MFMA(0, a0, bx, false)
+        }
+        if (ks < nkv) RP_MFMA(0, a1, by, false)
+    }
+#undef RP_LOAD_B
+#undef RP_LOAD_A
 #undef RP_MFMA
 }
 
