@@ -1714,8 +1714,14 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
     const int gi = w & 1, mir = w >> 1;
     const int L0 = gi ? LJT : LJ, L1 = gi ? LKT : LK, Q0 = gi ? R1T : R1, Q1 = gi ? R3T : R3;
     const int A0 = mir ? Q0 : L0, A1 = mir ? Q1 : L1, B0 = mir ? L0 : Q0, B1 = mir ? L1 : Q1;
+    // The item's small inputs are fetched into registers first and written to
+    // LDS only after the digit loop, so their latency overlaps the loop's first
+    // loads instead of preceding them behind a workgroup barrier (the energy
+    // epilogue's barrier orders the vector table; each wave reads only its own
+    // scales).
     // row scales: lane l < 48 fetches elements 2l, 2l+1 of the 96: A side [a][m][16],
     // then B side [b][16]
+    dbl2 sreg = {0.0, 0.0}, sreg_t = {0.0, 0.0}, vreg = {0.0, 0.0};
     if (lane < 48) {
         const int e = 2 * lane;
         int pid, row;
@@ -1727,9 +1733,8 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
             row = ci * 16 + (e & 15);
         }
         const int64_t so = (int64_t)pid * nvp + row;
-        *reinterpret_cast<dbl2 *>(&lds_s[w][e]) = gload2(jd.scales + so);
-        if (T8)
-            *reinterpret_cast<dbl2 *>(&lds_s[w][96 + e]) = gload2(jd.scales + (int64_t)sg.npanels * nvp + so);
+        sreg = gload2(jd.scales + so);
+        if (T8) sreg_t = gload2(jd.scales + (int64_t)sg.npanels * nvp + so);
     }
     // energy vectors: 7 x 3 blocks x 16 = 336 doubles, two per thread of waves 0..2
     if (TENGY && threadIdx.x < 168) {
@@ -1740,9 +1745,8 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
         for (int r = 0; r < 6; ++r) row = q == r + 1 ? uniform(tk.vrow[r]) : row;
         const double *src = q == 0 ? jd.ev : jd.vec6 + (int64_t)row * nvp;
         const int xb = blk == 0 ? r0 : blk == 1 ? r1 : ci;
-        *reinterpret_cast<dbl2 *>(&vst[e]) = gload2(src + xb * 16 + i);
+        vreg = gload2(src + xb * 16 + i);
     }
-    __syncthreads();
     const __amdgpu_buffer_rsrc_t rd = buffer_rsrc(reinterpret_cast<const double *>(jd.digits),
                                                   (uint32_t)(sg.npanels * NP * sg.slice_bytes));
     const uint32_t plane = (uint32_t)sg.slice_bytes;
@@ -1759,6 +1763,45 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
     int4v iacc[2][4][NP];
     if (two) rowpair_nvir<NP, true>(iacc, rd, oa, ob, plane, voff8, sg.nkv);
     else rowpair_nvir<NP, false>(iacc, rd, oa, ob, plane, voff8, sg.nkv);
+    if (lane < 48) {
+        *reinterpret_cast<dbl2 *>(&lds_s[w][2 * lane]) = sreg;
+        if (T8) *reinterpret_cast<dbl2 *>(&lds_s[w][96 + 2 * lane]) = sreg_t;
+    }
+    if (TENGY && threadIdx.x < 168) *reinterpret_cast<dbl2 *>(&vst[2 * threadIdx.x]) = vreg;
+    // LDS operations of one wave complete in order; the fence keeps the compiler
+    // from moving the scale reads below above these writes
+    asm volatile("" ::: "memory");
+    // folded occupied digit planes (k_split_pack, nocc <= 32): A [d1|d2], [d3|0];
+    // B [d1|0], [d2|d1], [d3|d2].  Fetched here (after the staged values
+    // above, whose stores would otherwise wait for them), so they arrive while
+    // the K = nvir products are recombined and stashed.
+    int4v obx[2][3], oax[2][2][2];
+    if constexpr (T8) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) obx[p][q] = int4v{0, 0, 0, 0};
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) oax[m][p][0] = oax[m][p][1] = int4v{0, 0, 0, 0};
+    }
+    if (T8 && sg.fold) {
+        const uint32_t k0 = (uint32_t)sg.nkv * 1024, k1 = k0 + 1024;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            obx[p][0] = buffer_load_i8x16(rd, voff8, ob[p] + 2 * plane + k0);
+            obx[p][1] = buffer_load_i8x16(rd, voff8, ob[p] + k1);
+            obx[p][2] = buffer_load_i8x16(rd, voff8, ob[p] + plane + k1);
+        }
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                oax[m][p][0] = buffer_load_i8x16(rd, voff8, oa[m][p] + k0);
+                oax[m][p][1] = buffer_load_i8x16(rd, voff8, oa[m][p] + plane + k0);
+            }
+    }
 
     const double *sst = lds_s[w];
     double *xw = &X[w][0][0][0][0];  // this wave's [m][slot][row][lane]
@@ -1798,24 +1841,11 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
         asm volatile("" ::: "memory");
         const int4v z = {0, 0, 0, 0};
         if (sg.fold) {
-            // folded slots (k_split_pack, nocc <= 32): A [d1|d2], [d3|0]; B [d1|0],
-            // [d2|d1], [d3|d2]; levels as split_wave's
-            const uint32_t k0 = (uint32_t)sg.nkv * 1024, k1 = k0 + 1024;
-            int4v bx[2][3];
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                bx[p][0] = buffer_load_i8x16(rd, voff8, ob[p] + 2 * plane + k0);
-                bx[p][1] = buffer_load_i8x16(rd, voff8, ob[p] + k1);
-                bx[p][2] = buffer_load_i8x16(rd, voff8, ob[p] + plane + k1);
-            }
+            // the folded planes fetched after the K = nvir loop; levels as split_wave's
+            const int4v (&bx)[2][3] = obx;
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
-                int4v ax[2][2];
-#pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    ax[p][0] = buffer_load_i8x16(rd, voff8, oa[m][p] + k0);
-                    ax[p][1] = buffer_load_i8x16(rd, voff8, oa[m][p] + plane + k0);
-                }
+                const int4v (&ax)[2][2] = oax[m];
 #pragma unroll
                 for (int lv = 0; lv < 3; ++lv) {
                     iacc[m][0][lv] = PlaneMath<false>::mfma(ax[0][0], bx[0][lv], z);
