@@ -353,15 +353,26 @@ def split_sweep_report(cc, steps, warmup):
                     "fp64": {"value": flops / (f["ms_per_step"] * 1e-3) / 1e12, "E": E}, "split": []}
             levels = [(3, 1)] + [(S, 0) for S in range(3, 8)]  # (slices, occ_digits)
             for S, occ in levels:
-                r = time_form(job, trip, steps, warmup, mode="ozaki_int8", slices=S, order=1,
-                              graph=True, occ_digits=occ)
-                e = r["res"]["emp4"] + r["res"]["emp5"]
-                case["split"].append({"slices": S, "occ_digits": occ,
-                                      "value": flops / (r["ms_per_step"] * 1e-3) / 1e12,
-                                      "ms_per_step": r["ms_per_step"], "rel_dE": abs(e - E) / abs(E)})
-                print("bench: split sweep %s S=%d%s %.1f TFLOP/s rel dE %.2e" % (
-                    case["inputs"], S, " occ_digits" if occ else "", case["split"][-1]["value"],
-                    case["split"][-1]["rel_dE"]), file=sys.stderr, flush=True)
+                # the block-pair tile for every S; the row-pair tile where it runs
+                # (S <= 4), whose E[T] is bitwise the same
+                tiles = (0, 2) if S <= 4 else (0,)
+                row = {"slices": S, "occ_digits": occ}
+                for v in tiles:
+                    r = time_form(job, trip, steps, warmup, mode="ozaki_int8", slices=S, order=1,
+                                  graph=True, occ_digits=occ, variant=v)
+                    e = r["res"]["emp4"] + r["res"]["emp5"]
+                    val = flops / (r["ms_per_step"] * 1e-3) / 1e12
+                    if v == 0:
+                        row.update({"value": val, "ms_per_step": r["ms_per_step"],
+                                    "rel_dE": abs(e - E) / abs(E)})
+                    else:
+                        row["row_pair"] = {"value": val, "ms_per_step": r["ms_per_step"],
+                                           "rel_dE": abs(e - E) / abs(E)}
+                case["split"].append(row)
+                print("bench: split sweep %s S=%d%s %.1f TFLOP/s%s rel dE %.2e" % (
+                    case["inputs"], S, " occ_digits" if occ else "", row["value"],
+                    " (row-pair tile %.1f)" % row["row_pair"]["value"] if "row_pair" in row else "",
+                    row["rel_dE"]), file=sys.stderr, flush=True)
             for mode in ("bf16x3", "fp32"):
                 r = time_form(job, trip, steps, warmup, mode=mode, order=1, graph=True)
                 e = r["res"]["emp4"] + r["res"]["emp5"]
