@@ -11,6 +11,11 @@ config0.npz -- the 8-occupied / 40-virtual case, 120 triples (i >= j >= k, one
   switch (eaijk + tengy + the i != k rule, omp/ccsd_trpdrv_omp_cbody.c:121-142).
 config1_head.npz -- the 30-occupied / 300-virtual case: reference terms of the
   first 6 triples of each k block.
+fp32_ref.npz -- the FP32 reference body (gpu-omp/ccsd_trpdrv_omp_cbody.c built
+  with its SKIP_DGEMM switch) on FP32-rounded inputs and FP32-rounded oracle f
+  matrices: per triple its emp4 and emp5 for the two cases of
+  tests/test_gpu_fp32.py, which compares with them when oracle/_ref is absent.
+  `python tests/golden/make_golden.py fp32` writes this file only.
 Inputs are not stored: they are regenerated from their seeds, and the fixture
 keeps checksums of them.
 """
@@ -70,8 +75,41 @@ def reference_terms(arrays, nocc, nvir, triples, Lt, Lb):
     return terms, np.array([e4, e5]), keep_f
 
 
+FP32_CASES = ((8, 40, 8, 808), (5, 37, 2, 808))  # nocc, nvir, kchunk, input seed
+
+
+def fp32_reference(cc):
+    """Per-triple (emp4, emp5) of the FP32 reference body for FP32_CASES."""
+    L = oracle.ref_cbody_f32_lib()
+    if L is None:
+        raise SystemExit("oracle/_ref not built: run `make -C oracle` with the reference mounted")
+    out = {}
+    for c, (nocc, nvir, kchunk, seed) in enumerate(FP32_CASES):
+        arrays = cc.make_inputs(nocc, nvir, kchunk, seed)
+        a32 = {n: np.asarray(v, dtype=np.float32).astype(np.float64) for n, v in arrays.items()}
+        trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+        e45 = np.zeros((len(trip), 2))
+        for t, (a, i, j, k, klo) in enumerate(trip.tolist()):
+            f = oracle.fmats(a32, nocc, nvir, k, klo)
+            fcm = [np.ascontiguousarray(f[m].T).astype(np.float32) for m in range(8)]
+            e45[t] = oracle.ref_cbody_f32_glue(L, fcm, a32, 0, nocc, nvir, a, i, j, k, klo)
+        names, sums, digests = checksums(arrays)
+        out["case%d_shape" % c] = np.array([nocc, nvir, kchunk, seed])
+        out["case%d_triples" % c] = trip
+        out["case%d_e45" % c] = e45
+        out["case%d_names" % c] = np.array(names)
+        out["case%d_digests" % c] = digests
+    np.savez_compressed(os.path.join(HERE, "fp32_ref.npz"), **out)
+    print("wrote fp32_ref.npz")
+
+
 def main():
     cc = load_package()
+    if len(sys.argv) > 1 and sys.argv[1] == "fp32":
+        oracle.blas_select(0)
+        omp_threads(1)
+        fp32_reference(cc)
+        return
     Lt, Lb = oracle.ref_tengy_lib(), oracle.ref_cbody_skipdgemm_lib()
     if Lt is None or Lb is None:
         raise SystemExit("oracle/_ref not built: run `make -C oracle` with the reference mounted")
@@ -97,6 +135,7 @@ def main():
                         terms=terms, names=np.array(names), sums=sums, digests=digests,
                         provider=np.array(oracle.blas_name()))
     print("wrote config0.npz, config1_head.npz with", oracle.blas_name())
+    fp32_reference(cc)
 
 
 if __name__ == "__main__":

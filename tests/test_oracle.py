@@ -15,6 +15,7 @@
 """
 import ctypes
 import ctypes.util
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -366,3 +367,28 @@ def test_tengy_constant_denominator_closed_form(orc, cc, one_thread, nvir):
     if nvir > 1:
         wrong = _closed_form_tengy(F, v, delta, swap_one=True)
         assert max(abs(g - w) for g, w in zip(got, wrong)) > 1e-3 * abs(scale)
+
+
+def test_fp32_golden_matches_reference_build(cc, orc):
+    """tests/golden/fp32_ref.npz (the FP32 reference body's per-triple emp4 and
+    emp5, used by tests/test_gpu_fp32.py where oracle/_ref is absent) is
+    reproduced bit for bit by the reference build, under the settings that wrote
+    it (the oracle's first DGEMM provider, one thread)."""
+    L = orc.ref_cbody_f32_lib()
+    if L is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fp32_ref.npz"),
+                allow_pickle=False)
+    orc.blas_select(0)
+    for case in (0, 1):
+        nocc, nvir, kchunk, seed = (int(x) for x in g["case%d_shape" % case])
+        arrays = cc.make_inputs(nocc, nvir, kchunk, seed)
+        a32 = {n: np.asarray(v, dtype=np.float32).astype(np.float64) for n, v in arrays.items()}
+        trip = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))
+        assert np.array_equal(trip, g["case%d_triples" % case])
+        for t, (a, i, j, k, klo) in enumerate(trip.tolist()):
+            f = orc.fmats(a32, nocc, nvir, k, klo)
+            fcm = [np.ascontiguousarray(f[m].T).astype(np.float32) for m in range(8)]
+            got = orc.ref_cbody_f32_glue(L, fcm, a32, 0, nocc, nvir, a, i, j, k, klo)
+            assert tuple(got) == tuple(g["case%d_e45" % case][t]), (case, t)
+
