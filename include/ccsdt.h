@@ -478,6 +478,14 @@ double ccsdt_debug_load_rate(int bytes, int to_lds);
  * and MFMAs, the same integer products, bitwise the same results).  For tests
  * that compare the two layouts.  No counterpart in the reference. */
 void ccsdt_debug_occ_fold(int on);
+/* Position of element (b, c) of f matrix m (0..7, body order f1n, f1t, ..,
+ * f4t) inside one triple's 8 nv^2 doubles: in the device layout of the separate
+ * energy pass (tiled = 1: 32x32 tile pairs one after the other, each pair's 16
+ * tiles, the (b,c) tile and its (c,b) mirror of all eight matrices, in one
+ * contiguous run) or in the reference's column-major layout (tiled = 0:
+ * m nv^2 + b + c nv).  -1 for indices out of range.  Needs no device.  No
+ * counterpart in the reference: for tests of the layout. */
+int64_t ccsdt_debug_f_index(int m, int b, int c, int nv, int tiled);
 /* Host-to-host copies of n buffers (dst[q] <- src[q], bytes[q] bytes) through the
  * copy pool the synchronous drop-ins stage their arrays and f matrices with (the
  * calling thread plus CCSDT_COPY_THREADS workers, default 4).  Needs no device.

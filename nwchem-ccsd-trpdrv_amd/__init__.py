@@ -191,6 +191,7 @@ def lib():
             "ccsdt_debug_load_rate": (ctypes.c_double, [ctypes.c_int, ctypes.c_int]),
             "ccsdt_debug_host_copy": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
             "ccsdt_debug_occ_fold": (None, [ctypes.c_int]),
+            "ccsdt_debug_f_index": (i64, [ctypes.c_int] * 5),
             "ccsdt_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
             "ccsdt_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(JobDesc), vp, i64,
                                                    ctypes.POINTER(i64)]),

@@ -1014,6 +1014,7 @@ void fill_tables(ccsdt_job j, const SplitGeom &sg, int64_t n, bool with_f, bool 
     sgm.ntasks = (int)n;
     sgm.order = pmajor ? j->d_order : nullptr;
     sgm.pmajor = pmajor ? 1 : 0;
+    sgm.ftiled = 0;
     WorkHdr &h = *j->h_hdr;
     memset(&h, 0, sizeof(h));  // also zeroes the persistent item counters
     h.total = (int)(n * tile_items(j->g, rows));
@@ -1219,6 +1220,10 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
         HIP_TRY(hipEventRecord(E[1], j->st));
         launches = 1;
         fill_tables(j, sg, n, true, false, false);
+        // f in the tiled layout (f_index) unless it leaves the device, where the
+        // caller gets the reference's column-major matrices
+        const bool ftiled = !fstaged && !fouts;
+        j->h_seg->ftiled = ftiled ? 1 : 0;
         for (int64_t t0 = 0; t0 < n; t0 += chunk) {
             const int m = (int)std::min<int64_t>(chunk, n - t0);
             j->h_seg->tasks = j->d_tasks + t0;
@@ -1232,7 +1237,7 @@ int job_run(ccsdt_job j, const int *trip, int64_t n, const ccsdt_run_opts *opts,
             HIP_TRY(launch_fused(j->d_jobdev, j->d_seg, j->d_hdr, m * g.npairs, false, variant,
                                  8 * vv, false, true, j->st));
             HIP_TRY(hipEventRecord(E[5], j->st));
-            HIP_TRY(launch_tengy(j->d_f, 8 * vv, g, j->d_tasks + t0, m, j->d_ev, j->d_vec6,
+            HIP_TRY(launch_tengy(j->d_f, 8 * vv, ftiled, g, j->d_tasks + t0, m, j->d_ev, j->d_vec6,
                                  j->d_partial + t0 * tnp * 4, j->st));
             HIP_TRY(hipEventRecord(E[6], j->st));
             launches += 2;
@@ -1725,7 +1730,7 @@ void tengy_impl(const char *who, const double *f1n, const double *f1t, const dou
                 j->h_tasks[0] = Task{0, 0, 0, 0, 1, (float)eaijk, eaijk, {0, 1, 2, 3, 4, 5}, 0};
                 HIP_TRY(hipMemcpyAsync(j->d_tasks, j->h_tasks, sizeof(Task), hipMemcpyHostToDevice,
                                        j->st));
-                HIP_TRY(launch_tengy(j->d_f, 8 * vv, j->g, j->d_tasks, 1, j->d_ev, j->d_vec6,
+                HIP_TRY(launch_tengy(j->d_f, 8 * vv, false, j->g, j->d_tasks, 1, j->d_ev, j->d_vec6,
                                      j->d_partial, j->st));
                 HIP_TRY(launch_reduce_terms(j->d_partial, tengy_npairs(j->g), 1, j->d_terms, j->st));
                 HIP_TRY(hipMemcpyAsync(j->h_out, j->d_terms, 4 * sizeof(double),
@@ -2035,6 +2040,7 @@ int server_launch_tick(ccsdt_server s, bool *launched)
             sg.ntasks = (int)sh.m;
             sg.order = nullptr;
             sg.pmajor = 1;  // block-pair-major over the share: workgroups in flight share panels
+            sg.ftiled = 0;
             fitems += sh.m * np;
             ftasks += sh.m;
             sj->next += sh.m;
@@ -2672,6 +2678,12 @@ double ccsdt_debug_load_rate(int bytes, int to_lds)
 void ccsdt_debug_occ_fold(int on)
 {
     g_occ_fold.store(on ? 1 : 0);
+}
+
+int64_t ccsdt_debug_f_index(int m, int b, int c, int nv, int tiled)
+{
+    if (nv < 1 || m < 0 || m > 7 || b < 0 || c < 0 || b >= nv || c >= nv) return -1;
+    return f_index(m, b, c, nv, tiled != 0);
 }
 
 int ccsdt_debug_host_copy(void *const *dst, const void *const *src, const int64_t *bytes, int n)

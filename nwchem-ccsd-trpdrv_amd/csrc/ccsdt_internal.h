@@ -128,7 +128,35 @@ struct Segment {
     int item0;           // first work item of the segment
     int ntasks;
     int pmajor;
+    int ftiled;          // fout in the tiled layout of f_index (the separate energy
+                         // pass); 0: column-major, as the reference's f1n..f4t
+    int pad;
 };
+
+// Element (b, c) of f matrix m (body order f1n, f1t, ..., f4t) inside one
+// triple's 8 nv^2 doubles.  Column-major, m nv^2 + b + c nv, is the reference's
+// layout (f1n..f4t).  The tiled layout serves the separate energy pass, whose
+// work unit is a 32x32 tile pair: the tile D at (block bi, block ci), bi >= ci,
+// and its mirror M at (ci, bi), of all eight matrices.  It stores the pairs one
+// after the other (bi = 0, 1, ..., ci = 0..bi), each as D of matrices 0..7 then
+// M of matrices 0..7 (a diagonal pair has D only), each tile column-major; the
+// last block row and column are narrower (w = nv - 32 (nb - 1)).  So a tile pair
+// is one contiguous run of at most 128 KiB, and the 8 nv^2 doubles hold every
+// element once: the pairs before row bi take 8 x 1024 bi^2 doubles, those
+// before (bi, ci) in its row 8 x 64 w(bi) ci more.
+__host__ __device__ inline int64_t f_index(int m, int b, int c, int nv, bool tiled)
+{
+    if (!tiled) return (int64_t)m * nv * nv + b + (int64_t)c * nv;
+    const int tb = b >> 5, tc = c >> 5;
+    const int bi = tb > tc ? tb : tc, ci = tb > tc ? tc : tb;
+    const int wbi = nv - 32 * bi < 32 ? nv - 32 * bi : 32;
+    const int wci = nv - 32 * ci < 32 ? nv - 32 * ci : 32;
+    const int64_t base = 8 * ((int64_t)1024 * bi * bi + (int64_t)64 * wbi * ci);
+    const int64_t sz = (int64_t)wbi * wci;
+    if (tb >= tc)  // D (rows in block bi = tb, columns in block ci = tc)
+        return base + m * sz + (b - 32 * tb) + (int64_t)(c - 32 * tc) * wbi;
+    return base + (8 + m) * sz + (b - 32 * tb) + (int64_t)(c - 32 * tc) * wci;  // M
+}
 
 // Size of the current work list, read by the kernels (so a persistent launch
 // of fixed shape can be replayed on new lists).
@@ -154,7 +182,7 @@ hipError_t launch_pack(const PanelDesc *d_desc, int npanels, const Geom &g, bool
 hipError_t launch_fused(const JobDev *jobs, const Segment *segs, WorkHdr *hdr, int grid,
                         bool persist, int variant, int64_t fstride, bool tengy, bool writef,
                         hipStream_t st);
-hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
+hipError_t launch_tengy(const double *f, int64_t fstride, bool tiled, const Geom &g, const Task *tasks,
                         int ntasks, const double *ev, const double *vec6, double *partial,
                         hipStream_t st);
 // One triple's energy-pass inputs in the caller's device arrays: f1n, f1t, ...,

@@ -402,7 +402,7 @@ __device__ __forceinline__ void fused_mainloop(dbl4 (&acc)[16], const __amdgpu_b
 template <bool TENGY, bool WRITEF>
 __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
                                            int bi, int ci, int lane, double *__restrict__ part4,
-                                           double *__restrict__ fo)
+                                           double *__restrict__ fo, bool ftiled)
 {
     const Geom &g = jd.g;
     const double *__restrict__ ev = jd.ev;
@@ -455,17 +455,16 @@ __device__ __forceinline__ void fused_item(const JobDev &jd, const Task &tk, con
     for (int q = 0; q < 6; ++q) vo[q] = uniform(tk.vrow[q] * nvp);
     const int cx = ci * 16 + col;
     if (WRITEF) {
-        const int64_t vv = (int64_t)nv * nv;
         const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};  // product -> body order f1n,f1t,...
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int bx = bi * 16 + row0 + 4 * r;
             if (bx < nv && cx < nv) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) fo[mat[q] * vv + bx + (int64_t)cx * nv] = acc[q][r];
+                for (int q = 0; q < 8; ++q) fo[f_index(mat[q], bx, cx, nv, ftiled)] = acc[q][r];
                 if (bi != ci) {
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) fo[mat[q] * vv + cx + (int64_t)bx * nv] = acc[8 + q][r];
+                    for (int q = 0; q < 8; ++q) fo[f_index(mat[q], cx, bx, nv, ftiled)] = acc[8 + q][r];
                 }
             }
         }
@@ -529,7 +528,7 @@ __global__ __launch_bounds__(64) void k_fused(const JobDev *__restrict__ jobs,
         const short2 pr = read_pair(jd.pairs + p);
         fused_item<TENGY, WRITEF>(jd, tk, panel_pairs(tk), uniform(pr.x), uniform(pr.y), threadIdx.x,
                                   sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
-                                  WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
+                                  WRITEF ? sg.fout + (int64_t)t * fstride : nullptr, sg.ftiled != 0);
     }
 }
 
@@ -1302,7 +1301,7 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
                                               const typename Vec4<T>::type (&facc)[8], int bi, int ci,
                                               int w, int lane, double *__restrict__ part4,
                                               double *__restrict__ fo, void *lds, dbl2 vreg,
-                                              double *__restrict__ vst)
+                                              double *__restrict__ vst, bool ftiled)
 {
     // the workgroup's 16 KiB exchange buffer (each wave's half first held its
     // stash, consumed before the products were finished)
@@ -1314,7 +1313,6 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
     const bool diag = (bi == ci);
     const int nv = g.nvir;
     if (WRITEF) {
-        const int64_t vv = (int64_t)nv * nv;
         const int mat[8] = {0, 2, 4, 6, 1, 3, 5, 7};
         if (w == 0 || bi != ci) {
 #pragma unroll
@@ -1323,8 +1321,8 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
                 if (bx < nv && cx < nv) {
 #pragma unroll
                     for (int q = 0; q < 8; ++q) {
-                        const int64_t o = (w == 0) ? bx + (int64_t)cx * nv : cx + (int64_t)bx * nv;
-                        fo[mat[q] * vv + o] = (double)facc[q][r];
+                        fo[(w == 0) ? f_index(mat[q], bx, cx, nv, ftiled) : f_index(mat[q], cx, bx, nv, ftiled)] =
+                            (double)facc[q][r];
                     }
                 }
             }
@@ -1413,7 +1411,8 @@ __device__ __forceinline__ void pair_epilogue(const JobDev &jd, const Task &tk,
 template <int S, bool TENGY, bool WRITEF>
 __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
                                           int bi, int ci, int w, int lane,
-                                          double *__restrict__ part4, double *__restrict__ fo)
+                                          double *__restrict__ part4, double *__restrict__ fo,
+                                          bool ftiled)
 {
     const bool diag = (bi == ci);
     __shared__ double lds_x[2 * 8 * 2 * 64];
@@ -1429,7 +1428,8 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
             fp32_wave<0>(f32, pan32, jd.g, pp, bi, ci, lane);
         else if (!diag)
             fp32_wave<1>(f32, pan32, jd.g, pp, bi, ci, lane);
-        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v);
+        pair_epilogue<float, TENGY, WRITEF>(jd, tk, f32, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v,
+                                            ftiled);
     } else {
         dbl4 facc[8];
 #pragma unroll
@@ -1454,7 +1454,8 @@ __device__ __forceinline__ void pair_item(const JobDev &jd, const Task &tk, cons
                 split_wave<NP, 1, BF, T8>(facc, rd, rt, jd.scales, jd.g, sg, pp, bi, ci, lane, lds_x + 1024,
                                           lds_s[1]);
         }
-        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v);
+        pair_epilogue<double, TENGY, WRITEF>(jd, tk, facc, bi, ci, w, lane, part4, fo, lds_x, vreg, lds_v,
+                                             ftiled);
     }
 }
 
@@ -1485,7 +1486,7 @@ __device__ __forceinline__ void pair_items(const JobDev *__restrict__ jobs,
         if (PERSIST) asm volatile("" : "+v"(lane_i));
         pair_item<S, TENGY, WRITEF>(jd, tk, panel_pairs(tk), uniform(pr.x), uniform(pr.y), w, lane_i,
                                     sg.partial + ((int64_t)t * jd.g.npairs + p) * 4,
-                                     WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
+                                    WRITEF ? sg.fout + (int64_t)t * fstride : nullptr, sg.ftiled != 0);
     }
 }
 
@@ -1691,7 +1692,7 @@ template <int S, bool TENGY, bool WRITEF>
 __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
                                              const RowPair &rp, int w, int lane,
                                              double *__restrict__ part0, double *__restrict__ part1,
-                                             double *__restrict__ fo)
+                                             double *__restrict__ fo, bool ftiled)
 {
     constexpr bool T8 = (S == 3 + kSlicesOccDigits);
     constexpr int NP = T8 ? 3 : S;
@@ -1930,7 +1931,6 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
         // body order f1n,f1t,f2n,f2t,f3n,f3t,f4n,f4t: slot s of group gi is matrix
         // 2s + gi (mirrored: 2 kMirSlot[s] + gi at (c,b)); on a diagonal block the
         // direct waves write every element
-        const int64_t vv = (int64_t)nv * nv;
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
             const int bm = m ? r1 : r0;
@@ -1939,11 +1939,12 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
             for (int r = 0; r < 4; ++r) {
                 const int bx = bm * 16 + row0 + r;
                 if (bx < nv && cx < nv) {
-                    const int64_t o = mir ? cx + (int64_t)bx * nv : bx + (int64_t)cx * nv;
+
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         const int sm = mir ? kMirSlot[s] : s;
-                        fo[(2 * sm + gi) * vv + o] = fw[m][s][r];
+                        fo[mir ? f_index(2 * sm + gi, cx, bx, nv, ftiled) : f_index(2 * sm + gi, bx, cx, nv, ftiled)] =
+                            fw[m][s][r];
                     }
                 }
             }
@@ -2036,7 +2037,7 @@ __device__ __forceinline__ void rowpair_items(const JobDev *__restrict__ jobs,
         double *pbase = sg.partial + (int64_t)t * jd.g.npairs * 4;
         rowpair_item<S, TENGY, WRITEF>(jd, tk, panel_pairs(tk), rp, w, lane_i, pbase + (int64_t)rp.p0 * 4,
                                        pbase + (int64_t)(rp.p1 >= 0 ? rp.p1 : rp.p0) * 4,
-                                       WRITEF ? sg.fout + (int64_t)t * fstride : nullptr);
+                                       WRITEF ? sg.fout + (int64_t)t * fstride : nullptr, sg.ftiled != 0);
     }
 }
 
@@ -2236,7 +2237,7 @@ int tengy_npairs(const Geom &g)
 // the job's f workspace, padded vector pool and task records.  Both forms run the
 // same arithmetic, so their results are bitwise the same.
 template <bool DIRECT>
-__global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int64_t fstride,
+__global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int64_t fstride, int tiled,
                                                Geom g, const Task *__restrict__ tasks,
                                                int ntasks, const double *__restrict__ ev,
                                                const double *__restrict__ vec6,
@@ -2318,14 +2319,20 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     for (int q = 0; q < 2; ++q) {
         const int bx = BI * 32 + r0, cx = CI * 32 + ty + 16 * q;
         okd[q] = bx < nv && cx < nv;
-        od[q] = okd[q] ? (uint32_t)(bx + cx * nv) * 8 : 0;
+        od[q] = okd[q] ? (uint32_t)f_index(0, bx, cx, nv, !DIRECT && tiled) * 8 : 0;
         const int cc = CI * 32 + r0, bb = BI * 32 + ty + 16 * q;
         okm[q] = cc < nv && bb < nv;
-        om[q] = okm[q] ? (uint32_t)(cc + bb * nv) * 8 : 0;
+        om[q] = okm[q] ? (uint32_t)f_index(0, cc, bb, nv, !DIRECT && tiled) * 8 : 0;
     }
+    // bytes from one matrix to the next: nv^2 doubles column-major, one tile in
+    // the tiled layout (f_index)
+    const bool tl = !DIRECT && tiled;
+    const uint32_t mstride = tl ? (uint32_t)((nv - 32 * BI < 32 ? nv - 32 * BI : 32) *
+                                             (nv - 32 * CI < 32 ? nv - 32 * CI : 32) * 8)
+                                : (uint32_t)(vv * 8);
     auto load_pair = [&](int m, uint32_t off, bool two) -> dbl2 {
         const __amdgpu_buffer_rsrc_t r = DIRECT ? rfm[m] : rf;
-        const uint32_t moff = DIRECT ? 0u : (uint32_t)(m * vv * 8);
+        const uint32_t moff = DIRECT ? 0u : (uint32_t)m * mstride;
         if (two) return buffer_load_f64x2(r, off, moff);
         return dbl2{buffer_load_f64(r, off, moff), 0.0};
     };
@@ -2399,15 +2406,15 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     }
 }
 
-hipError_t launch_tengy(const double *f, int64_t fstride, const Geom &g, const Task *tasks,
+hipError_t launch_tengy(const double *f, int64_t fstride, bool tiled, const Geom &g, const Task *tasks,
                         int ntasks, const double *ev, const double *vec6, double *partial,
                         hipStream_t st)
 {
     const int64_t total = (int64_t)ntasks * tengy_npairs(g);
     if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
     if ((int64_t)g.nvir * g.nvir * 64 >= (int64_t(1) << 32)) return hipErrorInvalidValue;  // 32-bit offsets
-    hipLaunchKernelGGL(k_tengy<false>, dim3((unsigned)total), dim3(256), 0, st, f, fstride, g, tasks,
-                       ntasks, ev, vec6, partial, TengyDirect{});
+    hipLaunchKernelGGL(k_tengy<false>, dim3((unsigned)total), dim3(256), 0, st, f, fstride, tiled ? 1 : 0,
+                       g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
     return hipGetLastError();
 }
 
@@ -2429,7 +2436,7 @@ hipError_t launch_tengy_direct(const TengyDirect &in, const Geom &g, double *scr
 {
     const int np = tengy_npairs(g);
     if (np <= 0 || np * 4 > kNoAllocPartials) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_tengy<true>, dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, g,
+    hipLaunchKernelGGL(k_tengy<true>, dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0, g,
                        nullptr, 1, nullptr, nullptr, scratch_ptr, in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -318,3 +318,39 @@ def test_tune_cache_server_picks(cc, tmp_path):
         o = cc.ServerOpts(0, items, cc.MODES["fp64"], 0, variant, 0)
         assert L.ccsdt_server_tune_store(ctypes.byref(o), 1.0) == -1, (variant, items)
     assert cc.tune_cache_size() == 0
+
+
+def test_f_index_tiled_layout(cc):
+    """The tiled f layout of the separate energy pass holds every element of a
+    triple's eight nv x nv matrices exactly once in 8 nv^2 places (no padding);
+    each 32x32 tile pair (the tile at block row bi, block column ci <= bi and its
+    mirror at (ci, bi), of all eight matrices; the last block row and column
+    narrower) is one contiguous run, its tiles column-major; and the
+    column-major layout is the reference's m nv^2 + b + c nv."""
+    L = cc.lib()
+    for nv in (1, 5, 31, 32, 33, 64, 65, 300):
+        idx = np.array([[[L.ccsdt_debug_f_index(m, b, c, nv, 1) for c in range(nv)] for b in range(nv)]
+                        for m in range(8)])
+        assert np.array_equal(np.sort(idx.ravel()), np.arange(8 * nv * nv)), nv
+        cm = np.array([[[L.ccsdt_debug_f_index(m, b, c, nv, 0) for c in range(nv)] for b in range(nv)]
+                       for m in range(8)])
+        want = (np.arange(8)[:, None, None] * nv * nv + np.arange(nv)[None, :, None]
+                + nv * np.arange(nv)[None, None, :])
+        assert np.array_equal(cm, want), nv
+        start = 0
+        for bi in range(0, nv, 32):
+            for ci in range(0, bi + 1, 32):
+                D = idx[:, bi:bi + 32, ci:ci + 32]
+                M = idx[:, ci:ci + 32, bi:bi + 32]
+                cells = np.concatenate([D.ravel(), M.ravel()]) if bi != ci else D.ravel()
+                # the pair's run follows the previous pair's
+                assert cells.min() == start and cells.max() == start + cells.size - 1, (nv, bi, ci)
+                start += cells.size
+                for T in ([D] + ([M] if bi != ci else [])):
+                    for m in range(8):
+                        t = T[m]
+                        rows, cols = t.shape
+                        assert np.array_equal(t, t[0, 0] + np.arange(rows)[:, None]
+                                              + rows * np.arange(cols)[None, :])
+        assert start == 8 * nv * nv
+    assert L.ccsdt_debug_f_index(0, 300, 0, 300, 1) == -1 and L.ccsdt_debug_f_index(8, 0, 0, 300, 0) == -1
