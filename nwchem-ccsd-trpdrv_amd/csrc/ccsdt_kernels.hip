@@ -1609,29 +1609,65 @@ __device__ __forceinline__ void rowpair_nvir(int4v (&iacc)[2][4][S], const __amd
             iacc[m][3][lv] = PlaneMath<false>::mfma(A_[1][sa], B_[1][sb], z_ ? z : iacc[m][3][lv]); \
         }                                                                                        \
     }
-    if constexpr (S > 3) {
-        // four planes: a second B buffer would spill (255 VGPRs); one buffer,
-        // each step's planes loaded at its start, B and A interleaved per plane
-        constexpr int NM = TWO ? 2 : 1;
-#define RP_LOADS(kstep)                                                                          \
-        _Pragma("unroll") for (int q = 0; q < S; ++q) {                                         \
-            const uint32_t ko_ = (uint32_t)(kstep) * 1024 + q * plane;                           \
-            bx[0][q] = buffer_load_i8x16(rd, voff8, ob[0] + ko_);                                \
-            bx[1][q] = buffer_load_i8x16(rd, voff8, ob[1] + ko_);                                \
-            _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                    \
-                (m ? a1 : a0)[0][q] = buffer_load_i8x16(rd, voff8, oa[m][0] + ko_);              \
-                (m ? a1 : a0)[1][q] = buffer_load_i8x16(rd, voff8, oa[m][1] + ko_);              \
+    if constexpr (S > 3 && TWO) {
+        // four planes: a second B buffer would spill (255 VGPRs).  One buffer per
+        // operand, reloaded in quarter steps: a step runs block 0 x B panel 0,
+        // block 0 x panel 1, block 1 x panel 0, block 1 x panel 1; block 0's A
+        // planes reload after its two quarters, B panel 0 after the third, block
+        // 1's A planes and B panel 1 after the fourth, so every reload has at
+        // least one quarter (20 MFMAs) to arrive before its first use
+#define RP_LOAD_BP(p, kstep)                                                                     \
+        _Pragma("unroll") for (int q = 0; q < S; ++q)                                           \
+            bx[p][q] = buffer_load_i8x16(rd, voff8, ob[p] + (uint32_t)(kstep) * 1024 + q * plane);
+#define RP_MFMA_Q(m, A_, p, first)                                                               \
+        _Pragma("unroll") for (int sa = 0; sa < S; ++sa) {                                      \
+            _Pragma("unroll") for (int sb = 0; sb < S - sa; ++sb) {                             \
+                const int lv = sa + sb;                                                          \
+                const bool z_ = (first) && sa == 0;                                              \
+                iacc[m][2 * p][lv] = PlaneMath<false>::mfma(A_[0][sa], bx[p][sb], z_ ? z : iacc[m][2 * p][lv]); \
+                iacc[m][2 * p + 1][lv] =                                                         \
+                    PlaneMath<false>::mfma(A_[1][sa], bx[p][sb], z_ ? z : iacc[m][2 * p + 1][lv]); \
             }                                                                                    \
         }
-        RP_LOADS(0)
-        RP_MFMA(0, a0, bx, true)
-        if constexpr (TWO) RP_MFMA(1, a1, bx, true)
-        for (int ks = 1; ks < nkv; ++ks) {
-            RP_LOADS(ks)
-            RP_MFMA(0, a0, bx, false)
-            if constexpr (TWO) RP_MFMA(1, a1, bx, false)
+        RP_LOAD_B(bx, 0)
+        RP_LOAD_A(a0, 0, 0)
+        RP_LOAD_A(a1, 1, 0)
+        const bool more0 = nkv > 1;
+        RP_MFMA_Q(0, a0, 0, true)
+        RP_MFMA_Q(0, a0, 1, true)
+        if (more0) RP_LOAD_A(a0, 0, 1)
+        RP_MFMA_Q(1, a1, 0, true)
+        if (more0) RP_LOAD_BP(0, 1)
+        RP_MFMA_Q(1, a1, 1, true)
+        if (more0) {
+            RP_LOAD_A(a1, 1, 1)
+            RP_LOAD_BP(1, 1)
         }
-#undef RP_LOADS
+        for (int ks = 1; ks < nkv; ++ks) {
+            const bool more = ks + 1 < nkv;
+            RP_MFMA_Q(0, a0, 0, false)
+            RP_MFMA_Q(0, a0, 1, false)
+            if (more) RP_LOAD_A(a0, 0, ks + 1)
+            RP_MFMA_Q(1, a1, 0, false)
+            if (more) RP_LOAD_BP(0, ks + 1)
+            RP_MFMA_Q(1, a1, 1, false)
+            if (more) {
+                RP_LOAD_A(a1, 1, ks + 1)
+                RP_LOAD_BP(1, ks + 1)
+            }
+        }
+#undef RP_LOAD_BP
+#undef RP_MFMA_Q
+    } else if constexpr (S > 3) {
+        // four planes, one block: each step's planes loaded at its start
+        RP_LOAD_B(bx, 0)
+        RP_LOAD_A(a0, 0, 0)
+        RP_MFMA(0, a0, bx, true)
+        for (int ks = 1; ks < nkv; ++ks) {
+            RP_LOAD_B(bx, ks)
+            RP_LOAD_A(a0, 0, ks)
+            RP_MFMA(0, a0, bx, false)
+        }
     } else if constexpr (TWO) {
         RP_LOAD_B(bx, 0)
         RP_LOAD_A(a0, 0, 0)
