@@ -65,12 +65,56 @@ __device__ __forceinline__ void tengy_point(T n1, T n2, T n3, T n4, T t1, T t2, 
            + d * u2 * y2 * (t1 + t4 + nr1 - (t2 + t3 + nr2) * 2);
 }
 
+// A 32-bit lane move through DPP (data-parallel primitives: the operand is read
+// from another lane of the same row of 16 as part of the VALU instruction).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+
+typedef int int2v __attribute__((ext_vector_type(2)));
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const int2v h = __builtin_bit_cast(int2v, v);
+    const int2v r = {dpp_i32<CTRL>(h.x), dpp_i32<CTRL>(h.y)};
+    return __builtin_bit_cast(double, r);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __builtin_bit_cast(float, dpp_i32<CTRL>(__builtin_bit_cast(int, v)));
+}
+
+__device__ __forceinline__ double lane_read(double v, int l)
+{
+    const int2v h = __builtin_bit_cast(int2v, v);
+    const int2v r = {__builtin_amdgcn_readlane(h.x, l), __builtin_amdgcn_readlane(h.y, l)};
+    return __builtin_bit_cast(double, r);
+}
+
+__device__ __forceinline__ float lane_read(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// Sum over the wave, returned in every lane; every lane must be active.  Within
+// each row of 16 lanes DPP moves add the lanes of each quad (quad_perm [1,0,3,2]
+// and [2,3,0,1]) and then the row's quads (row_ror 4 and 8); the four row sums
+// are read from lanes 0, 16, 32 and 48.  No LDS traffic (a shuffle is a
+// ds_bpermute per 32-bit half and step, each waited for), and a fixed order of
+// the additions, so the sum is the same on every run.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x124>(v);
+    v += dpp_mov<0x128>(v);
+    return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
 }
 
 // Raw buffer addressing: a 128-bit resource in scalar registers, a wave-uniform
@@ -2078,13 +2122,11 @@ __device__ __forceinline__ void rowpair_items(const JobDev *__restrict__ jobs,
 }
 
 // The Ozaki INT8 split form on the row-pair tile: S = 3 or 4, or 3 +
-// kSlicesOccDigits; three waves per SIMD for S = 3 (at most 168 VGPRs), two for
-// S = 4 (its 128 accumulator VGPRs and operands need more).
-#ifndef RP_WAVES_S3
-#define RP_WAVES_S3 2
-#endif
+// kSlicesOccDigits; two waves per SIMD (216 VGPRs at S = 3, 236 at S = 4; the
+// 74.5 KB of LDS also fits two workgroups per CU).  Forced to three waves per
+// SIMD (168 VGPRs) the first version of this tile ran 11 % slower.
 template <int S, bool TENGY, bool WRITEF, bool PERSIST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 4 ? 2 : RP_WAVES_S3)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_split_rows(const JobDev *__restrict__ jobs, const Segment *__restrict__ segs, WorkHdr *hdr,
                   int64_t fstride)
 {
@@ -2480,8 +2522,8 @@ hipError_t launch_tengy_direct(const TengyDirect &in, const Geom &g, double *scr
 }
 
 // ------------------------------------------------------------------ reductions
-// Per task: sum of the tile partials in a fixed order (lane-strided, then a
-// butterfly), so the result does not depend on scheduling.
+// Per task: sum of the tile partials in a fixed order (lane-strided, then
+// wave_sum), so the result does not depend on scheduling.
 __global__ __launch_bounds__(64) void k_reduce_terms(const double *__restrict__ partial,
                                                      int npairs, double *__restrict__ terms)
 {
