@@ -11,8 +11,10 @@ two-wave tile's to rounding: within 1e-13 of the largest term of the triple
 small term that cancels), far under the split form's 1e-8 bound.  Checked
 on ragged shapes (odd block counts leave an item with one block; nvir = 1 and
 17 leave a padded row block), with and without the occupied digit planes
-(folded for nocc <= 32 and not), at 3 and 4 slices, and on every schedule, where
-each tile's own results are bitwise the same whatever the schedule.
+(folded for nocc <= 32 and not), at 3 and 4 slices, on every schedule, where
+each tile's own results are bitwise the same whatever the schedule, and at the
+largest K the tile takes (nvir = 768), with operands that push its INT32 level
+sums near their bound.
 """
 import numpy as np
 import pytest
@@ -92,6 +94,40 @@ def test_f_matrices_bitwise(cc, slices, occ):
             ref = job.run(trip, variant=0, **kw)["f"]
             got = job.run(trip, variant=ROWS, **kw)["f"]
             assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("slices,occ", FORMS)
+def test_largest_k(cc, slices, occ):
+    """nvir = 768, the largest K the tile takes: twelve 64-wide K steps, where the
+    exact INT32 pairing of adjacent digit levels, (acc_L << 7) + acc_(L+1), still
+    holds (its bound, (20352 * 128 + 24448) K < 2^31, allows K <= 816).  Random
+    operands: f bitwise the two-wave tile's and terms close to it.  Constant
+    operands put the top digit of every entry near 127, so the level-0 sums reach
+    about 127^2 * 768 and their pairing about 1.6e9, near the 2^31 it can hold; an
+    overflow there would be an O(1) error in f, against a split error of ~1e-6
+    (three slices) or ~1e-8 (four) of the largest |f|, measured against the FP64
+    form."""
+    nocc, nvir, kchunk = 3, 768, 1
+    arrays = cc.make_inputs(nocc, nvir, kchunk, 31)
+    trips = cc.triple_list(nocc, cc.blocks_uniform(nocc, kchunk))[:2]
+    kw = dict(mode="ozaki_int8", slices=slices, occ_digits=occ, fout=True)
+    const = dict(arrays)
+    for name in ("tij", "tkj", "tia", "tka", "xia", "xka") + tuple(cc.JK_NAMES):
+        const[name] = np.full_like(arrays[name], 0.062)
+    tol = 1e-5 if slices == 3 else 1e-7
+    for t in trips:  # f output takes one triple per run
+        trip = t[None]
+        with cc.Job(arrays, nocc, nvir, kchunk) as job:
+            ref = job.run(trip, variant=0, terms=True, **kw)
+            got = job.run(trip, variant=ROWS, terms=True, **kw)
+        assert np.array_equal(got["f"], ref["f"])
+        close(got, ref)
+        with cc.Job(const, nocc, nvir, kchunk) as job:
+            f64 = job.run(trip, fout=True)["f"]
+            got = job.run(trip, variant=ROWS, **kw)["f"]
+        scale = np.max(np.abs(f64))
+        assert scale > 0.0
+        assert np.max(np.abs(got - f64)) <= tol * scale, np.max(np.abs(got - f64)) / scale
 
 
 def test_production_list_bitwise(cc):
