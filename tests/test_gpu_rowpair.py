@@ -177,6 +177,30 @@ def test_serving_on_row_pair_tile(cc, slices, occ):
         assert (a["emp4"], a["emp5"]) == (m["emp4"], m["emp5"])
 
 
+def test_serving_per_job_forms_on_row_pair_tile(cc):
+    """Per-job forms on the row-pair tile: a job of at least 1e8 energy summands
+    runs at three slices with occupied digit planes, the small ones at four, in
+    the same ticks (one launch per form); each job bitwise its plain run on the
+    tile in its form."""
+    big = dict(nocc=22, nvir=230, kchunk=22)
+    big["triples"] = cc.triple_list(22, cc.blocks_uniform(22, 22))
+    big["arrays"] = cc.make_inputs(22, 230, 22, 5151)
+    assert len(big["triples"]) * 230 ** 2 >= 1e8
+    jobs = [big] + cc.serving_jobs(4, 4242, nocc_range=(3, 9), nvir_range=(20, 70))
+    with cc.Server(mode="ozaki_int8", slices=cc.SLICES_PER_JOB, variant=ROWS, items_per_tick=4096) as srv:
+        ids = [srv.submit(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"], jb["triples"])
+               for jb in jobs]
+        srv.drain()
+        mixed = [srv.result(i) for i in ids]
+    assert [(m["slices"], m["occ_digits"]) for m in mixed] == [(3, 1)] + [(4, 0)] * 4
+    for jb, m in zip(jobs, mixed):
+        assert m["status"] == 0
+        with cc.Job(jb["arrays"], jb["nocc"], jb["nvir"], jb["kchunk"]) as job:
+            a = job.run(jb["triples"], mode="ozaki_int8", slices=m["slices"], occ_digits=m["occ_digits"],
+                        variant=ROWS)
+        assert (a["emp4"], a["emp5"]) == (m["emp4"], m["emp5"])
+
+
 def test_tuner_lists_the_tile(cc):
     arrays, blocks, trip, kchunk = cc.config_case(0)
     with cc.Job(arrays, 8, 40, kchunk) as job:
