@@ -2298,8 +2298,14 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
 // come through LDS with one 16-byte load per thread: on gfx950 a wave-wide load
 // costs the texture-data path the same ~16 cycles at 8 or 16 bytes per lane,
 // and that path, not HBM, bounded the 8-byte form.  All 32 tile loads of a
-// thread are issued before the first transpose; the transposes alternate
-// between two LDS tiles (one barrier each).
+// thread are issued before the first transpose; the eight mirrored tiles are
+// transposed through LDS together, behind one barrier (66 KiB, two workgroups
+// per CU as the registers allow).  EVEN (nvir even): every row pair a
+// thread loads lies inside the matrix, so each load is one 16-byte load.  With
+// nvir odd a pair may end at the last row, and each load decides at run time
+// between 16 and 8 bytes -- which the compiler issues as both loads and a
+// select, 64 load instructions per thread, more than the wait counter holds, so
+// the first LDS write then waits for all of them.
 __host__ __device__ inline int tengy_tiles(int nv) { return (nv + 31) / 32; }
 
 int tengy_npairs(const Geom &g)
@@ -2314,14 +2320,14 @@ int tengy_npairs(const Geom &g)
 // as 0, as the padding of the job's own arrays does), eaijk by value.  Otherwise
 // the job's f workspace, padded vector pool and task records.  Both forms run the
 // same arithmetic, so their results are bitwise the same.
-template <bool DIRECT>
+template <bool DIRECT, bool EVEN>
 __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int64_t fstride, int tiled,
                                                Geom g, const Task *__restrict__ tasks,
                                                int ntasks, const double *__restrict__ ev,
                                                const double *__restrict__ vec6,
                                                double *__restrict__ partial, TengyDirect dir)
 {
-    __shared__ double tile[2][32][33];
+    __shared__ double tile[8][32][33];
     __shared__ __attribute__((aligned(16))) double vst[7 * 2 * 32];
     __shared__ double red[4][4];
     const int nv = g.nvir, nvp = g.nvp;
@@ -2411,7 +2417,7 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     auto load_pair = [&](int m, uint32_t off, bool two) -> dbl2 {
         const __amdgpu_buffer_rsrc_t r = DIRECT ? rfm[m] : rf;
         const uint32_t moff = DIRECT ? 0u : (uint32_t)m * mstride;
-        if (two) return buffer_load_f64x2(r, off, moff);
+        if (EVEN || two) return buffer_load_f64x2(r, off, moff);
         return dbl2{buffer_load_f64(r, off, moff), 0.0};
     };
     // the mirrored tiles first: their transposes can start while the direct
@@ -2429,20 +2435,21 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
             dv[m][q] = load_pair(m, od[q], okd[q] ? two_d : nv > 1);
     if (2 * (int)threadIdx.x < 7 * 64) *reinterpret_cast<dbl2 *>(vst + 2 * threadIdx.x) = vreg;
     // transpose: thread (tx, ty) loaded f(c = CI*32 + 2tx + h, b = BI*32 + ty + 16q)
-    // and needs f(c = CI*32 + ty + 16q, b = BI*32 + 2tx + h)
+    // and needs f(c = CI*32 + ty + 16q, b = BI*32 + 2tx + h); all eight tiles go
+    // through LDS at once, each written as its loads land, behind one barrier
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        double(*T)[33] = tile[m & 1];
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) T[ty + 16 * q][r0 + h] = okm[q] ? mv[m][q][h] : 0.0;
-        __syncthreads();
+    for (int m = 0; m < 8; ++m)
 #pragma unroll
         for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) mv[m][q][h] = T[r0 + h][ty + 16 * q];
-    }
+            for (int h = 0; h < 2; ++h) tile[m][ty + 16 * q][r0 + h] = okm[q] ? mv[m][q][h] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) mv[m][q][h] = tile[m][r0 + h][ty + 16 * q];
     auto vb = [&](int q, int i) { return vst[q * 64 + i]; };       // B tile, row i
     auto vc = [&](int q, int i) { return vst[q * 64 + 32 + i]; };  // C tile, column i
     Sum4 s = {0.0, 0.0, 0.0, 0.0};
@@ -2491,8 +2498,12 @@ hipError_t launch_tengy(const double *f, int64_t fstride, bool tiled, const Geom
     const int64_t total = (int64_t)ntasks * tengy_npairs(g);
     if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
     if ((int64_t)g.nvir * g.nvir * 64 >= (int64_t(1) << 32)) return hipErrorInvalidValue;  // 32-bit offsets
-    hipLaunchKernelGGL(k_tengy<false>, dim3((unsigned)total), dim3(256), 0, st, f, fstride, tiled ? 1 : 0,
-                       g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
+    if (g.nvir % 2 == 0)
+        hipLaunchKernelGGL((k_tengy<false, true>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
+                           tiled ? 1 : 0, g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
+    else
+        hipLaunchKernelGGL((k_tengy<false, false>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
+                           tiled ? 1 : 0, g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
     return hipGetLastError();
 }
 
@@ -2514,8 +2525,12 @@ hipError_t launch_tengy_direct(const TengyDirect &in, const Geom &g, double *scr
 {
     const int np = tengy_npairs(g);
     if (np <= 0 || np * 4 > kNoAllocPartials) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_tengy<true>, dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0, g,
-                       nullptr, 1, nullptr, nullptr, scratch_ptr, in);
+    if (g.nvir % 2 == 0)
+        hipLaunchKernelGGL((k_tengy<true, true>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0,
+                           g, nullptr, 1, nullptr, nullptr, scratch_ptr, in);
+    else
+        hipLaunchKernelGGL((k_tengy<true, false>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0,
+                           0, g, nullptr, 1, nullptr, nullptr, scratch_ptr, in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_reduce_terms(scratch_ptr, np, 1, out4, st);
