@@ -2300,12 +2300,14 @@ int fused_blocks_per_cu(bool split, int slices, int variant)
 // and that path, not HBM, bounded the 8-byte form.  All 32 tile loads of a
 // thread are issued before the first transpose; the eight mirrored tiles are
 // transposed through LDS together, behind one barrier (66 KiB, two workgroups
-// per CU as the registers allow).  EVEN (nvir even): every row pair a
-// thread loads lies inside the matrix, so each load is one 16-byte load.  With
-// nvir odd a pair may end at the last row, and each load decides at run time
-// between 16 and 8 bytes -- which the compiler issues as both loads and a
-// select, 64 load instructions per thread, more than the wait counter holds, so
-// the first LDS write then waits for all of them.
+// per CU as the registers allow).  LOADS, chosen at launch from nvir: 0 (even):
+// every row pair a thread loads lies inside the matrix, one 16-byte load each.
+// 1 (odd, > 1): a pair may start at the last row b; it is loaded one row earlier
+// (rows b-1, b; element b sits at a byte offset of at least 8, as b >= 1) and its
+// upper element kept, again one 16-byte load.  2 (nvir = 1): single 8-byte
+// loads.  (A per-lane choice between a 16- and an 8-byte load is issued as both
+// loads and a select: 64 load instructions per thread, more than the wait
+// counter holds, so the first LDS write would wait for all of them.)
 __host__ __device__ inline int tengy_tiles(int nv) { return (nv + 31) / 32; }
 
 int tengy_npairs(const Geom &g)
@@ -2320,7 +2322,7 @@ int tengy_npairs(const Geom &g)
 // as 0, as the padding of the job's own arrays does), eaijk by value.  Otherwise
 // the job's f workspace, padded vector pool and task records.  Both forms run the
 // same arithmetic, so their results are bitwise the same.
-template <bool DIRECT, bool EVEN>
+template <bool DIRECT, int LOADS>
 __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int64_t fstride, int tiled,
                                                Geom g, const Task *__restrict__ tasks,
                                                int ntasks, const double *__restrict__ ev,
@@ -2386,7 +2388,7 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     // mirrored m[m][q] = f_m(cc, cc+1; bb), cc = CI*32 + 2tx, bb = BI*32 + ty + 16q,
     // transposed below.  Byte offsets inside one matrix; the matrix is selected by
     // the uniform offset m*vv*8.  Out-of-range points load from offset 0 and are
-    // masked; a pair whose second row is past nv (odd nv) loads one double.
+    // masked; a pair whose second row is past nv (odd nv) is read one row early.
     // one resource over the eight matrices (matrix m at byte offset m vv 8), or
     // one per matrix for DIRECT (each at offset 0)
     __amdgpu_buffer_rsrc_t rf = buffer_rsrc(ft, (uint32_t)(8 * vv * 8));
@@ -2417,8 +2419,14 @@ __global__ __launch_bounds__(256) void k_tengy(const double *__restrict__ f, int
     auto load_pair = [&](int m, uint32_t off, bool two) -> dbl2 {
         const __amdgpu_buffer_rsrc_t r = DIRECT ? rfm[m] : rf;
         const uint32_t moff = DIRECT ? 0u : (uint32_t)m * mstride;
-        if (EVEN || two) return buffer_load_f64x2(r, off, moff);
-        return dbl2{buffer_load_f64(r, off, moff), 0.0};
+        if constexpr (LOADS == 0) {
+            return buffer_load_f64x2(r, off, moff);
+        } else if constexpr (LOADS == 1) {
+            const dbl2 v = buffer_load_f64x2(r, two ? off : off - 8, moff);
+            return two ? v : dbl2{v.y, 0.0};
+        } else {
+            return dbl2{buffer_load_f64(r, off, moff), 0.0};
+        }
     };
     // the mirrored tiles first: their transposes can start while the direct
     // tiles are still arriving
@@ -2499,10 +2507,13 @@ hipError_t launch_tengy(const double *f, int64_t fstride, bool tiled, const Geom
     if (total <= 0 || total > 0x7fffffff) return hipErrorInvalidValue;
     if ((int64_t)g.nvir * g.nvir * 64 >= (int64_t(1) << 32)) return hipErrorInvalidValue;  // 32-bit offsets
     if (g.nvir % 2 == 0)
-        hipLaunchKernelGGL((k_tengy<false, true>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
+        hipLaunchKernelGGL((k_tengy<false, 0>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
+                           tiled ? 1 : 0, g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
+    else if (g.nvir > 1)
+        hipLaunchKernelGGL((k_tengy<false, 1>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
                            tiled ? 1 : 0, g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
     else
-        hipLaunchKernelGGL((k_tengy<false, false>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
+        hipLaunchKernelGGL((k_tengy<false, 2>), dim3((unsigned)total), dim3(256), 0, st, f, fstride,
                            tiled ? 1 : 0, g, tasks, ntasks, ev, vec6, partial, TengyDirect{});
     return hipGetLastError();
 }
@@ -2526,11 +2537,14 @@ hipError_t launch_tengy_direct(const TengyDirect &in, const Geom &g, double *scr
     const int np = tengy_npairs(g);
     if (np <= 0 || np * 4 > kNoAllocPartials) return hipErrorInvalidValue;
     if (g.nvir % 2 == 0)
-        hipLaunchKernelGGL((k_tengy<true, true>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0,
+        hipLaunchKernelGGL((k_tengy<true, 0>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0,
+                           g, nullptr, 1, nullptr, nullptr, scratch_ptr, in);
+    else if (g.nvir > 1)
+        hipLaunchKernelGGL((k_tengy<true, 1>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0,
                            g, nullptr, 1, nullptr, nullptr, scratch_ptr, in);
     else
-        hipLaunchKernelGGL((k_tengy<true, false>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0,
-                           0, g, nullptr, 1, nullptr, nullptr, scratch_ptr, in);
+        hipLaunchKernelGGL((k_tengy<true, 2>), dim3((unsigned)np), dim3(256), 0, st, nullptr, (int64_t)0, 0,
+                           g, nullptr, 1, nullptr, nullptr, scratch_ptr, in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_reduce_terms(scratch_ptr, np, 1, out4, st);

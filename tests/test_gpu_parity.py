@@ -88,6 +88,16 @@ def test_split_mode_matches_fused(cc, orc):
     assert rel_terms(got["terms"], ref["terms"], ref["abs4"]) < TOL
 
 
+@pytest.mark.parametrize("nocc,nvir,kchunk", [(3, 1, 1), (4, 33, 2), (5, 45, 3), (4, 47, 4)])
+def test_separate_pass_odd_nvir(cc, orc, nocc, nvir, kchunk):
+    """The separate energy pass (f written to HBM laid out by tile pair, then
+    k_tengy) at odd nvir, where a thread's row pair can start at the last row and
+    is read one row early: nvir = 1 (single loads), 33 (a last tile one row wide),
+    45 and 47 (odd last tiles), against the oracle per term."""
+    arrays, trip, ref, got = run_case(cc, orc, nocc, nvir, kchunk, 11 + nvir, mode="fp64_split")
+    assert rel_terms(got["terms"], ref["terms"], ref["abs4"]) < TOL
+
+
 def test_deterministic(cc):
     arrays, blocks, trip, kchunk = cc.config_case(0)
     with cc.Job(arrays, 8, 40, kchunk) as job:
