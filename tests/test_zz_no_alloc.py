@@ -88,7 +88,9 @@ def test_matches_dropin_two_calls(cc, hip):
     assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
     dev = Dev(hip)
     try:
-        for nocc, nvir, seed in ((30, 300, 41), (5, 45, 42)):  # second call smaller
+        # each call smaller than the one before; nvir 45 and 33 (a last tile one row
+        # wide) take the odd-nvir load form, nvir = 1 single loads
+        for nocc, nvir, seed in ((30, 300, 41), (5, 45, 42), (4, 33, 44), (2, 1, 45)):
             f, vec, eorb, eaijk = case(cc, nocc, nvir, seed)
             out = noalloc(cc, dev, f, vec, eorb, eaijk, nocc, nvir, stream)
             assert hip.hipStreamSynchronize(stream) == 0
