@@ -1768,7 +1768,7 @@ __device__ __forceinline__ void rowpair_nvir(int4v (&iacc)[2][4][S], const __amd
 
 // One row-pair item on four waves (see above).  part0/part1: the partials of
 // block pairs (r0, ci) and (r0+1, ci) (part1 unused without the second block).
-template <int S, bool TENGY, bool WRITEF>
+template <int S, bool TENGY, bool WRITEF, bool REGS>
 __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, const PanelPairs &pp,
                                              const RowPair &rp, int w, int lane,
                                              double *__restrict__ part0, double *__restrict__ part1,
@@ -1909,20 +1909,35 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
     };
     recombine(iacc, sst, false);
     if constexpr (T8) {
-        // the K = nocc half as digit planes with scales of their own (sst + 96); the
-        // K = nvir products wait in this wave's LDS slots meanwhile (xw is not
+        // the K = nocc half as digit planes with scales of their own (sst + 96).
+        // Where registers do not allow the K = nvir products to stay in fw over
+        // its products, they wait in this wave's LDS slots (xw is not
         // restrict-qualified, and the fence keeps the compiler from forwarding the
         // stores to the reloads, which would hold them in registers after all)
+        auto stash = [&]() {
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+            for (int m = 0; m < 2; ++m)
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) xw[((m * 4 + c) * 4 + r) * 64 + lane] = fw[m][c][r];
-        asm volatile("" ::: "memory");
+                    for (int r = 0; r < 4; ++r) xw[((m * 4 + c) * 4 + r) * 64 + lane] = fw[m][c][r];
+            asm volatile("" ::: "memory");
+        };
+        auto unstash = [&]() {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) fw[m][c][r] = xw[((m * 4 + c) * 4 + r) * 64 + lane];
+        };
         const int4v z = {0, 0, 0, 0};
         if (sg.fold) {
-            // the folded planes fetched after the K = nvir loop; levels as split_wave's
+            // the folded planes fetched after the K = nvir loop; levels as
+            // split_wave's.  REGS: the K = nvir products stay in fw meanwhile
+            // (236 VGPRs; the persistent kernel would spill, so it stashes)
+            if constexpr (!REGS) stash();
             const int4v (&bx)[2][3] = obx;
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
@@ -1939,9 +1954,11 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
                 iacc[m][2][2] = PlaneMath<false>::mfma(ax[0][1], bx[1][0], iacc[m][2][2]);
                 iacc[m][3][2] = PlaneMath<false>::mfma(ax[1][1], bx[1][0], iacc[m][3][2]);
             }
+            if constexpr (!REGS) unstash();
         } else {
             // one digit per 64-wide K step (nocc > 32): B planes, then each block's
-            // A panels in turn
+            // A panels in turn, the K = nvir products stashed
+            stash();
 #pragma unroll
             for (int m = 0; m < 2; ++m)
 #pragma unroll
@@ -1973,14 +1990,8 @@ __device__ __forceinline__ void rowpair_item(const JobDev &jd, const Task &tk, c
                             }
                     }
             }
+            unstash();
         }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int m = 0; m < 2; ++m)
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) fw[m][c][r] = xw[((m * 4 + c) * 4 + r) * 64 + lane];
         recombine(iacc, sst + 96, true);
     } else {
         // FP64 tail (K = nocc) in the step-pair layout, A rows fed in the order
@@ -2115,7 +2126,7 @@ __device__ __forceinline__ void rowpair_items(const JobDev *__restrict__ jobs,
         int lane_i = lane;
         if (PERSIST) asm volatile("" : "+v"(lane_i));
         double *pbase = sg.partial + (int64_t)t * jd.g.npairs * 4;
-        rowpair_item<S, TENGY, WRITEF>(jd, tk, panel_pairs(tk), rp, w, lane_i, pbase + (int64_t)rp.p0 * 4,
+        rowpair_item<S, TENGY, WRITEF, !PERSIST>(jd, tk, panel_pairs(tk), rp, w, lane_i, pbase + (int64_t)rp.p0 * 4,
                                        pbase + (int64_t)(rp.p1 >= 0 ? rp.p1 : rp.p0) * 4,
                                        WRITEF ? sg.fout + (int64_t)t * fstride : nullptr, sg.ftiled != 0);
     }
