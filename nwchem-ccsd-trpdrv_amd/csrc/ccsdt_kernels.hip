@@ -2133,8 +2133,9 @@ __device__ __forceinline__ void rowpair_items(const JobDev *__restrict__ jobs,
 }
 
 // The Ozaki INT8 split form on the row-pair tile: S = 3 or 4, or 3 +
-// kSlicesOccDigits; two waves per SIMD (216 VGPRs at S = 3, 236 at S = 4; the
-// 74.5 KB of LDS also fits two workgroups per CU).  Forced to three waves per
+// kSlicesOccDigits; two waves per SIMD (212 VGPRs at S = 3, 236 at S = 4 and at
+// S = 3 with occupied digit planes; the 74.5 KB of LDS also fits two workgroups
+// per CU).  Forced to three waves per
 // SIMD (168 VGPRs) the first version of this tile ran 11 % slower.
 template <int S, bool TENGY, bool WRITEF, bool PERSIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
